@@ -1,0 +1,194 @@
+/*
+ * pwm_hip.h -- C ABI of libpwmhip.so, the MI355X (gfx950) SpMV and power-method
+ * library.  Every entry point returns an int status (PWM_OK = 0, negative on
+ * error) and pwm_last_error() returns the message of the last failure.
+ *
+ * The drop-in boundary is the reference's abstract operator
+ * pwm::SparseMatrix<T,int_type> (Matrix/SparseMatrix.hpp:13-68) and its
+ * factory pwm::selectType(method, threads) (Util/DriverUtil.hpp:38-84).  The
+ * C++ classes in include/pwm/SparseMatrixHip.hpp derive from a restatement of
+ * that interface and forward each virtual to the entry point named beside it
+ * below.  Plain pointers and sizes only: no torch or HIP types cross it.
+ *
+ * Index arrays are 32-bit.  The reference instantiates int_type as `int` and
+ * `unsigned int` (Tests/poisson.cpp:28,146); every index it stores is
+ * non-negative and below 2^31, so both have the bits this ABI reads.  Host
+ * vectors are double (the reference's data_t, Util/Constants.hpp:32).
+ *
+ * Threading: calls on one handle are not re-entrant (like the reference's
+ * objects, whose CRS_Merge carry buffers are mutable members,
+ * SOA_Implementations/CRS_Merge.hpp:27-30); different handles are
+ * independent.  Each handle owns one HIP stream at default priority.
+ */
+#ifndef PWM_HIP_H
+#define PWM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PWM_OK               0
+#define PWM_ERR_INVALID     -1  /* bad argument (sizes, indices, aliasing) */
+#define PWM_ERR_IO          -2  /* file cannot be read / is malformed */
+#define PWM_ERR_HIP         -3  /* HIP runtime error (no device, launch failure) */
+#define PWM_ERR_NOMEM       -4  /* device allocation failed */
+#define PWM_ERR_NOT_SQUARE  -5  /* power method on a non-square matrix */
+#define PWM_ERR_STATE       -6  /* no matrix loaded yet */
+#define PWM_ERR_BUDGET      -7  /* serving admission: HBM budget exhausted */
+#define PWM_ERR_UNSUPPORTED -8  /* option or format not available */
+
+/* ---- formats: the GPU classes and the reference classes they stand beside.
+ * The ids are the reference's method ids (Util/DriverUtil.hpp:45-74); the
+ * GPU factory pwm::selectType adds 100 to them (see SparseMatrixHip.hpp). */
+#define PWM_FORMAT_CRS        1   /* CRS vector kernel   ~ CRS / CRSOMP (ids 1, 2) */
+#define PWM_FORMAT_CRS_MERGE  8   /* merge-path CRS      ~ CRS_Merge (id 8) */
+#define PWM_FORMAT_CSB        9   /* compressed sparse blocks ~ CSB (id 9) */
+#define PWM_FORMAT_BLOCKCOH   10  /* row-distributed block CO-H ~ BlockCOH (id 10) */
+
+/* ---- device storage precision of the matrix values ---------------------
+ * F64: values and vectors FP64, FP64 accumulation (parity mode).
+ * F32: values and vectors FP32, FP32 accumulation per row.
+ * BF16: values BF16 (accepted only when every value is exact in BF16, e.g.
+ *       the Poisson stencil's 4/-1), vectors FP32, FP32 accumulation.
+ * Norms, Rayleigh quotients and convergence tests are FP64 in every mode. */
+#define PWM_VALUES_F64  0
+#define PWM_VALUES_F32  1
+#define PWM_VALUES_BF16 2
+
+typedef struct pwm_matrix pwm_matrix;
+
+typedef struct {
+    int64_t nor, noc, nnz;     /* SparseMatrix fields (Matrix/SparseMatrix.hpp:17-23) */
+    int32_t format, value_type;
+    int32_t threads;           /* partitions actually used (tiles, bands) */
+    int32_t group_size;        /* CRS vector kernel: lanes per row */
+    int64_t device_bytes;      /* HBM held by the handle */
+    int64_t long_rows;         /* rows routed to the long-row path */
+    int32_t block_bits;        /* CSB beta = 2^block_bits (0 when not CSB) */
+    int32_t balanced;          /* CSB balanced flag (CSB.hpp:549-559) */
+} pwm_info_t;
+
+/* Constructor SparseMatrix(int threads) (Matrix/SparseMatrix.hpp:30) through
+ * the factory selectType(method, threads) (Util/DriverUtil.hpp:38-84).
+ * threads <= 0 lets the library pick the partition count. */
+int pwm_create(int format, int threads, int value_type, pwm_matrix** out);
+/* Destructor (e.g. Matrix/CRS.hpp:44-52). */
+int pwm_destroy(pwm_matrix* h);
+/* Message of the last failure on h (h == NULL: last failure of pwm_create
+ * or of a call that had no handle, on this thread). */
+const char* pwm_last_error(const pwm_matrix* h);
+
+/* SparseMatrix::generatePoissonMatrix(m, n, partitions)
+ * (Matrix/SparseMatrix.hpp:41; CRS: Matrix/CRS.hpp:60-74 via
+ * Util/Poisson.hpp:32-83).  The matrix is generated on the device. */
+int pwm_generate_poisson(pwm_matrix* h, int64_t m, int64_t n, int partitions);
+
+/* SparseMatrix::loadFromTriplets(Triplet*, partitions)
+ * (Matrix/SparseMatrix.hpp:48; CRS: Matrix/CRS.hpp:81-91 via
+ * Util/TripletToCRS.hpp:135-172).  The triplets are copied, never modified
+ * (the reference sorts the caller's Triplet in place). */
+int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows,
+                      const int32_t* cols, const double* vals, int partitions);
+
+/* Addition: load a matrix that is already in CRS form (rows ascending,
+ * columns ascending inside each row), e.g. from a generator. */
+int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32_t* row_start,
+                 const int32_t* col_ind, const double* vals, int partitions);
+
+/* SparseMatrix::mv(const T* x, T* y) (Matrix/SparseMatrix.hpp:57; oracle
+ * Matrix/CRS.hpp:99-109).  Host pointers; x and y must not overlap. */
+int pwm_mv(pwm_matrix* h, const double* x, double* y);
+
+/* SparseMatrix::powerIteration(T* x, T* y, it) (Matrix/SparseMatrix.hpp:66;
+ * Matrix/CRS.hpp:118-130).  Host pointers.  On return x and y hold what the
+ * reference leaves in them: the last normalised iterate is in x when `it` is
+ * even and in y when it is odd; the other holds the one before it. */
+int pwm_power_iteration(pwm_matrix* h, double* x, double* y, int64_t it);
+
+int pwm_info(const pwm_matrix* h, pwm_info_t* out);
+
+/* Options (addition).  Keys: "group_size" (CRS vector kernel lanes per row,
+ * 0 = automatic), "long_row" (CRS: rows longer than this use the long-row
+ * path), "items_per_thread" (merge path), "block_bits" (CSB beta override),
+ * "autotune" (1 = time the candidate kernels on the next call and keep the
+ * fastest).  Must be set before the matrix is loaded unless noted. */
+int pwm_set_option(pwm_matrix* h, const char* key, int64_t value);
+
+/* ---- device-resident additions ----------------------------------------
+ * Vectors live in HBM: double for PWM_VALUES_F64, float otherwise. */
+int pwm_mv_device(pwm_matrix* h, const void* x_dev, void* y_dev);
+
+/* Power method on device vectors with an FP64 eigenvalue estimate and a
+ * convergence test (the reference has none, Matrix/CRS.hpp:118-130):
+ *   lambda_k = Rayleigh quotient x_k^T A x_k / x_k^T x_k, also |A x_k|;
+ * stop when |lambda_k - lambda_{k-1}| <= tol*|lambda_k| (tol < 0: never) or
+ * after max_it iterations.  x_dev holds the start vector; on return x_dev and
+ * y_dev are left as pwm_power_iteration leaves x and y.  Outputs may be NULL. */
+int pwm_power_method_device(pwm_matrix* h, void* x_dev, void* y_dev, int64_t max_it, double tol,
+                            double* lambda, double* ynorm, int64_t* iterations);
+
+/* CRS_Merge split points actually used on the device (P+1 entries each;
+ * SOA_Implementations/CRS_Merge.hpp:96-112).  Call with a == NULL to get P. */
+int pwm_merge_splits(pwm_matrix* h, int64_t* P, int64_t* a, int64_t* b);
+
+/* CRS arrays as held on the device (row_start[nor+1], col_ind[nnz], values
+ * widened to double), for bit-exact checks of the device-side builders
+ * (generatePoissonMatrix: Util/Poisson.hpp:32-83; loadFromTriplets:
+ * Util/TripletToCRS.hpp:135-172). */
+int pwm_crs_export(pwm_matrix* h, int32_t* row_start, int32_t* col_ind, double* values);
+
+/* Device layout export for bit-exact parity checks.
+ * CSB (SOA_Implementations/CSB.hpp:464-567): ind[nnz], data[nnz] (double),
+ * blk_ptr[H*V+1].  NULL arrays: sizes only. */
+int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nblocks, uint32_t* ind, double* data,
+                   int32_t* blk_ptr);
+/* BlockCOH band p (SOA_Implementations/BlockCOH.hpp:319-455).  counts[0..5] =
+ * nnz, blocks, row_jumps, block_row_jumps, band_start, band_rows; meta[0..2] =
+ * beta_bits, horizontal_blocks, vertical_blocks.  NULL arrays: sizes only. */
+int pwm_blockcoh_band(pwm_matrix* h, int band, int64_t* counts, int32_t* meta, uint16_t* row_jump,
+                      uint16_t* col_jump, double* data, int16_t* row_jump_block, int16_t* col_jump_block,
+                      int32_t* block_nnz);
+
+/* ---- device memory helpers (plumbing for callers without a HIP runtime) */
+int pwm_device_alloc(size_t bytes, void** ptr);
+int pwm_device_free(void* ptr);
+int pwm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int pwm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int pwm_device_synchronize(void);
+/* Fill n entries of a device vector (double when is_f64, else float) with a
+ * constant. */
+int pwm_device_fill(void* ptr, int64_t n, int is_f64, double value);
+
+/* Timing (addition, used by bench.py): run `reps` SpMVs on the handle's
+ * stream between two HIP events; with flush_bytes > 0 a buffer of that many
+ * bytes is rewritten before each SpMV (outside the events' spans) so the
+ * matrix is not served from the 256 MB Infinity Cache.  Writes the mean
+ * milliseconds per SpMV of the whole launch sequence and, if kernel_ms is
+ * not NULL, of the dominant kernel alone. */
+int pwm_time_mv(pwm_matrix* h, const void* x_dev, void* y_dev, int reps, int64_t flush_bytes,
+                double* ms_per_mv, double* kernel_ms);
+
+/* ---- generators for the benchmark workloads (host side, deterministic) -
+ * Each writes CRS arrays into malloc'd buffers released with pwm_free.
+ * Power-law: N rows, lengths floor(xm * U^(-1/alpha)) capped at cap, columns
+ * uniform without repeats inside a row, values N(0,1).
+ * Stencil9: 9-point stencil (8 / -1) on a side x side grid plus `extra`
+ * uniform off-stencil columns per row with values U(-1,1). */
+int pwm_gen_powerlaw(int64_t n, double alpha, double xm, int64_t cap, uint64_t struct_seed,
+                     uint64_t value_seed, int32_t** row_start, int32_t** col_ind, double** vals,
+                     int64_t* nnz);
+int pwm_gen_stencil9(int64_t side, int64_t extra, uint64_t seed, int32_t** row_start, int32_t** col_ind,
+                     double** vals, int64_t* nnz);
+/* x ~ U[-1,1] from a 64-bit Mersenne twister with the given seed:
+ * (u >> 11) * 2^-53 * 2 - 1. */
+int pwm_gen_uniform(int64_t n, uint64_t seed, double* out);
+void pwm_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PWM_HIP_H */

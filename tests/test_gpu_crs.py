@@ -1,0 +1,293 @@
+"""GPU parity of the CRS vector kernel (CRSHip) and the merge-path kernel
+(CRSMergeHip) against the oracle, through the C ABI.
+
+Bars: integer artefacts (device-built CRS, merge split points) bit-exact;
+FP64 y within 1e-13 relative max-norm; golden vectors at the reference's own
+Boost tolerances (1e-14 mv, 1e-12 power); eigenvalues within 1e-10."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, boost_close, load_case_matrix
+
+pw = pytest.importorskip("thesis_spmv_amd")
+pytestmark = pytest.mark.gpu
+
+FORMATS = [pw.CRSHip, pw.CRSMergeHip]
+
+
+def _relerr(a, b):
+    m = np.max(np.abs(b)) if len(b) else 0.0
+    return float(np.max(np.abs(a - b)) / (m if m > 0 else 1.0)) if len(b) else 0.0
+
+
+def _load(cls, A, threads=-1, values="f64", **opt):
+    m = cls(threads, values, **opt)
+    m.loadCRS(A.nor, A.noc, A.row_start, A.col_ind, A.data)
+    return m
+
+
+# ---------------------------------------------------------------- goldens
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+@pytest.mark.parametrize("idx", range(24))
+def test_golden(goldens, cls, idx):
+    """Every reference test case (Tests/poisson.cpp, Tests/input.cpp) with the
+    same threads sweep the reference uses for its parallel classes."""
+    case = goldens[idx]
+    A, t = load_case_matrix(case)
+    print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
+    threads_list = (1, 2, 3, 4, 8, -1)
+    for threads in threads_list:
+        m = cls(threads)
+        if case["matrix"]["kind"] == "poisson":
+            m.generatePoissonMatrix(case["matrix"]["m"], case["matrix"]["n"], 1)
+        else:
+            rows = t.rows.astype(np.uint32) if case["index_type"] == "uint32" else t.rows
+            cols = t.cols.astype(np.uint32) if case["index_type"] == "uint32" else t.cols
+
+            class T:  # the reference Triplet's members (Matrix/Triplet.hpp:26-45)
+                row_coord, col_coord, data, row_size, col_size = rows, cols, t.vals, t.nor, t.noc
+            m.loadFromTriplets(T, 1)
+        x = np.ones(A.nor)
+        y = np.zeros(A.nor)
+        if case["op"] == "mv":
+            m.mv(x, y)
+            got = y
+        else:
+            m.powerIteration(x, y, case["iterations"])
+            got = x if case["checks"] == "x" else y
+        ok = boost_close(got, case["expected"], case["tolerance"])
+        assert ok.all(), (case["name"], threads, np.flatnonzero(~ok)[:5])
+
+
+# ---------------------------------------------------------------- integer artefacts
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 3), (9, 3), (1, 17), (17, 1), (64, 48), (1024, 1024)])
+def test_device_poisson_bit_exact(m, n):
+    A = oracle.poisson_crs(m, n)
+    h = pw.CRSHip()
+    h.generatePoissonMatrix(m, n, 1)
+    rs, ci, dv = h.crs_export()
+    assert np.array_equal(rs, A.row_start) and np.array_equal(ci, A.col_ind) and np.array_equal(dv, A.data)
+
+
+def test_triplets_to_crs_bit_exact():
+    rng = np.random.default_rng(1)
+    N = 3000
+    r = rng.integers(0, N - 50, 40000).astype(np.int32)
+    c = rng.integers(0, N, 40000).astype(np.int32)
+    v = rng.standard_normal(40000)
+    t = oracle.Triplets(N, N, r, c, v)
+    A = oracle.triplet_to_crs(t)
+    h = pw.CRSHip()
+    h.loadFromTriplets(t)
+    rs, ci, dv = h.crs_export()
+    assert np.array_equal(rs, A.row_start) and np.array_equal(ci, A.col_ind) and np.array_equal(dv, A.data)
+
+
+def _powerlaw(n, seed=1, cap=50000):
+    rs, ci, dv = pw.gen_powerlaw(n, struct_seed=seed, value_seed=seed + 1, cap=cap)
+    return oracle.CRSMatrix(n, n, rs, ci, dv)
+
+
+@pytest.mark.parametrize("threads", [1, 2, 7, 64, 1000, -1])
+def test_merge_splits_bit_exact(threads):
+    A = _powerlaw(20000)
+    m = _load(pw.CRSMergeHip, A, threads)
+    a, b = m.merge_splits()
+    P = len(a) - 1
+    if threads > 0 and -(-(A.nor + A.nnz) // threads) <= 256 * 8:
+        assert P == threads
+    ra, rb = oracle.merge_splits(A, P)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+
+
+# ---------------------------------------------------------------- FP64 SpMV
+
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16, 32, 64])
+def test_vector_kernel_group_sizes(G):
+    A = _powerlaw(30000, cap=3000)
+    x = pw.gen_uniform(A.noc, 0)
+    ref = oracle.crs_mv(A, x)
+    for long_row in (0, 40):  # default threshold, and most rows routed to the chunked path
+        m = _load(pw.CRSHip, A, group_size=G, long_row=long_row, long_chunk=256)
+        y = np.zeros(A.nor)
+        m.mv(x, y)
+        assert _relerr(y, ref) <= 1e-13, (G, long_row)
+
+
+@pytest.mark.parametrize("ipt", [4, 8, 12, 16])
+def test_merge_items_per_thread(ipt):
+    A = _powerlaw(30000, cap=20000)
+    x = pw.gen_uniform(A.noc, 0)
+    ref = oracle.crs_mv(A, x)
+    m = _load(pw.CRSMergeHip, A, items_per_thread=ipt)
+    y = np.zeros(A.nor)
+    m.mv(x, y)
+    assert _relerr(y, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_edge_shapes(cls):
+    cases = []
+    # one row, one column
+    cases.append(oracle.CRSMatrix(1, 1, [0, 1], [0], [2.5]))
+    # all rows empty
+    cases.append(oracle.CRSMatrix(5, 5, [0] * 6, [], []))
+    # leading / trailing empty rows and a single very long row
+    rs = [0, 0, 0, 4000, 4000, 4001, 4001]
+    rng = np.random.default_rng(0)
+    cases.append(oracle.CRSMatrix(6, 9000, rs, rng.integers(0, 9000, 4001), rng.standard_normal(4001)))
+    # rectangular (wide and tall)
+    cases.append(oracle.CRSMatrix(3, 700, [0, 300, 300, 700], np.sort(rng.integers(0, 700, 700)),
+                                  rng.standard_normal(700)))
+    cases.append(oracle.CRSMatrix(900, 2, np.arange(901) // 3 * 1, np.zeros(300, np.int32), np.ones(300)))
+    for A in cases:
+        x = rng.standard_normal(A.noc)
+        ref = oracle.crs_mv(A, x)
+        for threads in (1, 4, -1):
+            m = _load(cls, A, threads)
+            y = np.full(A.nor, np.nan)
+            m.mv(x, y)
+            assert _relerr(y, ref) <= 1e-13
+
+
+def test_stencil9_and_powerlaw_full_config1():
+    """Config-1 size (N = 4 194 304, ~67M nnz) against the oracle's CRS::mv."""
+    A = _powerlaw(4194304)
+    x = pw.gen_uniform(A.noc, 0)
+    ref = oracle.crs_mv(A, x)
+    for cls in FORMATS:
+        m = _load(cls, A)
+        y = np.zeros(A.nor)
+        m.mv(x, y)
+        err = _relerr(y, ref)
+        print(f"{cls.__name__} config-1 rel.err {err:.3e}", flush=True)
+        assert err <= 1e-13
+    # merge split points at the device's tile count, bit-exact
+    m = _load(pw.CRSMergeHip, A)
+    a, b = m.merge_splits()
+    ra, rb = oracle.merge_splits(A, len(a) - 1)
+    assert np.array_equal(a, ra) and np.array_equal(b, rb)
+
+
+def test_device_vectors_and_errors():
+    A = oracle.poisson_crs(40, 30)
+    m = _load(pw.CRSHip, A)
+    x = pw.DeviceVector.from_array(np.ones(A.nor))
+    y = pw.DeviceVector(A.nor)
+    m.mv_device(x, y)
+    assert np.array_equal(y.download(), oracle.crs_mv(A, np.ones(A.nor)))
+    with pytest.raises(pw.PwmError) as e:
+        m.mv_device(x, x)  # aliasing
+    assert e.value.code == -1
+    # power method on a rectangular matrix
+    R = oracle.CRSMatrix(3, 5, [0, 1, 2, 3], [0, 1, 4], [1.0, 2.0, 3.0])
+    mr = _load(pw.CRSHip, R)
+    with pytest.raises(pw.PwmError) as e:
+        mr.powerIteration(np.ones(3), np.zeros(3), 3)
+    assert e.value.code == -5
+    # out-of-range column index, bad row_start
+    with pytest.raises(pw.PwmError):
+        _load(pw.CRSHip, oracle.CRSMatrix(2, 2, [0, 1, 2], [0, 2], [1.0, 1.0]))
+    with pytest.raises(pw.PwmError):
+        pw.CRSHip().loadCRS(2, 2, [0, 2, 1], [0, 1], [1.0, 1.0])
+    # calls before a load
+    with pytest.raises(pw.PwmError) as e:
+        pw.CRSHip().mv(np.ones(2), np.zeros(2))
+    assert e.value.code == -6
+    assert pw.select_type(7) is None
+
+
+# ---------------------------------------------------------------- power method
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+@pytest.mark.parametrize("it", [0, 1, 2, 3, 10, 101])
+def test_power_iteration_state_matches_reference(cls, it):
+    """Both x and y are left as the reference leaves them (Matrix/CRS.hpp:118-130)."""
+    A = _powerlaw(5000, cap=500)
+    A.data[:] = np.abs(A.data)  # nonnegative: a dominant real eigenvalue
+    x0 = pw.gen_uniform(A.nor, 3)
+    xr, yr = oracle.crs_power_iteration(A, x0, it)
+    m = _load(cls, A)
+    x, y = x0.copy(), np.full(A.nor, 7.0)
+    m.powerIteration(x, y, it)
+    if it == 0:
+        assert np.array_equal(x, x0) and np.all(y == 7.0)
+        return
+    yr_cmp = yr if it >= 1 else np.full(A.nor, 7.0)
+    assert _relerr(x, xr) <= 1e-12 and _relerr(y, yr_cmp) <= 1e-12
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_power_method_eigenvalue(cls):
+    """Poisson 1024x1024 (config 0): 100 fixed iterations, lambda vs the FP64
+    oracle's Rayleigh quotient at the same iteration (1e-10), and a small
+    grid run to convergence against the closed form."""
+    mm = 1024
+    A = oracle.poisson_crs(mm, mm)
+    x0 = pw.gen_uniform(A.nor, 0)
+    m = cls()
+    m.generatePoissonMatrix(mm, mm, 1)
+    x = pw.DeviceVector.from_array(x0)
+    y = pw.DeviceVector(A.nor)
+    lam, ynorm, its = m.power_method_device(x, y, 100, -1.0)
+    assert its == 100
+    xr, yr = oracle.crs_power_iteration(A, x0, 99)
+    v = yr  # iterate 99 is in y (odd)
+    rq = float(v @ oracle.crs_mv(A, v) / (v @ v))
+    assert abs(lam - rq) <= 1e-10 * abs(rq), (lam, rq)
+    assert abs(ynorm - np.linalg.norm(oracle.crs_mv(A, v))) <= 1e-10 * ynorm
+    assert _relerr(x.download(), oracle.crs_power_iteration(A, x0, 100)[0]) <= 1e-12
+    # converge on 12x9 Poisson
+    m2 = cls()
+    m2.generatePoissonMatrix(12, 9, 1)
+    x2 = pw.DeviceVector.from_array(np.ones(108))
+    y2 = pw.DeviceVector(108)
+    lam2, _, its2 = m2.power_method_device(x2, y2, 5000, 1e-14)
+    exact = 4 + 2 * np.cos(np.pi / 13) + 2 * np.cos(np.pi / 10)
+    assert its2 < 5000 and abs(lam2 - exact) <= 1e-10 * exact
+
+
+# ---------------------------------------------------------------- mixed precision
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+@pytest.mark.parametrize("values", ["f32", "bf16"])
+def test_mixed_precision_error_reported(cls, values):
+    m = cls(-1, values)
+    m.generatePoissonMatrix(300, 200, 1)
+    A = oracle.poisson_crs(300, 200)
+    x0 = pw.gen_uniform(A.nor, 0)
+    y = np.zeros(A.nor)
+    m.mv(x0, y)
+    err = _relerr(y, oracle.crs_mv(A, x0))
+    print(f"{cls.__name__} {values} SpMV error vs FP64 oracle: {err:.3e}", flush=True)
+    assert err <= 1e-6
+    xd = pw.DeviceVector.from_array(x0, np.float32)
+    yd = pw.DeviceVector(A.nor, np.float32)
+    lam, _, _ = m.power_method_device(xd, yd, 200, -1.0)
+    m64 = cls()
+    m64.generatePoissonMatrix(300, 200, 1)
+    lam64, _, _ = m64.power_method_device(pw.DeviceVector.from_array(x0), pw.DeviceVector(A.nor), 200, -1.0)
+    print(f"{cls.__name__} {values} eigenvalue rel. error vs FP64: {abs(lam - lam64) / lam64:.3e}", flush=True)
+    assert abs(lam - lam64) <= 1e-6 * lam64
+
+
+def test_bf16_rejects_inexact_values():
+    A = oracle.CRSMatrix(2, 2, [0, 1, 2], [0, 1], [1.0, 0.1])
+    with pytest.raises(pw.PwmError):
+        _load(pw.CRSHip, A, values="bf16")
+
+
+def test_cpp_dropin_classes():
+    exe = os.path.join(ROOT, "tests", "cpp", "test_classes")
+    if not os.path.exists(exe):
+        pytest.skip("C++ driver not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]

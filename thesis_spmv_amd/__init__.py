@@ -1,0 +1,323 @@
+"""MI355X (gfx950) SpMV and power-method library: Python host mirror.
+
+The classes here mirror the reference's operator interface
+pwm::SparseMatrix<T,int_type> (Matrix/SparseMatrix.hpp:13-68): the same four
+operations under the same names -- generatePoissonMatrix, loadFromTriplets,
+mv, powerIteration -- with `mult` / `powerMethod` as aliases, and a factory
+select_type(method, threads) beside the reference's selectType
+(Util/DriverUtil.hpp:38-84).  Every call goes through the C ABI of
+libpwmhip.so (include/pwm_hip.h); there is no CPU fallback: if the library
+is not built, importing a class raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpwmhip.so")
+
+FORMAT_CRS, FORMAT_CRS_MERGE, FORMAT_CSB, FORMAT_BLOCKCOH = 1, 8, 9, 10
+VALUES = {"f64": 0, "f32": 1, "bf16": 2}
+ERRORS = {-1: "invalid argument", -2: "I/O", -3: "HIP error", -4: "out of device memory",
+          -5: "not square", -6: "no matrix loaded", -7: "HBM budget", -8: "unsupported"}
+
+_lib = None
+I64, I32, F64 = C.c_int64, C.c_int32, C.c_double
+VP = C.c_void_p
+
+
+class PwmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Info(C.Structure):
+    _fields_ = [("nor", I64), ("noc", I64), ("nnz", I64), ("format", I32), ("value_type", I32),
+                ("threads", I32), ("group_size", I32), ("device_bytes", I64), ("long_rows", I64),
+                ("block_bits", I32), ("balanced", I32)]
+
+
+_SIGS = {
+    "pwm_create": [C.c_int, C.c_int, C.c_int, C.POINTER(VP)],
+    "pwm_destroy": [VP],
+    "pwm_last_error": [VP],
+    "pwm_generate_poisson": [VP, I64, I64, C.c_int],
+    "pwm_load_triplets": [VP, I64, I64, I64, VP, VP, VP, C.c_int],
+    "pwm_load_crs": [VP, I64, I64, VP, VP, VP, C.c_int],
+    "pwm_mv": [VP, VP, VP],
+    "pwm_power_iteration": [VP, VP, VP, I64],
+    "pwm_info": [VP, C.POINTER(Info)],
+    "pwm_set_option": [VP, C.c_char_p, I64],
+    "pwm_mv_device": [VP, VP, VP],
+    "pwm_power_method_device": [VP, VP, VP, I64, F64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64)],
+    "pwm_merge_splits": [VP, C.POINTER(I64), VP, VP],
+    "pwm_crs_export": [VP, VP, VP, VP],
+    "pwm_csb_layout": [VP, C.POINTER(I32), C.POINTER(I64), VP, VP, VP],
+    "pwm_blockcoh_band": [VP, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP],
+    "pwm_device_alloc": [C.c_size_t, C.POINTER(VP)],
+    "pwm_device_free": [VP],
+    "pwm_memcpy_h2d": [VP, VP, C.c_size_t],
+    "pwm_memcpy_d2h": [VP, VP, C.c_size_t],
+    "pwm_device_synchronize": [],
+    "pwm_device_fill": [VP, I64, C.c_int, F64],
+    "pwm_time_mv": [VP, VP, VP, C.c_int, I64, C.POINTER(F64), C.POINTER(F64)],
+    "pwm_gen_powerlaw": [I64, F64, F64, I64, C.c_uint64, C.c_uint64, C.POINTER(VP), C.POINTER(VP),
+                         C.POINTER(VP), C.POINTER(I64)],
+    "pwm_gen_stencil9": [I64, I64, C.c_uint64, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(I64)],
+    "pwm_gen_uniform": [I64, C.c_uint64, VP],
+    "pwm_free": [VP],
+}
+
+
+def lib():
+    """Load libpwmhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name == "pwm_last_error" else (None if name == "pwm_free" else C.c_int)
+        _lib = L
+    return _lib
+
+
+def _check(rc, h=None):
+    if rc != 0:
+        msg = lib().pwm_last_error(h)
+        raise PwmError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(VP)
+
+
+# ------------------------------------------------------------ device memory
+class DeviceVector:
+    """A vector in HBM (double for FP64 handles, float otherwise)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n, self.dtype = int(n), np.dtype(dtype)
+        p = VP()
+        _check(lib().pwm_device_alloc(max(self.n, 1) * self.dtype.itemsize, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        v = cls(len(a), dtype)
+        v.upload(a)
+        return v
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert len(a) == self.n
+        _check(lib().pwm_memcpy_h2d(self.ptr, _ptr(a), a.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().pwm_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def fill(self, v):
+        _check(lib().pwm_device_fill(self.ptr, self.n, int(self.dtype == np.float64), float(v)))
+
+    def free(self):
+        if self.ptr:
+            lib().pwm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(lib().pwm_device_synchronize())
+
+
+# ------------------------------------------------------------ operators
+class SparseMatrixHip:
+    """GPU counterpart of pwm::SparseMatrix<double, int_type>
+    (Matrix/SparseMatrix.hpp:13-68).  `threads` is the reference's constructor
+    argument (:30); on the GPU it sets the partition count of the formats
+    that have one (merge-path tiles, BlockCOH bands)."""
+
+    FORMAT = None
+
+    def __init__(self, threads=-1, values="f64", **options):
+        self.h = VP()
+        _check(lib().pwm_create(self.FORMAT, int(threads), VALUES[values], C.byref(self.h)))
+        self.values = values
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pwm_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def vector_dtype(self):
+        return np.float64 if self.values == "f64" else np.float32
+
+    def set_option(self, key, value):
+        _check(lib().pwm_set_option(self.h, key.encode(), int(value)), self.h)
+
+    # SparseMatrix::generatePoissonMatrix (:41)
+    def generatePoissonMatrix(self, m, n, partitions=1):
+        _check(lib().pwm_generate_poisson(self.h, int(m), int(n), int(partitions)), self.h)
+
+    # SparseMatrix::loadFromTriplets (:48).  `t` has row_coord/col_coord/data
+    # (or rows/cols/vals) and row_size/col_size (or nor/noc), as the
+    # reference's Triplet (Matrix/Triplet.hpp:26-45).
+    def loadFromTriplets(self, t, partitions=1):
+        rows = np.ascontiguousarray(getattr(t, "row_coord", getattr(t, "rows", None)), dtype=np.int32)
+        cols = np.ascontiguousarray(getattr(t, "col_coord", getattr(t, "cols", None)), dtype=np.int32)
+        vals = np.ascontiguousarray(getattr(t, "data", getattr(t, "vals", None)), dtype=np.float64)
+        nor = int(getattr(t, "row_size", getattr(t, "nor", 0)))
+        noc = int(getattr(t, "col_size", getattr(t, "noc", 0)))
+        _check(lib().pwm_load_triplets(self.h, nor, noc, len(rows), _ptr(rows), _ptr(cols), _ptr(vals),
+                                       int(partitions)), self.h)
+
+    def loadCRS(self, nor, noc, row_start, col_ind, data, partitions=1):
+        rs = np.ascontiguousarray(row_start, dtype=np.int32)
+        ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+        dv = np.ascontiguousarray(data, dtype=np.float64)
+        _check(lib().pwm_load_crs(self.h, int(nor), int(noc), _ptr(rs), _ptr(ci), _ptr(dv), int(partitions)),
+               self.h)
+
+    # SparseMatrix::mv (:57): y = A x on host arrays (y is written in place)
+    def mv(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        assert isinstance(y, np.ndarray) and y.dtype == np.float64 and y.flags.c_contiguous
+        _check(lib().pwm_mv(self.h, _ptr(x), _ptr(y)), self.h)
+
+    mult = mv
+
+    # SparseMatrix::powerIteration (:66): x and y are updated in place
+    def powerIteration(self, x, y, it):
+        for a in (x, y):
+            assert isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous
+        _check(lib().pwm_power_iteration(self.h, _ptr(x), _ptr(y), int(it)), self.h)
+
+    powerMethod = powerIteration
+
+    # device-resident additions
+    def mv_device(self, x, y):
+        _check(lib().pwm_mv_device(self.h, x.ptr, y.ptr), self.h)
+
+    def power_method_device(self, x, y, max_it, tol=-1.0):
+        lam, nrm, its = F64(), F64(), I64()
+        _check(lib().pwm_power_method_device(self.h, x.ptr, y.ptr, int(max_it), float(tol), C.byref(lam),
+                                             C.byref(nrm), C.byref(its)), self.h)
+        return lam.value, nrm.value, its.value
+
+    def time_mv(self, x, y, reps, flush_bytes=0):
+        ms, kms = F64(), F64()
+        _check(lib().pwm_time_mv(self.h, x.ptr, y.ptr, int(reps), int(flush_bytes), C.byref(ms), C.byref(kms)),
+               self.h)
+        return ms.value, kms.value
+
+    def crs_export(self):
+        """The CRS arrays held on the device (values widened to double)."""
+        inf = self.info()
+        rs = np.zeros(inf["nor"] + 1, np.int32)
+        ci = np.zeros(max(inf["nnz"], 1), np.int32)
+        dv = np.zeros(max(inf["nnz"], 1), np.float64)
+        _check(lib().pwm_crs_export(self.h, _ptr(rs), _ptr(ci), _ptr(dv)), self.h)
+        return rs, ci[:inf["nnz"]], dv[:inf["nnz"]]
+
+    def info(self):
+        i = Info()
+        _check(lib().pwm_info(self.h, C.byref(i)), self.h)
+        return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    @property
+    def nor(self):
+        return self.info()["nor"]
+
+
+class CRSHip(SparseMatrixHip):
+    """CRS vector kernel (beside CRS / CRSOMP, Matrix/CRS.hpp, Env_Implementations/CRSOMP.hpp)."""
+    FORMAT = FORMAT_CRS
+
+
+class CRSMergeHip(SparseMatrixHip):
+    """Merge-path CRS (beside CRS_Merge, SOA_Implementations/CRS_Merge.hpp)."""
+    FORMAT = FORMAT_CRS_MERGE
+
+    def merge_splits(self):
+        P = I64()
+        _check(lib().pwm_merge_splits(self.h, C.byref(P), None, None), self.h)
+        a = np.zeros(P.value + 1, np.int64)
+        b = np.zeros(P.value + 1, np.int64)
+        _check(lib().pwm_merge_splits(self.h, C.byref(P), _ptr(a), _ptr(b)), self.h)
+        return a, b
+
+
+class CSBHip(SparseMatrixHip):
+    """Compressed sparse blocks (beside CSB, SOA_Implementations/CSB.hpp)."""
+    FORMAT = FORMAT_CSB
+
+
+class BlockCOHHip(SparseMatrixHip):
+    """Row-distributed block CO-H (beside BlockCOH, SOA_Implementations/BlockCOH.hpp)."""
+    FORMAT = FORMAT_BLOCKCOH
+
+
+GPU_METHODS = {101: CRSHip, 102: CRSHip, 108: CRSMergeHip, 109: CSBHip, 110: BlockCOHHip}
+
+
+def select_type(method, threads=-1, values="f64"):
+    """selectType (Util/DriverUtil.hpp:38-84) for the GPU classes: the
+    reference's ids + 100 (101/102 CRS, 108 CRS_Merge, 109 CSB, 110 BlockCOH).
+    Unknown ids return None, as the reference returns NULL."""
+    cls = GPU_METHODS.get(int(method))
+    return cls(threads, values) if cls else None
+
+
+# ------------------------------------------------------------ generators
+def _take_crs(n, rs, ci, dv, nnz):
+    L = lib()
+    z = nnz.value
+    rsa = np.ctypeslib.as_array(C.cast(rs, C.POINTER(C.c_int32)), shape=(n + 1,)).copy()
+    cia = np.ctypeslib.as_array(C.cast(ci, C.POINTER(C.c_int32)), shape=(max(z, 1),))[:z].copy()
+    dva = np.ctypeslib.as_array(C.cast(dv, C.POINTER(C.c_double)), shape=(max(z, 1),))[:z].copy()
+    for p in (rs, ci, dv):
+        L.pwm_free(p)
+    return rsa, cia, dva
+
+
+def gen_powerlaw(n, alpha=1.8, xm=7.3333, cap=50000, struct_seed=1, value_seed=2):
+    """Power-law CRS: lengths floor(xm*U^(-1/alpha)) capped, columns uniform
+    without repeats in a row, values N(0,1).  Returns (row_start, col_ind, data)."""
+    rs, ci, dv, nnz = VP(), VP(), VP(), I64()
+    _check(lib().pwm_gen_powerlaw(int(n), float(alpha), float(xm), int(cap), int(struct_seed), int(value_seed),
+                                  C.byref(rs), C.byref(ci), C.byref(dv), C.byref(nnz)))
+    return _take_crs(int(n), rs, ci, dv, nnz)
+
+
+def gen_stencil9(side, extra=2, seed=3):
+    """9-point stencil (8/-1) on side x side plus `extra` random off-stencil
+    columns per row (values U(-1,1))."""
+    rs, ci, dv, nnz = VP(), VP(), VP(), I64()
+    _check(lib().pwm_gen_stencil9(int(side), int(extra), int(seed), C.byref(rs), C.byref(ci), C.byref(dv),
+                                  C.byref(nnz)))
+    return _take_crs(int(side) * int(side), rs, ci, dv, nnz)
+
+
+def gen_uniform(n, seed=0):
+    """x ~ U[-1,1] from mt19937_64(seed): (u >> 11) * 2^-53 * 2 - 1."""
+    out = np.empty(int(n), np.float64)
+    _check(lib().pwm_gen_uniform(int(n), int(seed), _ptr(out)))
+    return out

@@ -1,0 +1,758 @@
+// C ABI of libpwmhip.so (declared in include/pwm_hip.h): handle lifetime,
+// matrix loading, planning, and the SpMV / power-method entry points.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "api_internal.hpp"
+
+using namespace pwm_impl;
+using pwmk::CrsDev;
+using pwmk::PowerArgs;
+
+namespace pwm_impl {
+
+static thread_local std::string g_err;
+void set_global_error(const std::string& msg) { g_err = msg; }
+
+hipError_t DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return e;
+    }
+    bytes = n;
+    return hipSuccess;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int fail(pwm_matrix* h, int code, const std::string& msg) {
+    if (h)
+        h->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+int hip_fail(pwm_matrix* h, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP, m);
+}
+size_t value_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : (vtype == PWM_VALUES_F32 ? 4 : 2); }
+size_t vector_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : 4; }
+
+static CrsDev crs_of(const pwm_matrix* h) {
+    CrsDev A;
+    A.nrows = h->nor;
+    A.ncols = h->noc;
+    A.nnz = h->nnz;
+    A.rs = h->rs.as<int32_t>();
+    A.ci = h->ci.as<int32_t>();
+    A.val = h->val.p;
+    A.vtype = h->vtype;
+    return A;
+}
+
+static int pow2_ceil(int64_t v) {
+    int g = 1;
+    while (g < v && g < 64) g <<= 1;
+    return g;
+}
+
+// CRS vector kernel: lanes per row, grid size and the long-row chunk plan.
+int plan_crs(pwm_matrix* h) {
+    const double avg = h->nor > 0 ? (double)h->nnz / (double)h->nor : 0.0;
+    int G = h->group;
+    if (G <= 0) G = std::min(32, std::max(2, pow2_ceil((int64_t)ceil(avg / 2.0))));
+    h->group = G;
+    const int64_t rpw = 64 / G;
+    const int64_t units = (h->nor + rpw - 1) / rpw;
+    int64_t blocks = h->vec_blocks > 0 ? h->vec_blocks : std::min<int64_t>((units + 3) / 4, 2048);
+    h->vec_blocks = (int)std::max<int64_t>(blocks, 1);
+
+    int64_t thr = h->long_row > 0 ? h->long_row : std::max<int64_t>(1024, 128 * G);
+    h->lr = pwmk::LongRows{};
+    h->lr.threshold = (int32_t)std::min<int64_t>(thr, INT32_MAX);
+    std::vector<pwmk::LongChunk> chunks;
+    std::vector<int32_t> ids, begin;
+    if (!h->h_rs.empty()) {
+        for (int64_t r = 0; r < h->nor; ++r) {
+            const int64_t s = h->h_rs[r], e = h->h_rs[r + 1];
+            if (e - s <= thr) continue;
+            begin.push_back((int32_t)chunks.size());
+            for (int64_t k = s; k < e; k += h->chunk) {
+                pwmk::LongChunk c;
+                c.row = (int32_t)r;
+                c.slot = (int32_t)ids.size();
+                c.k0 = (int32_t)k;
+                c.k1 = (int32_t)std::min<int64_t>(e, k + h->chunk);
+                chunks.push_back(c);
+            }
+            ids.push_back((int32_t)r);
+        }
+    }
+    begin.push_back((int32_t)chunks.size());
+    h->lr.n_rows = (int32_t)ids.size();
+    h->lr.n_chunks = (int32_t)chunks.size();
+    if (!ids.empty()) {
+        hipError_t e;
+        if ((e = h->lr_chunks.alloc(chunks.size() * sizeof(pwmk::LongChunk))) != hipSuccess ||
+            (e = h->lr_rows.alloc(ids.size() * 4)) != hipSuccess ||
+            (e = h->lr_begin.alloc(begin.size() * 4)) != hipSuccess ||
+            (e = h->lr_partial.alloc(chunks.size() * 8)) != hipSuccess)
+            return hip_fail(h, e, "long-row plan allocation");
+        (void)hipMemcpy(h->lr_chunks.p, chunks.data(), chunks.size() * sizeof(pwmk::LongChunk), hipMemcpyHostToDevice);
+        (void)hipMemcpy(h->lr_rows.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice);
+        if ((e = hipMemcpy(h->lr_begin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(h, e, "long-row plan upload");
+        h->lr.chunks = h->lr_chunks.as<pwmk::LongChunk>();
+        h->lr.row_ids = h->lr_rows.as<int32_t>();
+        h->lr.chunk_begin = h->lr_begin.as<int32_t>();
+        h->lr.chunk_partial = h->lr_partial.p;
+    }
+    return PWM_OK;
+}
+
+// Merge path: P tiles of at most 256*ipt merge items (the threads count
+// given to the constructor is used when its tiles fit that capacity).
+int plan_merge(pwm_matrix* h) {
+    const int64_t total = h->nor + h->nnz;
+    const int64_t cap = 256 * (int64_t)h->ipt;
+    int64_t P = (total + cap - 1) / cap;
+    if (P < 1) P = 1;
+    if (h->threads > 0) {
+        const int64_t want = h->threads;
+        const int64_t items = (total + want - 1) / want;
+        if (items <= cap) P = want;
+    }
+    h->mp = pwmk::MergePlan{};
+    h->mp.P = P;
+    h->mp.items_per_tile = (total + P - 1) / P;
+    h->mp.ipt = h->ipt;
+    hipError_t e;
+    if ((e = h->split_a.alloc((P + 1) * 8)) != hipSuccess || (e = h->split_b.alloc((P + 1) * 8)) != hipSuccess ||
+        (e = h->carry_row.alloc(P * 4)) != hipSuccess || (e = h->carry_val.alloc(P * 8)) != hipSuccess)
+        return hip_fail(h, e, "merge plan allocation");
+    h->mp.split_a = h->split_a.as<int64_t>();
+    h->mp.split_b = h->split_b.as<int64_t>();
+    h->mp.carry_row = h->carry_row.as<int32_t>();
+    h->mp.carry_val = h->carry_val.p;
+    e = pwmk::launch_merge_splits(crs_of(h), h->mp, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "merge split search");
+    return PWM_OK;
+}
+
+int spmv_launch(pwm_matrix* h, const void* x, void* y, const PowerArgs* pw, int* n_partials_out) {
+    const CrsDev A = crs_of(h);
+    hipError_t e = hipSuccess;
+    int np = 0;
+    switch (h->format) {
+        case PWM_FORMAT_CRS:
+            e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, pw, h->stream);
+            if (e == hipSuccess && (pw || h->lr.n_rows > 0))
+                e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, pw, h->stream);
+            np = h->vec_blocks;
+            break;
+        case PWM_FORMAT_CRS_MERGE:
+            e = pwmk::launch_merge_spmv(A, h->mp, x, y, pw, h->stream);
+            if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, pw, h->stream);
+            np = (int)h->mp.P;
+            break;
+        case PWM_FORMAT_CSB:
+            e = csb_launch(h, x, y, pw);
+            np = (int)csb_partials(h);
+            break;
+        case PWM_FORMAT_BLOCKCOH:
+            e = coh_launch(h, x, y, pw);
+            np = (int)coh_partials(h);
+            break;
+        default:
+            return fail(h, PWM_ERR_INVALID, "unknown format");
+    }
+    if (n_partials_out) *n_partials_out = np;
+    if (e != hipSuccess) return hip_fail(h, e, "SpMV launch");
+    return PWM_OK;
+}
+
+static int64_t partials_needed(const pwm_matrix* h) {
+    switch (h->format) {
+        case PWM_FORMAT_CRS: return h->vec_blocks;
+        case PWM_FORMAT_CRS_MERGE: return h->mp.P;
+        case PWM_FORMAT_CSB: return csb_partials(h);
+        case PWM_FORMAT_BLOCKCOH: return coh_partials(h);
+    }
+    return 0;
+}
+
+}  // namespace pwm_impl
+
+// ---------------------------------------------------------------- lifetime
+
+extern "C" int pwm_create(int format, int threads, int value_type, pwm_matrix** out) {
+    if (!out) return fail(nullptr, PWM_ERR_INVALID, "pwm_create: out is NULL");
+    *out = nullptr;
+    if (format != PWM_FORMAT_CRS && format != PWM_FORMAT_CRS_MERGE && format != PWM_FORMAT_CSB &&
+        format != PWM_FORMAT_BLOCKCOH)
+        return fail(nullptr, PWM_ERR_INVALID, "pwm_create: unknown format id " + std::to_string(format));
+    if (value_type < PWM_VALUES_F64 || value_type > PWM_VALUES_BF16)
+        return fail(nullptr, PWM_ERR_INVALID, "pwm_create: unknown value type");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1)
+        return fail(nullptr, PWM_ERR_HIP, "pwm_create: no HIP device available");
+    pwm_matrix* h = new pwm_matrix();
+    h->format = format;
+    h->threads = threads;
+    h->vtype = value_type;
+    e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return hip_fail(nullptr, e, "pwm_create: stream");
+    }
+    *out = h;
+    return PWM_OK;
+}
+
+extern "C" int pwm_destroy(pwm_matrix* h) {
+    if (!h) return PWM_OK;
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return PWM_OK;
+}
+
+extern "C" const char* pwm_last_error(const pwm_matrix* h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
+    if (!h || !key) return fail(h, PWM_ERR_INVALID, "pwm_set_option: NULL argument");
+    const std::string k(key);
+    if (k == "group_size") {
+        if (value != 0 && (value < 1 || value > 64 || (value & (value - 1))))
+            return fail(h, PWM_ERR_INVALID, "group_size must be 0 or a power of two in 1..64");
+        h->group = (int)value;
+    } else if (k == "vec_blocks") {
+        h->vec_blocks = (int)std::max<int64_t>(0, value);
+    } else if (k == "long_row") {
+        h->long_row = std::max<int64_t>(0, value);
+    } else if (k == "long_chunk") {
+        if (value < 256) return fail(h, PWM_ERR_INVALID, "long_chunk must be >= 256");
+        h->chunk = (int)value;
+    } else if (k == "items_per_thread") {
+        if (value != 4 && value != 8 && value != 12 && value != 16)
+            return fail(h, PWM_ERR_INVALID, "items_per_thread must be 4, 8, 12 or 16");
+        h->ipt = (int)value;
+    } else if (k == "block_bits") {
+        if (value < 0 || value > 16) return fail(h, PWM_ERR_INVALID, "block_bits must be 0..16");
+        h->csb_bits_override = (int32_t)value;
+    } else if (k == "autotune") {
+        h->autotune = value != 0;
+    } else {
+        return fail(h, PWM_ERR_UNSUPPORTED, "unknown option " + k);
+    }
+    if (h->loaded && (k == "group_size" || k == "vec_blocks" || k == "long_row" || k == "long_chunk") &&
+        h->format == PWM_FORMAT_CRS)
+        return plan_crs(h);
+    if (h->loaded && k == "items_per_thread" && h->format == PWM_FORMAT_CRS_MERGE) return plan_merge(h);
+    return PWM_OK;
+}
+
+// ---------------------------------------------------------------- loading
+
+static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
+    hipError_t e = h->val.alloc(std::max<int64_t>(nnz, 1) * value_size(h->vtype));
+    if (e != hipSuccess) return hip_fail(h, e, "value allocation");
+    if (nnz == 0) return PWM_OK;
+    if (h->vtype == PWM_VALUES_F64) {
+        e = hipMemcpy(h->val.p, vals, nnz * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(h, e, "value upload");
+        return PWM_OK;
+    }
+    if (h->vtype == PWM_VALUES_BF16) {
+        for (int64_t i = 0; i < nnz; ++i) {
+            const float f = (float)vals[i];
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            if ((double)f != vals[i] || (b & 0xffffu))
+                return fail(h, PWM_ERR_INVALID, "BF16 storage needs values exact in BF16 (entry " +
+                                                    std::to_string(i) + ")");
+        }
+    }
+    DevBuf tmp;
+    if ((e = tmp.alloc(nnz * 8)) != hipSuccess) return hip_fail(h, e, "staging allocation");
+    if ((e = hipMemcpy(tmp.p, vals, nnz * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "value upload");
+    e = pwmk::launch_convert_values(tmp.as<double>(), h->val.p, nnz, h->vtype, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "value conversion");
+    return PWM_OK;
+}
+
+static void reset_matrix(pwm_matrix* h) {
+    h->loaded = false;
+    h->rs.release();
+    h->ci.release();
+    h->val.release();
+    h->h_rs.clear();
+    h->lr = pwmk::LongRows{};
+    h->mp = pwmk::MergePlan{};
+    h->csb.clear();
+    h->coh.clear();
+}
+
+static int finish_load(pwm_matrix* h, const int32_t* hrs, const int32_t* hci, const double* hval) {
+    int rc = PWM_OK;
+    switch (h->format) {
+        case PWM_FORMAT_CRS: rc = plan_crs(h); break;
+        case PWM_FORMAT_CRS_MERGE: rc = plan_merge(h); break;
+        case PWM_FORMAT_CSB: rc = build_csb(h, hrs, hci, hval); break;
+        case PWM_FORMAT_BLOCKCOH: rc = build_blockcoh(h, hrs, hci, hval); break;
+    }
+    if (rc == PWM_OK) h->loaded = true;
+    return rc;
+}
+
+// Validated host CRS -> device.
+static int load_crs_host(pwm_matrix* h, int64_t nor, int64_t noc, const int32_t* rs, const int32_t* ci,
+                         const double* vals) {
+    const int64_t nnz = rs[nor];
+    h->nor = nor;
+    h->noc = noc;
+    h->nnz = nnz;
+    hipError_t e;
+    if ((e = h->rs.alloc((nor + 1) * 4)) != hipSuccess || (e = h->ci.alloc(std::max<int64_t>(nnz, 1) * 4)) != hipSuccess)
+        return hip_fail(h, e, "CRS allocation");
+    if ((e = hipMemcpy(h->rs.p, rs, (nor + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (nnz && (e = hipMemcpy(h->ci.p, ci, nnz * 4, hipMemcpyHostToDevice)) != hipSuccess))
+        return hip_fail(h, e, "CRS upload");
+    int rc = upload_values(h, vals, nnz);
+    if (rc != PWM_OK) return rc;
+    h->h_rs.assign(rs, rs + nor + 1);
+    return finish_load(h, rs, ci, vals);
+}
+
+extern "C" int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32_t* rs, const int32_t* ci,
+                            const double* vals, int partitions) {
+    (void)partitions;
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_load_crs: NULL handle");
+    if (nor <= 0 || noc <= 0 || nor > INT32_MAX - 1 || noc > INT32_MAX || !rs)
+        return fail(h, PWM_ERR_INVALID, "pwm_load_crs: need 0 < nor, noc < 2^31 and row_start");
+    if (rs[0] != 0) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: row_start[0] must be 0");
+    for (int64_t r = 0; r < nor; ++r)
+        if (rs[r + 1] < rs[r]) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: row_start must be non-decreasing");
+    const int64_t nnz = rs[nor];
+    if (nnz > 0 && (!ci || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: NULL col_ind/values");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (ci[k] < 0 || ci[k] >= noc) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: column index out of range");
+    reset_matrix(h);
+    return load_crs_host(h, nor, noc, rs, ci, vals);
+}
+
+// COO -> CRS (Util/TripletToCRS.hpp:135-172): rows ascending, columns
+// ascending inside a row, duplicates kept in input order.  Counting sort by
+// row, then a stable sort of each row by column.
+extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows,
+                                 const int32_t* cols, const double* vals, int partitions) {
+    (void)partitions;
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_load_triplets: NULL handle");
+    if (nor <= 0 || noc <= 0 || nnz < 0 || nor > INT32_MAX - 1 || noc > INT32_MAX || nnz > INT32_MAX)
+        return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: need 0 < nor, noc < 2^31 and 0 <= nnz < 2^31");
+    if (nnz > 0 && (!rows || !cols || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: NULL arrays");
+    std::vector<int32_t> rs(nor + 1, 0);
+    for (int64_t i = 0; i < nnz; ++i) {
+        if (rows[i] < 0 || rows[i] >= nor || cols[i] < 0 || cols[i] >= noc)
+            return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: index out of range at entry " + std::to_string(i));
+        rs[rows[i] + 1]++;
+    }
+    for (int64_t r = 0; r < nor; ++r) rs[r + 1] += rs[r];
+    std::vector<int32_t> ci(std::max<int64_t>(nnz, 1));
+    std::vector<double> dv(std::max<int64_t>(nnz, 1));
+    {
+        std::vector<int32_t> pos(rs.begin(), rs.end() - 1);
+        for (int64_t i = 0; i < nnz; ++i) {
+            const int32_t p = pos[rows[i]]++;
+            ci[p] = cols[i];
+            dv[p] = vals[i];
+        }
+    }
+    std::vector<std::pair<int32_t, double>> tmp;
+    for (int64_t r = 0; r < nor; ++r) {
+        const int32_t s = rs[r], e = rs[r + 1];
+        if (e - s < 2) continue;
+        bool sorted = true;
+        for (int32_t k = s + 1; k < e && sorted; ++k) sorted = ci[k - 1] <= ci[k];
+        if (sorted) continue;
+        tmp.resize(e - s);
+        for (int32_t k = s; k < e; ++k) tmp[k - s] = {ci[k], dv[k]};
+        std::stable_sort(tmp.begin(), tmp.end(),
+                         [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                             return a.first < b.first;
+                         });
+        for (int32_t k = s; k < e; ++k) {
+            ci[k] = tmp[k - s].first;
+            dv[k] = tmp[k - s].second;
+        }
+    }
+    reset_matrix(h);
+    return load_crs_host(h, nor, noc, rs.data(), ci.data(), dv.data());
+}
+
+// SparseMatrix::generatePoissonMatrix: CRS filled on the device
+// (Util/Poisson.hpp:32-83).  The blocked formats build from the same matrix.
+extern "C" int pwm_generate_poisson(pwm_matrix* h, int64_t m, int64_t n, int partitions) {
+    (void)partitions;
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_generate_poisson: NULL handle");
+    if (m <= 0 || n <= 0 || m * n > INT32_MAX - 1 || 5 * m * n > INT32_MAX)
+        return fail(h, PWM_ERR_INVALID, "pwm_generate_poisson: need m, n > 0 and 5*m*n < 2^31");
+    reset_matrix(h);
+    const int64_t N = m * n, Z = n * (m + 2 * (m - 1)) + 2 * (n - 1) * m;
+    h->nor = h->noc = N;
+    h->nnz = Z;
+    hipError_t e;
+    if ((e = h->rs.alloc((N + 1) * 4)) != hipSuccess || (e = h->ci.alloc(Z * 4)) != hipSuccess ||
+        (e = h->val.alloc(Z * value_size(h->vtype))) != hipSuccess)
+        return hip_fail(h, e, "Poisson allocation");
+    e = pwmk::launch_fill_poisson(m, n, h->rs.as<int32_t>(), h->ci.as<int32_t>(), h->val.p, h->vtype, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "Poisson fill");
+    if (h->format == PWM_FORMAT_CSB || h->format == PWM_FORMAT_BLOCKCOH) {
+        // the blocked builders work from host CRS arrays
+        std::vector<int32_t> rs(N + 1), ci(Z);
+        std::vector<double> dv(Z);
+        DevBuf tmp;
+        if ((e = tmp.alloc(Z * 8)) != hipSuccess) return hip_fail(h, e, "staging allocation");
+        e = pwmk::launch_fill_poisson(m, n, h->rs.as<int32_t>(), h->ci.as<int32_t>(), tmp.p, pwmk::VT_F64, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(rs.data(), h->rs.p, (N + 1) * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ci.data(), h->ci.p, Z * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(dv.data(), tmp.p, Z * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(h, e, "Poisson download");
+        h->h_rs = rs;
+        return finish_load(h, rs.data(), ci.data(), dv.data());
+    }
+    return finish_load(h, nullptr, nullptr, nullptr);
+}
+
+extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
+    if (!h || !out) return fail(const_cast<pwm_matrix*>(h), PWM_ERR_INVALID, "pwm_info: NULL argument");
+    memset(out, 0, sizeof(*out));
+    out->nor = h->nor;
+    out->noc = h->noc;
+    out->nnz = h->nnz;
+    out->format = h->format;
+    out->value_type = h->vtype;
+    out->group_size = h->group;
+    out->long_rows = h->lr.n_rows;
+    out->threads = h->format == PWM_FORMAT_CRS_MERGE ? (int32_t)h->mp.P
+                   : h->format == PWM_FORMAT_BLOCKCOH ? (int32_t)h->coh.bands.size()
+                   : h->format == PWM_FORMAT_CSB      ? (int32_t)h->csb.n_tasks
+                                                      : h->vec_blocks;
+    out->block_bits = h->format == PWM_FORMAT_CSB ? h->csb.bits : 0;
+    out->balanced = h->format == PWM_FORMAT_CSB ? h->csb.balanced : 0;
+    const pwm_impl::DevBuf* bufs[] = {&h->rs, &h->ci, &h->val, &h->lr_chunks, &h->lr_rows, &h->lr_begin,
+                                      &h->lr_partial, &h->split_a, &h->split_b, &h->carry_row, &h->carry_val,
+                                      &h->csb.ind, &h->csb.data, &h->csb.blk_ptr, &h->csb.work,
+                                      &h->coh.row_jump, &h->coh.col_jump, &h->coh.data, &h->coh.row_jump_block,
+                                      &h->coh.col_jump_block, &h->coh.block_nnz, &h->coh.block_coords,
+                                      &h->coh.band_table, &h->coh.tasks, &h->pw_ctrl, &h->pw_rec, &h->pw_partials,
+                                      &h->hx, &h->hy};
+    int64_t b = 0;
+    for (auto* d : bufs) b += (int64_t)d->bytes;
+    out->device_bytes = b;
+    return PWM_OK;
+}
+
+// ---------------------------------------------------------------- SpMV
+
+extern "C" int pwm_mv_device(pwm_matrix* h, const void* x, void* y) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_mv_device: NULL handle");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_mv_device: no matrix loaded");
+    if (!x || !y) return fail(h, PWM_ERR_INVALID, "pwm_mv_device: NULL vector");
+    const size_t vs = vector_size(h->vtype);
+    const char* xb = (const char*)x;
+    const char* yb = (const char*)y;
+    if (xb < yb + h->nor * vs && yb < xb + h->noc * vs)
+        return fail(h, PWM_ERR_INVALID, "pwm_mv_device: x and y overlap");
+    int rc = spmv_launch(h, x, y, nullptr, nullptr);
+    if (rc != PWM_OK) return rc;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_mv_device");
+    return PWM_OK;
+}
+
+static int stage_vectors(pwm_matrix* h) {
+    const size_t vs = vector_size(h->vtype);
+    const size_t need = (size_t)std::max(h->nor, h->noc) * vs;
+    hipError_t e;
+    if (h->hx.bytes < need && (e = h->hx.alloc(need)) != hipSuccess) return hip_fail(h, e, "staging allocation");
+    if (h->hy.bytes < need && (e = h->hy.alloc(need)) != hipSuccess) return hip_fail(h, e, "staging allocation");
+    return PWM_OK;
+}
+
+// Host double vector -> device vector of the handle's type.
+static int upload_vec(pwm_matrix* h, const double* src, void* dst, int64_t n) {
+    hipError_t e;
+    if (h->vtype == PWM_VALUES_F64) {
+        e = hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, h->stream);
+    } else {
+        std::vector<float> f(src, src + n);
+        e = hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "vector upload");
+    return PWM_OK;
+}
+static int download_vec(pwm_matrix* h, const void* src, double* dst, int64_t n) {
+    hipError_t e;
+    if (h->vtype == PWM_VALUES_F64) {
+        e = hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    } else {
+        std::vector<float> f(n);
+        e = hipMemcpyAsync(f.data(), src, n * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        for (int64_t i = 0; i < n; ++i) dst[i] = f[i];
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "vector download");
+    return PWM_OK;
+}
+
+extern "C" int pwm_mv(pwm_matrix* h, const double* x, double* y) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_mv: NULL handle");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_mv: no matrix loaded");
+    if (!x || !y) return fail(h, PWM_ERR_INVALID, "pwm_mv: NULL vector");
+    if (x < y + h->nor && y < x + h->noc) return fail(h, PWM_ERR_INVALID, "pwm_mv: x and y overlap");
+    int rc = stage_vectors(h);
+    if (rc == PWM_OK) rc = upload_vec(h, x, h->hx.p, h->noc);
+    if (rc == PWM_OK) rc = spmv_launch(h, h->hx.p, h->hy.p, nullptr, nullptr);
+    if (rc == PWM_OK) rc = download_vec(h, h->hy.p, y, h->nor);
+    return rc;
+}
+
+// ---------------------------------------------------------------- power method
+
+static int power_workspace(pwm_matrix* h, int64_t max_it) {
+    hipError_t e;
+    if (!h->pw_ctrl.p && (e = h->pw_ctrl.alloc(64)) != hipSuccess) return hip_fail(h, e, "power workspace");
+    const int64_t np = std::max<int64_t>(partials_needed(h), 1);
+    if ((int64_t)h->pw_partials.bytes < np * 24 && (e = h->pw_partials.alloc(np * 24)) != hipSuccess)
+        return hip_fail(h, e, "power workspace");
+    if (h->pw_rec_cap < max_it) {
+        if ((e = h->pw_rec.alloc(std::max<int64_t>(max_it, 1) * 32)) != hipSuccess)
+            return hip_fail(h, e, "power workspace");
+        h->pw_rec_cap = max_it;
+    }
+    return PWM_OK;
+}
+
+// Runs the device power loop on buf0 (start vector) / buf1; returns steps done.
+static int run_power(pwm_matrix* h, void* buf0, void* buf1, int64_t max_it, double tol, int64_t* steps,
+                     double* lambda, double* ynorm) {
+    if (h->nor != h->noc) return fail(h, PWM_ERR_NOT_SQUARE, "power method needs a square matrix");
+    if (max_it < 0) return fail(h, PWM_ERR_INVALID, "negative iteration count");
+    if (max_it == 0) {
+        if (steps) *steps = 0;
+        return PWM_OK;
+    }
+    int rc = power_workspace(h, max_it);
+    if (rc != PWM_OK) return rc;
+    PowerArgs pw;
+    pw.buf0 = buf0;
+    pw.buf1 = buf1;
+    pw.ctrl = h->pw_ctrl.as<int>();
+    pw.rec = h->pw_rec.as<double>();
+    pw.partials = h->pw_partials.as<double>();
+    pw.max_it = max_it;
+    pw.tol = tol;
+    hipError_t e = pwmk::launch_power_reset(pw, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "power reset");
+    const int64_t chunk = tol < 0 ? max_it : 32;
+    int64_t issued = 0;
+    int ctrl[2] = {0, 0};
+    while (issued < max_it) {
+        const int64_t n = std::min(chunk, max_it - issued);
+        for (int64_t i = 0; i < n; ++i) {
+            rc = spmv_launch(h, nullptr, nullptr, &pw, nullptr);
+            if (rc != PWM_OK) return rc;
+        }
+        issued += n;
+        if (tol >= 0) {
+            e = hipMemcpyAsync(ctrl, h->pw_ctrl.p, 8, hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) return hip_fail(h, e, "power loop");
+            if (ctrl[1]) break;
+        }
+    }
+    e = pwmk::launch_power_finish(pw, h->nor, h->vtype == PWM_VALUES_F64, 0, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctrl, h->pw_ctrl.p, 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "power finish");
+    if (steps) *steps = ctrl[0];
+    if ((lambda || ynorm) && ctrl[0] > 0) {
+        double rec[4];
+        e = hipMemcpy(rec, h->pw_rec.as<double>() + 4 * (ctrl[0] - 1), 32, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(h, e, "power record");
+        if (lambda) *lambda = rec[3];
+        if (ynorm) *ynorm = sqrt(rec[0]);
+    }
+    return PWM_OK;
+}
+
+extern "C" int pwm_power_iteration(pwm_matrix* h, double* x, double* y, int64_t it) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_power_iteration: NULL handle");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_power_iteration: no matrix loaded");
+    if (!x || !y) return fail(h, PWM_ERR_INVALID, "pwm_power_iteration: NULL vector");
+    if (h->nor != h->noc) return fail(h, PWM_ERR_NOT_SQUARE, "power method needs a square matrix");
+    if (x < y + h->nor && y < x + h->nor) return fail(h, PWM_ERR_INVALID, "pwm_power_iteration: x and y overlap");
+    if (it <= 0) return it < 0 ? fail(h, PWM_ERR_INVALID, "negative iteration count") : PWM_OK;
+    int rc = stage_vectors(h);
+    if (rc == PWM_OK) rc = upload_vec(h, x, h->hx.p, h->nor);
+    int64_t steps = 0;
+    if (rc == PWM_OK) rc = run_power(h, h->hx.p, h->hy.p, it, -1.0, &steps, nullptr, nullptr);
+    if (rc == PWM_OK) rc = download_vec(h, h->hx.p, x, h->nor);
+    if (rc == PWM_OK) rc = download_vec(h, h->hy.p, y, h->nor);
+    return rc;
+}
+
+extern "C" int pwm_power_method_device(pwm_matrix* h, void* x, void* y, int64_t max_it, double tol, double* lambda,
+                                       double* ynorm, int64_t* iterations) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_power_method_device: NULL handle");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_power_method_device: no matrix loaded");
+    if (!x || !y) return fail(h, PWM_ERR_INVALID, "pwm_power_method_device: NULL vector");
+    const size_t vs = vector_size(h->vtype);
+    if ((const char*)x < (const char*)y + h->nor * vs && (const char*)y < (const char*)x + h->nor * vs)
+        return fail(h, PWM_ERR_INVALID, "pwm_power_method_device: x and y overlap");
+    int64_t steps = 0;
+    int rc = run_power(h, x, y, max_it, tol, &steps, lambda, ynorm);
+    if (iterations) *iterations = steps;
+    return rc;
+}
+
+// ---------------------------------------------------------------- exports
+
+extern "C" int pwm_merge_splits(pwm_matrix* h, int64_t* P, int64_t* a, int64_t* b) {
+    if (!h || !P) return fail(h, PWM_ERR_INVALID, "pwm_merge_splits: NULL argument");
+    if (h->format != PWM_FORMAT_CRS_MERGE || !h->loaded)
+        return fail(h, PWM_ERR_STATE, "pwm_merge_splits: not a loaded CRS_MERGE handle");
+    *P = h->mp.P;
+    if (!a || !b) return PWM_OK;
+    hipError_t e = hipMemcpy(a, h->split_a.p, (h->mp.P + 1) * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(b, h->split_b.p, (h->mp.P + 1) * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_merge_splits");
+    return PWM_OK;
+}
+
+extern "C" int pwm_crs_export(pwm_matrix* h, int32_t* row_start, int32_t* col_ind, double* values) {
+    if (!h || !row_start || !col_ind || !values) return fail(h, PWM_ERR_INVALID, "pwm_crs_export: NULL argument");
+    if (!h->loaded || !h->rs.p) return fail(h, PWM_ERR_STATE, "pwm_crs_export: no CRS arrays on the device");
+    hipError_t e = hipMemcpy(row_start, h->rs.p, (h->nor + 1) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h->nnz) e = hipMemcpy(col_ind, h->ci.p, h->nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h->nnz) {
+        DevBuf tmp;
+        e = tmp.alloc(h->nnz * 8);
+        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(h->val.p, h->vtype, tmp.as<double>(), h->nnz, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(values, tmp.p, h->nnz * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_crs_export");
+    return PWM_OK;
+}
+
+// ---------------------------------------------------------------- device helpers
+
+extern "C" int pwm_device_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(nullptr, PWM_ERR_INVALID, "pwm_device_alloc: NULL out");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "pwm_device_alloc");
+    return PWM_OK;
+}
+extern "C" int pwm_device_free(void* ptr) {
+    if (ptr) {
+        hipError_t e = hipFree(ptr);
+        if (e != hipSuccess) return hip_fail(nullptr, e, "pwm_device_free");
+    }
+    return PWM_OK;
+}
+extern "C" int pwm_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_memcpy_h2d");
+}
+extern "C" int pwm_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_memcpy_d2h");
+}
+extern "C" int pwm_device_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_device_synchronize");
+}
+extern "C" int pwm_device_fill(void* ptr, int64_t n, int is_f64, double value) {
+    hipError_t e = pwmk::launch_fill(ptr, n, is_f64, value, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_device_fill");
+}
+
+// ---------------------------------------------------------------- timing
+
+extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int64_t flush_bytes, double* ms_per_mv,
+                           double* kernel_ms) {
+    if (!h || !x || !y || reps <= 0) return fail(h, PWM_ERR_INVALID, "pwm_time_mv: bad argument");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_time_mv: no matrix loaded");
+    hipError_t e;
+    if (flush_bytes > 0 && (int64_t)h->flush.bytes < flush_bytes && (e = h->flush.alloc(flush_bytes)) != hipSuccess)
+        return hip_fail(h, e, "flush buffer");
+    std::vector<hipEvent_t> ev(3 * reps);
+    for (auto& v : ev)
+        if ((e = hipEventCreate(&v)) != hipSuccess) return hip_fail(h, e, "event create");
+    const CrsDev A = crs_of(h);
+    int rc = PWM_OK;
+    for (int r = 0; r < reps && rc == PWM_OK; ++r) {
+        if (flush_bytes > 0) (void)pwmk::launch_flush(h->flush.p, flush_bytes, h->stream);
+        (void)hipEventRecord(ev[3 * r], h->stream);
+        // dominant kernel first, then the rest of the sequence
+        switch (h->format) {
+            case PWM_FORMAT_CRS:
+                e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, nullptr, h->stream);
+                (void)hipEventRecord(ev[3 * r + 1], h->stream);
+                if (e == hipSuccess && h->lr.n_rows > 0)
+                    e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, nullptr, h->stream);
+                break;
+            case PWM_FORMAT_CRS_MERGE:
+                e = pwmk::launch_merge_spmv(A, h->mp, x, y, nullptr, h->stream);
+                (void)hipEventRecord(ev[3 * r + 1], h->stream);
+                if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+                break;
+            default:
+                rc = spmv_launch(h, x, y, nullptr, nullptr);
+                (void)hipEventRecord(ev[3 * r + 1], h->stream);
+                e = hipSuccess;
+        }
+        if (e != hipSuccess) rc = hip_fail(h, e, "timed SpMV");
+        (void)hipEventRecord(ev[3 * r + 2], h->stream);
+    }
+    e = hipStreamSynchronize(h->stream);
+    if (rc == PWM_OK && e != hipSuccess) rc = hip_fail(h, e, "timed SpMV");
+    double tot = 0, ker = 0;
+    for (int r = 0; r < reps && rc == PWM_OK; ++r) {
+        float a = 0, b = 0;
+        (void)hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 2]);
+        (void)hipEventElapsedTime(&b, ev[3 * r], ev[3 * r + 1]);
+        tot += a;
+        ker += b;
+    }
+    for (auto& v : ev) (void)hipEventDestroy(v);
+    if (rc != PWM_OK) return rc;
+    if (ms_per_mv) *ms_per_mv = tot / reps;
+    if (kernel_ms) *kernel_ms = ker / reps;
+    return PWM_OK;
+}

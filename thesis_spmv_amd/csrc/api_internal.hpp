@@ -1,0 +1,167 @@
+// Internal state of a pwm_matrix handle and helpers shared by the host-side
+// translation units of libpwmhip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <atomic>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pwm_hip.h"
+#include "kernels.hpp"
+
+namespace pwm_impl {
+
+// Host-side parallel loop over [0, n) in chunks (std::thread; the library
+// does not depend on an OpenMP runtime).  fn(begin, end, worker).
+template <typename F>
+void parallel_for(int64_t n, int64_t chunk, F fn) {
+    if (n <= 0) return;
+    unsigned hw = std::thread::hardware_concurrency();
+    int64_t nt = std::min<int64_t>(hw ? hw : 4, 16);  // the GPU host grants 16 cores per GPU
+    nt = std::min<int64_t>(nt, (n + chunk - 1) / chunk);
+    if (nt <= 1) {
+        fn((int64_t)0, n, 0);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> pool;
+    for (int64_t t = 0; t < nt; ++t)
+        pool.emplace_back([&, t]() {
+            for (;;) {
+                const int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                fn(b, std::min(n, b + chunk), (int)t);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+// Thread-local message for calls without a handle.
+void set_global_error(const std::string& msg);
+
+// Owned device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    hipError_t alloc(size_t n);
+    void release();
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+// CSB device layout (SOA_Implementations/CSB.hpp:34-59).
+struct CsbDev {
+    int32_t bits = 0;
+    int64_t H = 0, V = 0;
+    int32_t balanced = 1;
+    DevBuf ind, data, blk_ptr;   // u32[Z], value[Z], i32[H*V+1]
+    DevBuf work;                 // per-workgroup tasks
+    int64_t n_tasks = 0;
+    int32_t slab_bits = 0;
+    void clear() {
+        bits = 0;
+        H = V = 0;
+        balanced = 1;
+        ind.release();
+        data.release();
+        blk_ptr.release();
+        work.release();
+        n_tasks = 0;
+        slab_bits = 0;
+    }
+};
+
+// BlockCOH device layout (SOA_Implementations/BlockCOH.hpp:27-67), all bands
+// concatenated; offsets say where each band's part of each array starts.
+struct CohBandMeta {
+    int64_t start, rows, nnz, blocks, row_jumps, block_row_jumps;
+    int32_t beta_bits, H, V;
+    int64_t off_nz, off_rj, off_blk, off_rjb;
+};
+struct CohDev {
+    std::vector<CohBandMeta> bands;
+    DevBuf row_jump, col_jump, data, row_jump_block, col_jump_block, block_nnz;
+    DevBuf block_coords;   // decoded (block_row, block_col, entry offset, row_jump offset) per block
+    DevBuf band_table;     // per band: start, rows, beta_bits, first block, blocks
+    DevBuf tasks;          // per workgroup: band, first block, block count
+    int64_t n_tasks = 0, total_blocks = 0;
+    void clear() {
+        bands.clear();
+        for (DevBuf* b : {&row_jump, &col_jump, &data, &row_jump_block, &col_jump_block, &block_nnz, &block_coords,
+                          &band_table, &tasks})
+            b->release();
+        n_tasks = total_blocks = 0;
+    }
+};
+
+}  // namespace pwm_impl
+
+struct pwm_matrix {
+    int format = 0, vtype = 0, threads = 0;
+    int64_t nor = 0, noc = 0, nnz = 0;
+    bool loaded = false;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // CRS arrays (used by CRS and CRS_MERGE; CSB/BlockCOH keep them for their
+    // parity exports only when asked)
+    pwm_impl::DevBuf rs, ci, val;
+    // host copy of row_start (long-row planning)
+    std::vector<int32_t> h_rs;
+
+    // CRS vector kernel plan
+    int group = 0;          // 0 = auto
+    int vec_blocks = 0;     // 0 = auto
+    int64_t long_row = 0;   // 0 = auto
+    int chunk = 4096;
+    pwmk::LongRows lr;
+    pwm_impl::DevBuf lr_chunks, lr_rows, lr_begin, lr_partial;
+
+    // merge plan
+    int ipt = 8;
+    pwmk::MergePlan mp;
+    pwm_impl::DevBuf split_a, split_b, carry_row, carry_val;
+
+    // blocked formats
+    int32_t csb_bits_override = 0;
+    pwm_impl::CsbDev csb;
+    pwm_impl::CohDev coh;
+
+    // power-method workspace
+    pwm_impl::DevBuf pw_ctrl, pw_rec, pw_partials;
+    int64_t pw_rec_cap = 0;
+    // host-API staging vectors
+    pwm_impl::DevBuf hx, hy;
+    // timing flush buffer
+    pwm_impl::DevBuf flush;
+
+    bool autotune = false;
+};
+
+namespace pwm_impl {
+
+int fail(pwm_matrix* h, int code, const std::string& msg);
+int hip_fail(pwm_matrix* h, hipError_t e, const char* what);
+size_t value_size(int vtype);
+size_t vector_size(int vtype);
+
+// per-format planning after the CRS arrays are on the device
+int plan_crs(pwm_matrix* h);
+int plan_merge(pwm_matrix* h);
+int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals);
+int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals);
+// one SpMV of the loaded format on device vectors; pw != nullptr: power step
+int spmv_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw, int* n_partials_out);
+hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw);
+hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw);
+int64_t csb_partials(const pwm_matrix* h);
+int64_t coh_partials(const pwm_matrix* h);
+
+}  // namespace pwm_impl

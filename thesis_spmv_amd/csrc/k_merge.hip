@@ -1,0 +1,317 @@
+// Merge-path CRS SpMV for gfx950 (the GPU counterpart of CRS_Merge,
+// SOA_Implementations/CRS_Merge.hpp:95-141).
+//
+// The merge of list A = row_end_offsets (row_start+1) with list B = 0..Z-1 is
+// cut into P tiles with exactly the reference's formulas:
+// items = ceil((N+Z)/P), tile p covers diagonals [min(items*p, N+Z),
+// min(items*p + items, N+Z)), and each cut is found with searchPathOnDiag
+// (k_merge_splits, one thread per cut; the cuts are computed once at load
+// time and are bit-identical to the reference's).
+//
+// One 256-thread workgroup per tile:
+//   1. stage the tile's row ends and its products val[k]*x[col_ind[k]] in LDS
+//      (coalesced loads of val/col_ind, x gathered);
+//   2. every thread merges ITEMS consecutive items of the tile (its own
+//      merge-path search inside the tile), writing completed rows to LDS;
+//   3. a segmented inclusive scan over the threads' carries (key = row) adds
+//      the partial sums of rows that span threads;
+//   4. rows completed in the tile are written to y with coalesced stores, and
+//      the tile's trailing partial row is left as (carry_row, carry_val).
+// k_merge_fixup then adds the carries (the reference's serial fix-up,
+// :136-140): each run of equal carry rows is summed in tile order and added
+// once, so the result does not depend on scheduling.
+#include "kernels.hpp"
+
+namespace pwmk {
+
+constexpr int kMBS = 256;
+
+// searchPathOnDiag (CRS_Merge.hpp:42-62) for every cut p = 0..P.
+__global__ void k_merge_splits(int64_t nrows, int64_t nnz, const int32_t* __restrict__ rs, int64_t P, int64_t items,
+                               int64_t* __restrict__ sa, int64_t* __restrict__ sb) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > P) return;
+    const int64_t total = nrows + nnz;
+    const int64_t d = items * p < total ? items * p : total;
+    const int32_t* row_end = rs + 1;
+    int64_t lo = d > nnz ? d - nnz : 0;
+    int64_t hi = d < nrows ? d : nrows;
+    while (lo < hi) {
+        const int64_t piv = (lo + hi) >> 1;
+        if ((int64_t)row_end[piv] <= d - piv - 1)
+            lo = piv + 1;
+        else
+            hi = piv;
+    }
+    sa[p] = lo < nrows ? lo : nrows;
+    sb[p] = d - lo;
+}
+
+template <typename VT, int ITEMS, bool POWER>
+__global__ __launch_bounds__(kMBS) void k_merge_spmv(CrsDev A, MergePlan mp, const typename Arith<VT>::X* __restrict__ x_arg,
+                                                     typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    constexpr int CAP = kMBS * ITEMS;
+    __shared__ int32_t s_rowend[CAP];
+    __shared__ Acc s_prod[CAP];
+    __shared__ Acc s_out[CAP];
+    __shared__ int32_t s_key[kMBS];
+    __shared__ Acc s_val[kMBS];
+    __shared__ double red[16];
+
+    const X* __restrict__ x = x_arg;
+    X* __restrict__ y = y_arg;
+    double scale = 1.0;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = step_scale(pw, k);
+    }
+    const int32_t* __restrict__ rs = A.rs;
+    const int32_t* __restrict__ ci = A.ci;
+    const VT* __restrict__ val = (const VT*)A.val;
+    const int tid = threadIdx.x;
+
+    const int64_t p = xcd_remap(blockIdx.x, mp.P);
+    const int64_t a0 = mp.split_a[p], a1 = mp.split_a[p + 1];
+    const int64_t b0 = mp.split_b[p], b1 = mp.split_b[p + 1];
+    const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+
+    for (int i = tid; i < na; i += kMBS) s_rowend[i] = rs[a0 + 1 + i];
+    {
+        int i = tid;
+        for (; i + 3 * kMBS < nb; i += 4 * kMBS) {
+            const int64_t k = b0 + i;
+            const int32_t j0 = ci[k], j1 = ci[k + kMBS], j2 = ci[k + 2 * kMBS], j3 = ci[k + 3 * kMBS];
+            const Acc v0 = widen(val[k]), v1 = widen(val[k + kMBS]), v2 = widen(val[k + 2 * kMBS]),
+                      v3 = widen(val[k + 3 * kMBS]);
+            s_prod[i] = v0 * (Acc)x[j0];
+            s_prod[i + kMBS] = v1 * (Acc)x[j1];
+            s_prod[i + 2 * kMBS] = v2 * (Acc)x[j2];
+            s_prod[i + 3 * kMBS] = v3 * (Acc)x[j3];
+        }
+        for (; i < nb; i += kMBS) s_prod[i] = widen(val[b0 + i]) * (Acc)x[ci[b0 + i]];
+    }
+    __syncthreads();
+
+    // this thread's sub-range of the tile's merge items
+    const int items = na + nb;
+    const int d0 = tid * ITEMS < items ? tid * ITEMS : items;
+    const int d1 = d0 + ITEMS < items ? d0 + ITEMS : items;
+    int lo = d0 > nb ? d0 - nb : 0;
+    int hi = d0 < na ? d0 : na;
+    while (lo < hi) {
+        const int piv = (lo + hi) >> 1;
+        if ((int64_t)s_rowend[piv] <= b0 + d0 - piv - 1)
+            lo = piv + 1;
+        else
+            hi = piv;
+    }
+    int ai = lo, bi = d0 - lo;
+    const int a_start = ai;
+    Acc run = 0;
+    int first = -1;
+    Acc first_val = 0;
+    for (int d = d0; d < d1; ++d) {
+        if (ai < na && (int64_t)s_rowend[ai] <= b0 + bi) {
+            if (first < 0) {
+                first = ai;
+                first_val = run;
+            } else {
+                s_out[ai] = run;
+            }
+            run = 0;
+            ++ai;
+        } else {
+            run += s_prod[bi];
+            ++bi;
+        }
+    }
+
+    // segmented inclusive scan of (row, partial) over threads; keys ascend
+    s_key[tid] = ai;
+    s_val[tid] = run;
+    __syncthreads();
+#pragma unroll
+    for (int off = 1; off < kMBS; off <<= 1) {
+        Acc add = 0;
+        const bool take = tid >= off && s_key[tid - off] == s_key[tid];
+        if (take) add = s_val[tid - off];
+        __syncthreads();
+        if (take) s_val[tid] = add + s_val[tid];
+        __syncthreads();
+    }
+    if (first >= 0) {
+        const Acc carry_in = (tid > 0 && s_key[tid - 1] == a_start) ? s_val[tid - 1] : (Acc)0;
+        s_out[first] = carry_in + first_val;
+    }
+    __syncthreads();
+
+    // row a0 is unfinished here when it began in an earlier tile
+    const bool head_open = na > 0 && (int64_t)rs[a0] < b0;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = tid; i < na; i += kMBS) {
+        const Acc v = s_out[i];
+        if constexpr (POWER) {
+            const X out = (X)((double)v * scale);
+            y[a0 + i] = out;
+            if (!(i == 0 && head_open)) {
+                const double yd = (double)out, xd = (double)x[a0 + i] * scale;
+                yy += yd * yd;
+                xy += xd * yd;
+                xx += xd * xd;
+            }
+        } else {
+            y[a0 + i] = (X)v;
+        }
+    }
+    if (tid == kMBS - 1) {
+        mp.carry_row[p] = (int32_t)a1;
+        ((Acc*)mp.carry_val)[p] = s_val[kMBS - 1];
+    }
+    if constexpr (POWER) {
+        const double ra = block_sum<kMBS>(yy, red);
+        const double rb = block_sum<kMBS>(xy, red);
+        const double rc = block_sum<kMBS>(xx, red);
+        if (tid == 0) {
+            pw.partials[3 * p + 0] = ra;
+            pw.partials[3 * p + 1] = rb;
+            pw.partials[3 * p + 2] = rc;
+        }
+    }
+}
+
+// Carry fix-up.  A run of tiles p..q-1 carrying into the same row r is summed
+// in tile order and added to y[r] once (by the thread of the run's head).
+template <typename VT>
+__global__ void k_merge_fixup(int64_t nrows, MergePlan mp, typename Arith<VT>::X* __restrict__ y) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= mp.P) return;
+    const int32_t r = mp.carry_row[p];
+    if (r >= nrows || (p > 0 && mp.carry_row[p - 1] == r)) return;
+    const Acc* cv = (const Acc*)mp.carry_val;
+    Acc s = 0;
+    for (int64_t q = p; q < mp.P && mp.carry_row[q] == r; ++q) s += cv[q];
+    y[r] = (X)((Acc)y[r] + s);
+}
+
+constexpr int kFixBS = 1024;
+
+// Power mode: carry fix-up plus the FP64 reduction of the tiles' partials,
+// in one workgroup (fixed order), then step k's record and k+1.
+template <typename VT>
+__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(int64_t nrows, MergePlan mp, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    __shared__ double red[16];
+    if (pw.ctrl[1]) return;
+    const int k = pw.ctrl[0];
+    const X* __restrict__ x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+    X* __restrict__ y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+    const double scale = step_scale(pw, k);
+    const Acc* cv = (const Acc*)mp.carry_val;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int64_t p = threadIdx.x; p < mp.P; p += kFixBS) {
+        yy += pw.partials[3 * p + 0];
+        xy += pw.partials[3 * p + 1];
+        xx += pw.partials[3 * p + 2];
+        const int32_t r = mp.carry_row[p];
+        if (r >= nrows || (p > 0 && mp.carry_row[p - 1] == r)) continue;
+        Acc s = 0;
+        for (int64_t q = p; q < mp.P && mp.carry_row[q] == r; ++q) s += cv[q];
+        const X out = (X)((double)y[r] + (double)s * scale);
+        y[r] = out;
+        const double yd = (double)out, xd = (double)x[r] * scale;
+        yy += yd * yd;
+        xy += xd * yd;
+        xx += xd * xd;
+    }
+    const double a = block_sum<kFixBS>(yy, red);
+    const double b = block_sum<kFixBS>(xy, red);
+    const double c = block_sum<kFixBS>(xx, red);
+    if (threadIdx.x == 0) {
+        const double lam = b / c;
+        pw.rec[4 * k + 0] = a;
+        pw.rec[4 * k + 1] = b;
+        pw.rec[4 * k + 2] = c;
+        pw.rec[4 * k + 3] = lam;
+        int done = (k + 1 >= pw.max_it);
+        if (pw.tol >= 0.0 && k >= 1) {
+            const double prev = pw.rec[4 * (k - 1) + 3];
+            if (fabs(lam - prev) <= pw.tol * fabs(lam)) done = 1;
+        }
+        pw.ctrl[0] = k + 1;
+        pw.ctrl[1] = done;
+    }
+}
+
+hipError_t launch_merge_splits(const CrsDev& A, const MergePlan& mp, hipStream_t s) {
+    const int64_t n = mp.P + 1;
+    hipLaunchKernelGGL(k_merge_splits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, A.nrows, A.nnz, A.rs, mp.P,
+                       mp.items_per_tile, mp.split_a, mp.split_b);
+    return hipGetLastError();
+}
+
+template <typename VT, int ITEMS>
+static hipError_t launch_tiles(const CrsDev& A, const MergePlan& mp, const void* x, void* y, const PowerArgs* pw,
+                               hipStream_t s) {
+    using X = typename Arith<VT>::X;
+    if (pw)
+        hipLaunchKernelGGL((k_merge_spmv<VT, ITEMS, true>), dim3((unsigned)mp.P), dim3(kMBS), 0, s, A, mp,
+                           (const X*)nullptr, (X*)nullptr, *pw);
+    else
+        hipLaunchKernelGGL((k_merge_spmv<VT, ITEMS, false>), dim3((unsigned)mp.P), dim3(kMBS), 0, s, A, mp,
+                           (const X*)x, (X*)y, PowerArgs{});
+    return hipGetLastError();
+}
+
+template <typename VT>
+static hipError_t launch_tiles_t(const CrsDev& A, const MergePlan& mp, const void* x, void* y, const PowerArgs* pw,
+                                 hipStream_t s) {
+    switch (mp.ipt) {
+        case 4: return launch_tiles<VT, 4>(A, mp, x, y, pw, s);
+        case 8: return launch_tiles<VT, 8>(A, mp, x, y, pw, s);
+        case 12: return launch_tiles<VT, 12>(A, mp, x, y, pw, s);
+        case 16: return launch_tiles<VT, 16>(A, mp, x, y, pw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_merge_spmv(const CrsDev& A, const MergePlan& mp, const void* x, void* y, const PowerArgs* pw,
+                             hipStream_t s) {
+    if (mp.P <= 0) return hipSuccess;
+    switch (A.vtype) {
+        case VT_F64: return launch_tiles_t<double>(A, mp, x, y, pw, s);
+        case VT_F32: return launch_tiles_t<float>(A, mp, x, y, pw, s);
+        case VT_BF16: return launch_tiles_t<bf16_t>(A, mp, x, y, pw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename VT>
+static hipError_t launch_fix_t(const CrsDev& A, const MergePlan& mp, void* y, const PowerArgs* pw, hipStream_t s) {
+    using X = typename Arith<VT>::X;
+    if (pw)
+        hipLaunchKernelGGL((k_merge_fixup_power<VT>), dim3(1), dim3(kFixBS), 0, s, A.nrows, mp, *pw);
+    else
+        hipLaunchKernelGGL((k_merge_fixup<VT>), dim3((unsigned)((mp.P + 255) / 256)), dim3(256), 0, s, A.nrows, mp,
+                           (X*)y);
+    return hipGetLastError();
+}
+
+hipError_t launch_merge_fixup(const CrsDev& A, const MergePlan& mp, void* y, const PowerArgs* pw, hipStream_t s) {
+    if (mp.P <= 0) return hipSuccess;
+    switch (A.vtype) {
+        case VT_F64: return launch_fix_t<double>(A, mp, y, pw, s);
+        case VT_F32: return launch_fix_t<float>(A, mp, y, pw, s);
+        case VT_BF16: return launch_fix_t<bf16_t>(A, mp, y, pw, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pwmk

@@ -1,0 +1,163 @@
+// Utility kernels: device Poisson generator, value conversion, vector fills
+// and copies, the power method's final normalisation, cache flush.
+#include "kernels.hpp"
+
+namespace pwmk {
+
+__device__ __forceinline__ void store_val(void* val, int64_t k, double v, int vtype) {
+    if (vtype == VT_F64)
+        ((double*)val)[k] = v;
+    else if (vtype == VT_F32)
+        ((float*)val)[k] = (float)v;
+    else
+        ((uint16_t*)val)[k] = (uint16_t)(__float_as_uint((float)v) >> 16);  // exact for 4 and -1
+}
+
+// Util/Poisson.hpp:32-83, one thread per row.  The row's first entry offset
+// is the closed-form count of entries in rows q < r (one per row, plus
+// q >= m, q % m != 0, q % m != m-1 and q < N-m), the same decomposition the
+// reference's fillPoissonOMP uses to fill rows independently (:111-122).
+__global__ void k_fill_poisson(int64_t m, int64_t n, int32_t* __restrict__ rs, int32_t* __restrict__ ci,
+                               void* __restrict__ val, int vtype) {
+    const int64_t N = m * n;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const int64_t below_m = r > m ? r - m : 0;              // q < r with q >= m
+    const int64_t not_first = r - (r + m - 1) / m;          // q < r with q % m != 0
+    const int64_t not_last = r - r / m;                     // q < r with q % m != m-1
+    const int64_t before_last_block = r < N - m ? r : N - m;  // q < r with q < N-m
+    int64_t k = r + below_m + not_first + not_last + before_last_block;
+    if (r == 0) rs[0] = 0;
+    if (r >= m) { ci[k] = (int32_t)(r - m); store_val(val, k++, -1.0, vtype); }
+    if (r % m != 0) { ci[k] = (int32_t)(r - 1); store_val(val, k++, -1.0, vtype); }
+    ci[k] = (int32_t)r; store_val(val, k++, 4.0, vtype);
+    if (r % m != m - 1) { ci[k] = (int32_t)(r + 1); store_val(val, k++, -1.0, vtype); }
+    if (r < N - m) { ci[k] = (int32_t)(r + m); store_val(val, k++, -1.0, vtype); }
+    rs[r + 1] = (int32_t)k;
+}
+
+hipError_t launch_fill_poisson(int64_t m, int64_t n, int32_t* rs, int32_t* ci, void* val, int vtype, hipStream_t s) {
+    const int64_t N = m * n;
+    hipLaunchKernelGGL(k_fill_poisson, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, m, n, rs, ci, val, vtype);
+    return hipGetLastError();
+}
+
+__global__ void k_convert(const double* __restrict__ src, void* __restrict__ dst, int64_t n, int vtype) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        store_val(dst, i, src[i], vtype);
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (unsigned)(g < 65536 ? (g > 0 ? g : 1) : 65536);
+}
+
+hipError_t launch_convert_values(const double* src, void* dst, int64_t n, int vtype, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_convert, dim3(grid_for(n)), dim3(256), 0, s, src, dst, n, vtype);
+    return hipGetLastError();
+}
+
+__global__ void k_widen(const void* __restrict__ src, int vtype, double* __restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (vtype == VT_F64)
+            dst[i] = ((const double*)src)[i];
+        else if (vtype == VT_F32)
+            dst[i] = (double)((const float*)src)[i];
+        else
+            dst[i] = (double)__uint_as_float(((uint32_t)((const uint16_t*)src)[i]) << 16);
+    }
+}
+
+hipError_t launch_copy_values_f64(const void* src, int vtype, double* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(n)), dim3(256), 0, s, src, vtype, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(void* p, int64_t n, int is_f64, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (is_f64)
+            ((double*)p)[i] = v;
+        else
+            ((float*)p)[i] = (float)v;
+    }
+}
+
+hipError_t launch_fill(void* p, int64_t n, int is_f64, double v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, n, is_f64, v);
+    return hipGetLastError();
+}
+
+__global__ void k_copy_vec(const void* src, int src_f64, void* dst, int dst_f64, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = src_f64 ? ((const double*)src)[i] : (double)((const float*)src)[i];
+        if (dst_f64)
+            ((double*)dst)[i] = v;
+        else
+            ((float*)dst)[i] = (float)v;
+    }
+}
+
+hipError_t launch_copy_vec(const void* src, int src_f64, void* dst, int dst_f64, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_vec, dim3(grid_for(n)), dim3(256), 0, s, src, src_f64, dst, dst_f64, n);
+    return hipGetLastError();
+}
+
+// After K = ctrl[0] steps, buf[K&1] holds A*x_{K-1} (not yet divided by its
+// norm) and buf[(K-1)&1] holds A*x_{K-2} (or the caller's start vector when
+// K == 1).  Divide each by its own norm so both buffers hold what the
+// reference's powerIteration leaves in x and y (Matrix/CRS.hpp:118-130).
+__global__ void k_power_finish(PowerArgs pw, int64_t n, int is_f64) {
+    const int K = pw.ctrl[0];
+    if (K <= 0) return;
+    void* last = (K & 1) ? pw.buf1 : pw.buf0;
+    void* prev = (K & 1) ? pw.buf0 : pw.buf1;
+    const double s_last = 1.0 / sqrt(pw.rec[4 * (K - 1)]);
+    const double s_prev = K >= 2 ? 1.0 / sqrt(pw.rec[4 * (K - 2)]) : 1.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (is_f64) {
+            ((double*)last)[i] = ((double*)last)[i] * s_last;
+            if (K >= 2) ((double*)prev)[i] = ((double*)prev)[i] * s_prev;
+        } else {
+            ((float*)last)[i] = (float)((double)((float*)last)[i] * s_last);
+            if (K >= 2) ((float*)prev)[i] = (float)((double)((float*)prev)[i] * s_prev);
+        }
+    }
+}
+
+hipError_t launch_power_finish(const PowerArgs& pw, int64_t n, int is_f64, int64_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_power_finish, dim3(grid_for(n)), dim3(256), 0, s, pw, n, is_f64);
+    return hipGetLastError();
+}
+
+__global__ void k_power_reset(int* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl[0] = 0;
+        ctrl[1] = 0;
+    }
+}
+
+hipError_t launch_power_reset(const PowerArgs& pw, hipStream_t s) {
+    hipLaunchKernelGGL(k_power_reset, dim3(1), dim3(64), 0, s, pw.ctrl);
+    return hipGetLastError();
+}
+
+__global__ void k_flush(uint4* p, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
+        uint4 v = p[i];
+        v.x += 1u;
+        p[i] = v;
+    }
+}
+
+hipError_t launch_flush(void* p, int64_t bytes, hipStream_t s) {
+    const int64_t n16 = bytes / 16;
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flush, dim3(4096), dim3(256), 0, s, (uint4*)p, n16);
+    return hipGetLastError();
+}
+
+}  // namespace pwmk

@@ -81,6 +81,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="powerlaw")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
+    ap.add_argument("--formats", default="crs_vector,merge_path")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="bytes: run the library's streaming copy kernel once first (profiler byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -88,6 +92,13 @@ def main():
         sys.exit(2)
 
     import thesis_spmv_amd as pw
+    if args.calibrate_copy > 0:
+        nb = args.calibrate_copy // 16 * 16
+        src, dst = pw.DeviceVector(nb // 8), pw.DeviceVector(nb // 8)
+        src.fill(1.0)
+        pw._check(pw.lib().pwm_device_copy(dst.ptr, src.ptr, nb))
+        print(f"# calibration copy: {nb} bytes read and written once", file=sys.stderr, flush=True)
+        del src, dst
     n, rs, ci, dv, desc, gen_s = make_workload(args.workload)
     z = int(rs[-1])
     x_host = pw.gen_uniform(n, 0)
@@ -98,7 +109,10 @@ def main():
     x = pw.DeviceVector.from_array(x_host)
     y = pw.DeviceVector(n)
     results = {}
+    wanted = set(args.formats.split(","))
     for name, cls in (("crs_vector", pw.CRSHip), ("merge_path", pw.CRSMergeHip)):
+        if name not in wanted:
+            continue
         m = cls(-1, "f64")
         m.loadCRS(n, n, rs, ci, dv)
         for _ in range(args.warmup):
@@ -108,7 +122,7 @@ def main():
         ms_ev, kms = m.time_mv(x, y, args.steps)
         pw.synchronize()
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        ms_flush, kms_flush = m.time_mv(x, y, 10, flush_bytes=512 << 20)
+        ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
         inf = m.info()
         results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
                          "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush,
@@ -146,7 +160,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "formats": {k: {"GBs_wall": round(v["GBs"], 1), "us_per_step": round(v["ms_per_step"] * 1e3, 2),
                         "dominant_kernel_us": round(v["kernel_ms"] * 1e3, 2),
-                        "dominant_kernel_us_flushed": round(v["kernel_ms_flushed"] * 1e3, 2),
+                        "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
+                                                       else round(v["kernel_ms_flushed"] * 1e3, 2)),
                         "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
                         "long_rows": v["info"]["long_rows"]} for k, v in results.items()},
     }

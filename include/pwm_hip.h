@@ -162,6 +162,10 @@ int pwm_device_synchronize(void);
 /* Fill n entries of a device vector (double when is_f64, else float) with a
  * constant. */
 int pwm_device_fill(void* ptr, int64_t n, int is_f64, double value);
+/* Copy `bytes` (a multiple of 16) between device buffers with the library's
+ * own streaming kernel: reads and writes exactly `bytes` each, the reference
+ * point for converting profiler byte counters. */
+int pwm_device_copy(void* dst, const void* src, int64_t bytes);
 
 /* Timing (addition, used by bench.py): run `reps` SpMVs on the handle's
  * stream between two HIP events; with flush_bytes > 0 a buffer of that many

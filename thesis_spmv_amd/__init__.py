@@ -62,6 +62,7 @@ _SIGS = {
     "pwm_memcpy_d2h": [VP, VP, C.c_size_t],
     "pwm_device_synchronize": [],
     "pwm_device_fill": [VP, I64, C.c_int, F64],
+    "pwm_device_copy": [VP, VP, I64],
     "pwm_time_mv": [VP, VP, VP, C.c_int, I64, C.POINTER(F64), C.POINTER(F64)],
     "pwm_gen_powerlaw": [I64, F64, F64, I64, C.c_uint64, C.c_uint64, C.POINTER(VP), C.POINTER(VP),
                          C.POINTER(VP), C.POINTER(I64)],

@@ -702,6 +702,13 @@ extern "C" int pwm_device_fill(void* ptr, int64_t n, int is_f64, double value) {
     return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_device_fill");
 }
 
+extern "C" int pwm_device_copy(void* dst, const void* src, int64_t bytes) {
+    if (!dst || !src || bytes < 0 || bytes % 16) return fail(nullptr, PWM_ERR_INVALID, "pwm_device_copy: bad argument");
+    hipError_t e = pwmk::launch_stream_copy(dst, src, bytes, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e == hipSuccess ? PWM_OK : hip_fail(nullptr, e, "pwm_device_copy");
+}
+
 // ---------------------------------------------------------------- timing
 
 extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int64_t flush_bytes, double* ms_per_mv,

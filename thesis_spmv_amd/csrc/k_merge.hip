@@ -205,7 +205,7 @@ constexpr int kFixBS = 1024;
 // Power mode: carry fix-up plus the FP64 reduction of the tiles' partials,
 // in one workgroup (fixed order), then step k's record and k+1.
 template <typename VT>
-__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(int64_t nrows, MergePlan mp, PowerArgs pw) {
+__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(CrsDev A, MergePlan mp, PowerArgs pw) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
     __shared__ double red[16];
@@ -215,6 +215,7 @@ __global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(int64_t nrows, Mer
     X* __restrict__ y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
     const double scale = step_scale(pw, k);
     const Acc* cv = (const Acc*)mp.carry_val;
+    const int64_t nrows = A.nrows;
     double yy = 0.0, xy = 0.0, xx = 0.0;
     for (int64_t p = threadIdx.x; p < mp.P; p += kFixBS) {
         yy += pw.partials[3 * p + 0];
@@ -223,7 +224,12 @@ __global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(int64_t nrows, Mer
         const int32_t r = mp.carry_row[p];
         if (r >= nrows || (p > 0 && mp.carry_row[p - 1] == r)) continue;
         Acc s = 0;
-        for (int64_t q = p; q < mp.P && mp.carry_row[q] == r; ++q) s += cv[q];
+        int64_t q = p;
+        for (; q < mp.P && mp.carry_row[q] == r; ++q) s += cv[q];
+        // Tile q completes row r.  If row r starts at or after that tile's
+        // first nonzero, the carries are empty (a tile ended exactly where
+        // row r begins) and tile q already counted the row: nothing to do.
+        if ((int64_t)A.rs[r] >= mp.split_b[q]) continue;
         const X out = (X)((double)y[r] + (double)s * scale);
         y[r] = out;
         const double yd = (double)out, xd = (double)x[r] * scale;
@@ -297,7 +303,7 @@ template <typename VT>
 static hipError_t launch_fix_t(const CrsDev& A, const MergePlan& mp, void* y, const PowerArgs* pw, hipStream_t s) {
     using X = typename Arith<VT>::X;
     if (pw)
-        hipLaunchKernelGGL((k_merge_fixup_power<VT>), dim3(1), dim3(kFixBS), 0, s, A.nrows, mp, *pw);
+        hipLaunchKernelGGL((k_merge_fixup_power<VT>), dim3(1), dim3(kFixBS), 0, s, A, mp, *pw);
     else
         hipLaunchKernelGGL((k_merge_fixup<VT>), dim3((unsigned)((mp.P + 255) / 256)), dim3(256), 0, s, A.nrows, mp,
                            (X*)y);

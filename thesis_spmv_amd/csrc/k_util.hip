@@ -145,6 +145,18 @@ hipError_t launch_power_reset(const PowerArgs& pw, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void k_stream_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_stream_copy(void* dst, const void* src, int64_t bytes, hipStream_t s) {
+    const int64_t n16 = bytes / 16;
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
+    return hipGetLastError();
+}
+
 __global__ void k_flush(uint4* p, int64_t n16) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x) {
         uint4 v = p[i];

@@ -77,5 +77,7 @@ hipError_t launch_power_finish(const PowerArgs& pw, int64_t n, int is_f64, int64
 hipError_t launch_power_reset(const PowerArgs& pw, hipStream_t s);
 // Rewrite a buffer (Infinity-Cache flush for the roofline timing).
 hipError_t launch_flush(void* p, int64_t bytes, hipStream_t s);
+// Plain streaming copy (profiler counter calibration).
+hipError_t launch_stream_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
 
 }  // namespace pwmk

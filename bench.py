@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--workload", default="powerlaw")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
-    ap.add_argument("--formats", default="crs_vector,merge_path")
+    ap.add_argument("--formats", default="crs_vector,merge_path,csb,blockcoh")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes: run the library's streaming copy kernel once first (profiler byte-counter reference)")
     args = ap.parse_args()
@@ -110,11 +110,16 @@ def main():
     y = pw.DeviceVector(n)
     results = {}
     wanted = set(args.formats.split(","))
-    for name, cls in (("crs_vector", pw.CRSHip), ("merge_path", pw.CRSMergeHip)):
+    formats = (("crs_vector", pw.CRSHip, {}), ("merge_path", pw.CRSMergeHip, {}),
+               ("csb", pw.CSBHip, {"block_bits": 14} if args.workload == "stencil9" else {}),
+               ("blockcoh", pw.BlockCOHHip, {}))
+    for name, cls, opts in formats:
         if name not in wanted:
             continue
-        m = cls(-1, "f64")
+        t_load = time.time()
+        m = cls(-1, "f64", **opts)
         m.loadCRS(n, n, rs, ci, dv)
+        t_load = time.time() - t_load
         for _ in range(args.warmup):
             m.time_mv(x, y, 1)
         pw.synchronize()
@@ -124,9 +129,12 @@ def main():
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
         inf = m.info()
+        # parity spot check of this run's result against the first format's
+        yv = y.download()
         results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
                          "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush,
-                         "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf}
+                         "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf, "load_s": t_load,
+                         "y_checksum": float(np.sum(np.abs(yv)))}
         print(f"# {name}: wall {wall_ms * 1e3:.1f} us/step, events {ms_ev * 1e3:.1f} us, dominant kernel "
               f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us",
               file=sys.stderr, flush=True)
@@ -163,7 +171,9 @@ def main():
                         "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
                                                        else round(v["kernel_ms_flushed"] * 1e3, 2)),
                         "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
-                        "long_rows": v["info"]["long_rows"]} for k, v in results.items()},
+                        "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
+                        "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"]}
+                    for k, v in results.items()},
     }
     if not args.no_cpu:
         try:

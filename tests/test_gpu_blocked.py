@@ -1,0 +1,214 @@
+"""GPU parity of the blocked formats (CSBHip, BlockCOHHip) against the oracle.
+
+Bars: stored layouts (CSB ind/data/blk_ptr and balanced flag; BlockCOH band
+partition, Hilbert block order, ICRS/BICRS arrays) bit-exact with the oracle's
+restatement of the reference builders; FP64 y within 1e-13 relative max-norm;
+golden vectors at the reference's tolerances."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import boost_close, load_case_matrix
+
+pw = pytest.importorskip("thesis_spmv_amd")
+pytestmark = pytest.mark.gpu
+
+FORMATS = [pw.CSBHip, pw.BlockCOHHip]
+
+
+def _relerr(a, b):
+    m = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (m if m > 0 else 1.0))
+
+
+def _load(cls, A, threads=-1, values="f64", **opt):
+    m = cls(threads, values, **opt)
+    m.loadCRS(A.nor, A.noc, A.row_start, A.col_ind, A.data)
+    return m
+
+
+def _triplets_of(A):
+    rows = np.repeat(np.arange(A.nor, dtype=np.int32), np.diff(A.row_start))
+    return oracle.Triplets(A.nor, A.noc, rows, A.col_ind, A.data)
+
+
+def _powerlaw(n, seed=1, cap=50000):
+    rs, ci, dv = pw.gen_powerlaw(n, struct_seed=seed, value_seed=seed + 1, cap=cap)
+    return oracle.CRSMatrix(n, n, rs, ci, dv)
+
+
+def _stencil(side, extra=2, seed=3):
+    rs, ci, dv = pw.gen_stencil9(side, extra, seed)
+    return oracle.CRSMatrix(side * side, side * side, rs, ci, dv)
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+@pytest.mark.parametrize("idx", range(24))
+def test_golden(goldens, cls, idx):
+    case = goldens[idx]
+    A, t = load_case_matrix(case)
+    print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
+    for threads in ((1, 2, 3, 4, 8, -1) if cls is pw.BlockCOHHip else (-1,)):
+        m = cls(threads)
+        if case["matrix"]["kind"] == "poisson":
+            m.generatePoissonMatrix(case["matrix"]["m"], case["matrix"]["n"], 1)
+        else:
+            m.loadFromTriplets(t, 1)
+        x = np.ones(A.nor)
+        y = np.zeros(A.nor)
+        if case["op"] == "mv":
+            m.mv(x, y)
+            got = y
+        else:
+            m.powerIteration(x, y, case["iterations"])
+            got = x if case["checks"] == "x" else y
+        ok = boost_close(got, case["expected"], case["tolerance"])
+        assert ok.all(), (case["name"], threads, np.flatnonzero(~ok)[:5])
+
+
+# ---------------------------------------------------------------- layouts
+
+@pytest.mark.parametrize("bits", [0, 2, 3, 5])
+def test_csb_layout_bit_exact(bits):
+    for A in (oracle.poisson_crs(9, 7), oracle.poisson_crs(40, 31), _powerlaw(3000, cap=400)):
+        m = _load(pw.CSBHip, A, block_bits=bits)
+        gb, gind, gdata, gblk = m.csb_layout()
+        rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), bits)
+        assert gb == rb
+        assert np.array_equal(gblk, rblk) and np.array_equal(gind, rind) and np.array_equal(gdata, rdata)
+        assert m.info()["balanced"] == rbal
+
+
+def test_csb_layout_duplicates_and_unbalanced():
+    rng = np.random.default_rng(4)
+    N = 700
+    r = np.concatenate([rng.integers(0, 40, 3000), rng.integers(0, N, 500)]).astype(np.int32)
+    c = rng.integers(0, N, 3500).astype(np.int32)
+    r[:50], c[:50] = 3, 7  # duplicates
+    v = rng.integers(1, 5, 3500).astype(np.float64)
+    t = oracle.Triplets(N, N, r, c, v)
+    A = oracle.triplet_to_crs(t)
+    m = pw.CSBHip()
+    m.loadFromTriplets(t)
+    gb, gind, gdata, gblk = m.csb_layout()
+    rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), 0)
+    assert np.array_equal(gblk, rblk) and np.array_equal(gind, rind) and np.array_equal(gdata, rdata)
+    assert m.info()["balanced"] == rbal == 0
+    y = np.zeros(N)
+    x = rng.standard_normal(N)
+    m.mv(x, y)
+    assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13
+
+
+@pytest.mark.parametrize("threads", [1, 2, 3, 8, 64, -1])
+def test_blockcoh_layout_bit_exact(threads):
+    for A in (oracle.poisson_crs(40, 31), _powerlaw(5000, cap=600), _stencil(60)):
+        m = _load(pw.BlockCOHHip, A, threads)
+        P = m.info()["threads"]
+        if threads > 0:
+            ref = oracle.blockcoh_build(A, threads)
+            assert P == len(ref)
+        else:
+            ref = oracle.blockcoh_build(A, P)
+        for p in range(P):
+            g, o = m.band(p), ref[p]
+            for k in ("nnz", "blocks", "start", "rows", "beta_bits", "horizontal_blocks", "vertical_blocks"):
+                assert g[k] == o[k], (p, k)
+            for k in ("row_jump", "col_jump", "data", "row_jump_block", "col_jump_block", "block_nnz"):
+                assert np.array_equal(g[k], o[k]), (p, k)
+
+
+# ---------------------------------------------------------------- SpMV values
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_spmv_random_and_stencil(cls):
+    cases = [_powerlaw(20000, cap=5000), _stencil(150), oracle.poisson_crs(200, 150)]
+    for A in cases:
+        x = pw.gen_uniform(A.noc, 0)
+        ref = oracle.crs_mv(A, x)
+        for opt in ({}, {"block_bits": 7}) if cls is pw.CSBHip else ({}, ):
+            for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
+                m = _load(cls, A, threads, **opt)
+                y = np.zeros(A.nor)
+                m.mv(x, y)
+                assert _relerr(y, ref) <= 1e-13, (A.nor, opt, threads)
+
+
+def test_blockcoh_wide_bands_several_windows():
+    """Bands with more rows than one workgroup's LDS window (4096 FP64 rows)."""
+    A = _powerlaw(30000, cap=2000)
+    x = pw.gen_uniform(A.noc, 1)
+    m = _load(pw.BlockCOHHip, A, 2)
+    assert max(m.band(p)["rows"] for p in range(2)) > 4096
+    y = np.zeros(A.nor)
+    m.mv(x, y)
+    assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_edge_shapes(cls):
+    rng = np.random.default_rng(0)
+    cases = [oracle.CRSMatrix(1, 1, [0, 1], [0], [2.5]),
+             oracle.CRSMatrix(40, 40, [0] * 41, [], []),
+             oracle.CRSMatrix(9, 9000, [0, 0, 0, 4000, 4000, 4001, 4001, 4001, 4001, 4001],
+                              np.sort(rng.integers(0, 9000, 4001)), rng.standard_normal(4001)),
+             oracle.CRSMatrix(900, 2, np.arange(901) // 3, np.zeros(300, np.int32), np.ones(300))]
+    for A in cases:
+        x = rng.standard_normal(A.noc)
+        ref = oracle.crs_mv(A, x)
+        m = _load(cls, A)
+        y = np.full(A.nor, np.nan)
+        m.mv(x, y)
+        assert _relerr(y, ref) <= 1e-13
+
+
+def test_config2_full_size():
+    """9-point stencil 2896^2 + 2 random off-band per row (~92M nnz), FP64,
+    CSB at beta = 2^14 (the workload's setting) and the reference rule (2^15),
+    BlockCOH with its default bands; and the FP32-value run with its error."""
+    A = _stencil(2896)
+    assert A.nnz == (3 * 2896 - 2) ** 2 + 2 * 2896 * 2896
+    x = pw.gen_uniform(A.noc, 0)
+    ref = oracle.crs_mv(A, x)
+    for cls, opt in ((pw.CSBHip, {"block_bits": 14}), (pw.CSBHip, {}), (pw.BlockCOHHip, {})):
+        m = _load(cls, A, **opt)
+        y = np.zeros(A.nor)
+        m.mv(x, y)
+        err = _relerr(y, ref)
+        print(f"{cls.__name__} {opt} config-2 FP64 rel.err {err:.3e} info {m.info()}", flush=True)
+        assert err <= 1e-13
+        del m
+    for cls in FORMATS:
+        m = _load(cls, A, values="f32")
+        y = np.zeros(A.nor)
+        m.mv(x, y)
+        err = _relerr(y, ref)
+        print(f"{cls.__name__} config-2 FP32 values/accumulation rel.err vs FP64 oracle {err:.3e}", flush=True)
+        assert err <= 1e-5
+        del m
+
+
+# ---------------------------------------------------------------- power method
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+@pytest.mark.parametrize("it", [1, 2, 7, 50])
+def test_power_iteration_state(cls, it):
+    A = _powerlaw(6000, cap=700)
+    A.data[:] = np.abs(A.data)
+    x0 = pw.gen_uniform(A.nor, 3)
+    xr, yr = oracle.crs_power_iteration(A, x0, it)
+    m = _load(cls, A)
+    x, y = x0.copy(), np.zeros(A.nor)
+    m.powerIteration(x, y, it)
+    assert _relerr(x, xr) <= 1e-12 and _relerr(y, yr) <= 1e-12
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_power_method_eigenvalue_poisson(cls):
+    m = cls()
+    m.generatePoissonMatrix(20, 17, 1)
+    x = pw.DeviceVector.from_array(pw.gen_uniform(340, 5))
+    y = pw.DeviceVector(340)
+    lam, _, its = m.power_method_device(x, y, 20000, 1e-13)
+    exact = 4 + 2 * np.cos(np.pi / 21) + 2 * np.cos(np.pi / 18)
+    assert its < 20000 and abs(lam - exact) <= 1e-10 * exact

@@ -270,10 +270,41 @@ class CSBHip(SparseMatrixHip):
     """Compressed sparse blocks (beside CSB, SOA_Implementations/CSB.hpp)."""
     FORMAT = FORMAT_CSB
 
+    def csb_layout(self):
+        """(block_bits, ind, data, blk_ptr) as stored on the device."""
+        bits, nb = I32(), I64()
+        _check(lib().pwm_csb_layout(self.h, C.byref(bits), C.byref(nb), None, None, None), self.h)
+        z = self.info()["nnz"]
+        ind = np.zeros(max(z, 1), np.uint32)
+        data = np.zeros(max(z, 1), np.float64)
+        blk = np.zeros(nb.value + 1, np.int32)
+        _check(lib().pwm_csb_layout(self.h, C.byref(bits), C.byref(nb), _ptr(ind), _ptr(data), _ptr(blk)), self.h)
+        return bits.value, ind[:z], data[:z], blk
+
 
 class BlockCOHHip(SparseMatrixHip):
     """Row-distributed block CO-H (beside BlockCOH, SOA_Implementations/BlockCOH.hpp)."""
     FORMAT = FORMAT_BLOCKCOH
+
+    def band(self, p):
+        """Band p's stored arrays (BlockCOH.hpp:27-67) and sizes."""
+        counts = np.zeros(6, np.int64)
+        meta = np.zeros(3, np.int32)
+        L = lib()
+        _check(L.pwm_blockcoh_band(self.h, int(p), _ptr(counts), _ptr(meta), None, None, None, None, None, None),
+               self.h)
+        nnz, blocks, rj, rjb, start, rows = (int(v) for v in counts)
+        a = {"row_jump": np.zeros(max(rj, 1), np.uint16), "col_jump": np.zeros(max(nnz, 1), np.uint16),
+             "data": np.zeros(max(nnz, 1), np.float64), "row_jump_block": np.zeros(max(rjb, 1), np.int16),
+             "col_jump_block": np.zeros(max(blocks, 1), np.int16), "block_nnz": np.zeros(max(blocks, 1), np.int32)}
+        _check(L.pwm_blockcoh_band(self.h, int(p), _ptr(counts), _ptr(meta), _ptr(a["row_jump"]),
+                                   _ptr(a["col_jump"]), _ptr(a["data"]), _ptr(a["row_jump_block"]),
+                                   _ptr(a["col_jump_block"]), _ptr(a["block_nnz"])), self.h)
+        return {"nnz": nnz, "blocks": blocks, "start": start, "rows": rows, "beta_bits": int(meta[0]),
+                "horizontal_blocks": int(meta[1]), "vertical_blocks": int(meta[2]),
+                "row_jump": a["row_jump"][:rj], "col_jump": a["col_jump"][:nnz], "data": a["data"][:nnz],
+                "row_jump_block": a["row_jump_block"][:rjb], "col_jump_block": a["col_jump_block"][:blocks],
+                "block_nnz": a["block_nnz"][:blocks]}
 
 
 GPU_METHODS = {101: CRSHip, 102: CRSHip, 108: CRSMergeHip, 109: CSBHip, 110: BlockCOHHip}

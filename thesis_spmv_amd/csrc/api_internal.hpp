@@ -65,6 +65,8 @@ struct CsbDev {
     DevBuf work;                 // per-workgroup tasks
     int64_t n_tasks = 0;
     int32_t slab_bits = 0;
+    int32_t slabs = 1, pieces_per_block = 1;
+    int64_t slab_rows = 0;
     void clear() {
         bits = 0;
         H = V = 0;
@@ -90,7 +92,7 @@ struct CohDev {
     DevBuf row_jump, col_jump, data, row_jump_block, col_jump_block, block_nnz;
     DevBuf block_coords;   // decoded (block_row, block_col, entry offset, row_jump offset) per block
     DevBuf band_table;     // per band: start, rows, beta_bits, first block, blocks
-    DevBuf tasks;          // per workgroup: band, first block, block count
+    DevBuf tasks;          // per band: chunk -> first block table
     int64_t n_tasks = 0, total_blocks = 0;
     void clear() {
         bands.clear();

@@ -1,0 +1,71 @@
+// Device-side descriptions of the CSB and BlockCOH layouts and the launchers
+// of their kernels (k_blocked.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "k_common.hpp"
+
+namespace blocked {
+
+// Rows of y one workgroup accumulates in LDS (FP64: 32 KB, FP32: 32 KB).
+inline int lds_rows(int vtype) { return vtype == 0 ? 4096 : 8192; }
+constexpr int kLdsRowsF64 = 4096;
+constexpr int kLdsRowsF32 = 8192;
+
+// A contiguous run [start, end) of CSB entries: the part of one block that
+// falls into one workgroup's row slab.
+struct Piece {
+    int32_t start, end;
+};
+
+struct CsbArgs {
+    int64_t nor = 0;
+    int32_t bits = 0;        // beta = 2^bits
+    int64_t H = 0;           // block columns
+    int32_t slabs = 1;       // slabs per block row
+    int64_t slab_rows = 0;   // beta / slabs
+    int32_t np = 1;          // pieces per (block, slab)
+    const uint32_t* ind = nullptr;
+    const void* val = nullptr;
+    const Piece* pieces = nullptr;  // [task][bc][piece]
+    int64_t n_tasks = 0;
+    int vtype = 0;
+};
+
+// One BlockCOH block (offsets are relative to its band's arrays).
+struct CohBlock {
+    int32_t entry;      // first entry (col_jump / data index)
+    int32_t row_jump;   // first row_jump index
+    int32_t brow, bcol; // block coordinates inside the band
+};
+
+// Workgroup size and entries per thread of the blocked kernels; a BlockCOH
+// workgroup decodes kCohChunk consecutive entries of its band per step.
+constexpr int kBlockedBS = 512;
+constexpr int kCohIPT = 4;
+constexpr int kCohChunk = kBlockedBS * kCohIPT;
+
+struct CohBand {
+    int32_t start, rows, beta_bits, first_block, blocks, entry, nnz;
+    int32_t chunk_table;  // offset of the band's chunk -> first-block table
+};
+
+struct CohArgs {
+    int64_t nor = 0;
+    int64_t n_bands = 0;
+    const CohBand* bands = nullptr;
+    const CohBlock* blocks = nullptr;
+    const int32_t* chunk_block = nullptr;  // per band: block holding entry c*kCohChunk, c = 0..chunks
+    const uint16_t* col_jump = nullptr;
+    const uint16_t* row_jump = nullptr;
+    const void* val = nullptr;
+    int vtype = 0;
+};
+
+hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
+hipError_t launch_coh(const CohArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
+// FP64 reduction of the per-workgroup power-method partials (one workgroup).
+hipError_t launch_blocked_finalize(int vtype, int n_partials, const pwmk::PowerArgs& pw, hipStream_t s);
+
+}  // namespace blocked

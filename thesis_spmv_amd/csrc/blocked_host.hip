@@ -1,0 +1,561 @@
+// Host-side builders of the CSB and BlockCOH device layouts, plus their
+// parity exports.  The stored arrays follow the reference's layouts exactly
+// (SOA_Implementations/CSB.hpp:464-567, BlockCOH.hpp:319-455); the extra
+// tables the kernels need (slab pieces, per-block offsets and coordinates)
+// are derived from them here.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "blocked.hpp"
+
+namespace pwm_impl {
+
+namespace {
+
+// bit i of v moves to bit 2i
+inline uint64_t spread16(uint64_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0full;
+    v = (v | (v << 2)) & 0x33333333ull;
+    v = (v | (v << 1)) & 0x55555555ull;
+    return v;
+}
+// CSB.hpp:147-173: at each bit level the row bit ranks above the column bit.
+inline uint32_t zkey(uint32_t r, uint32_t c) { return (uint32_t)((spread16(r) << 1) | spread16(c)); }
+
+// ceil(log2(sqrt(M))) as the smallest k with 4^k >= M (equal to the
+// reference's floating-point form for every M < 2^52).
+inline int ceil_log4(int64_t M) {
+    int k = 0;
+    while (((int64_t)1 << (2 * k)) < M) ++k;
+    return k;
+}
+
+// SOA_Implementations/CSB.hpp:96-113 with sizeof(T) = 8 (the reference's
+// data_t): bits = min(16, 3 + lg), shrunk while bits > lg and the x and y
+// slices of a block exceed 0.85 MB.
+int csb_reference_bits(int64_t nor, int64_t noc) {
+    const int lg = ceil_log4(std::min(nor, noc));
+    int bits = std::min(16, 3 + lg);
+    while (bits > lg) {
+        const double beta = (double)((int64_t)1 << bits);
+        if (8.0 * 8 * beta * 2 > 1 * 0.85 * 1e6 * 8)
+            --bits;
+        else
+            break;
+    }
+    return bits;
+}
+
+hipError_t upload(DevBuf& d, const void* src, size_t bytes) {
+    hipError_t e = d.alloc(std::max<size_t>(bytes, 4));
+    if (e != hipSuccess || bytes == 0) return e;
+    return hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+// Values of the handle's storage type: uploaded as double, converted on the
+// device (same path as the CRS formats).
+static int upload_blocked_values(pwm_matrix* h, DevBuf& dst, const std::vector<double>& v) {
+    hipError_t e = dst.alloc(std::max<size_t>(v.size(), 1) * value_size(h->vtype));
+    if (e != hipSuccess) return hip_fail(h, e, "blocked values");
+    if (v.empty()) return PWM_OK;
+    if (h->vtype == PWM_VALUES_F64) {
+        e = hipMemcpy(dst.p, v.data(), v.size() * 8, hipMemcpyHostToDevice);
+    } else {
+        DevBuf tmp;
+        e = tmp.alloc(v.size() * 8);
+        if (e == hipSuccess) e = hipMemcpy(tmp.p, v.data(), v.size() * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = pwmk::launch_convert_values(tmp.as<double>(), dst.p, (int64_t)v.size(), h->vtype, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "blocked values upload");
+    return PWM_OK;
+}
+
+static void check_bf16(pwm_matrix* h, const double* vals, int64_t n, int* rc) {
+    if (h->vtype != PWM_VALUES_BF16) return;
+    for (int64_t i = 0; i < n; ++i) {
+        const float f = (float)vals[i];
+        uint32_t b;
+        memcpy(&b, &f, 4);
+        if ((double)f != vals[i] || (b & 0xffffu)) {
+            *rc = fail(h, PWM_ERR_INVALID, "BF16 storage needs values exact in BF16");
+            return;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CSB
+
+int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
+    const int64_t nor = h->nor, noc = h->noc, nnz = h->nnz;
+    int rc = PWM_OK;
+    check_bf16(h, vals, nnz, &rc);
+    if (rc != PWM_OK) return rc;
+    const int bits = h->csb_bits_override > 0 ? h->csb_bits_override : csb_reference_bits(nor, noc);
+    const int64_t beta = (int64_t)1 << bits;
+    const int64_t H = (noc + beta - 1) / beta, V = (nor + beta - 1) / beta;
+    CsbDev& c = h->csb;
+    c.bits = bits;
+    c.H = H;
+    c.V = V;
+
+    std::vector<uint32_t> ind(std::max<int64_t>(nnz, 1));
+    std::vector<double> data(std::max<int64_t>(nnz, 1));
+    std::vector<int32_t> blk_ptr(H * V + 1, 0);
+    std::vector<double> brn(V, 0.0);
+    // Per block row: bucket by block column (stable), then sort each block by
+    // Z-Morton key (stable, so duplicates keep their CRS order).
+    parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+        std::vector<int64_t> cnt;
+        struct KP {
+            uint32_t key;
+            uint32_t lrow;
+            int64_t e;
+        };
+        std::vector<KP> kp;
+        for (int64_t br = b0; br < b1; ++br) {
+            const int64_t r0 = br * beta, r1 = std::min(nor, r0 + beta);
+            const int64_t e0 = rs[r0], e1 = rs[r1];
+            cnt.assign(H + 1, 0);
+            for (int64_t e = e0; e < e1; ++e) cnt[ci[e] / beta + 1]++;
+            for (int64_t bc = 0; bc < H; ++bc) cnt[bc + 1] += cnt[bc];
+            for (int64_t bc = 0; bc < H; ++bc) blk_ptr[br * H + bc + 1] = (int32_t)(e0 + cnt[bc + 1]);
+            brn[br] = (double)(e1 - e0);
+            kp.resize(e1 - e0);
+            std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+            for (int64_t r = r0; r < r1; ++r)
+                for (int64_t e = rs[r]; e < rs[r + 1]; ++e) {
+                    const int64_t bc = ci[e] / beta;
+                    kp[pos[bc]++] = {zkey((uint32_t)(r - r0), (uint32_t)(ci[e] - bc * beta)), (uint32_t)(r - r0), e};
+                }
+            for (int64_t bc = 0; bc < H; ++bc)
+                std::stable_sort(kp.begin() + cnt[bc], kp.begin() + cnt[bc + 1],
+                                 [](const KP& a, const KP& b) { return a.key < b.key; });
+            for (int64_t i = 0; i < e1 - e0; ++i) {
+                const int64_t e = kp[i].e;
+                ind[e0 + i] = (kp[i].lrow << 16) | (uint32_t)(ci[e] % beta);
+                data[e0 + i] = vals[e];
+            }
+        }
+    });
+    blk_ptr[0] = 0;
+    // CSB.hpp:549-559: unbalanced when a block row holds more than twice the mean
+    c.balanced = 1;
+    const double avg = std::accumulate(brn.begin(), brn.end(), 0.0) / (double)V;
+    for (int64_t i = 0; i < V; ++i)
+        if (brn[i] > 2 * avg) {
+            c.balanced = 0;
+            break;
+        }
+
+    // Slab pieces: a workgroup owns `slab` rows of a block row (its y-slice
+    // lives in LDS).  With slab = beta >> k, the entries of one block whose
+    // top k row bits equal s form 2^(k-1) contiguous Z-order runs (one per
+    // value of the top k-1 column bits), or the whole block when k = 0.
+    const int64_t lds_rows = (int64_t)blocked::lds_rows(h->vtype);
+    int k = 0;
+    while ((beta >> k) > lds_rows) ++k;
+    const int64_t slab = beta >> k;
+    const int64_t np = k == 0 ? 1 : ((int64_t)1 << (k - 1));
+    const int64_t S = (int64_t)1 << k;  // slabs per block row
+    c.slab_bits = bits - k;
+    // task t = br*S + s; pieces[t][bc][j] = (start, end)
+    std::vector<blocked::Piece> pieces(V * S * H * np);
+    const int shift = 2 * bits - (k == 0 ? 0 : 2 * k - 1);  // key bits below the piece prefix
+    parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+        std::vector<int64_t> cntp;
+        for (int64_t br = b0; br < b1; ++br)
+            for (int64_t bc = 0; bc < H; ++bc) {
+                const int64_t s0 = blk_ptr[br * H + bc], s1 = blk_ptr[br * H + bc + 1];
+                if (k == 0) {
+                    pieces[((br * S) * H + bc) * np] = {(int32_t)s0, (int32_t)s1};
+                    continue;
+                }
+                // prefixes in key order: 2^(2k-1) runs; prefix bits are
+                // r_{b-1} c_{b-1} ... r_{b-k}: slab = its k row bits, piece = its k-1 column bits
+                const int64_t nprefix = (int64_t)1 << (2 * k - 1);
+                cntp.assign(nprefix + 1, 0);
+                for (int64_t e = s0; e < s1; ++e) {
+                    const uint32_t key = zkey(ind[e] >> 16, ind[e] & 0xffffu);
+                    cntp[(key >> shift) + 1]++;
+                }
+                for (int64_t q = 0; q < nprefix; ++q) cntp[q + 1] += cntp[q];
+                for (int64_t q = 0; q < nprefix; ++q) {
+                    // decode prefix q: bit 2k-2 is r_{b-1}, then c_{b-1}, r_{b-2}, ...
+                    int64_t sr = 0, pc = 0;
+                    for (int lvl = 0; lvl < k; ++lvl) {
+                        const int rb = (int)((q >> (2 * k - 2 - 2 * lvl)) & 1);
+                        sr = (sr << 1) | rb;
+                        if (lvl < k - 1) pc = (pc << 1) | ((q >> (2 * k - 3 - 2 * lvl)) & 1);
+                    }
+                    pieces[((br * S + sr) * H + bc) * np + pc] = {(int32_t)(s0 + cntp[q]), (int32_t)(s0 + cntp[q + 1])};
+                }
+            }
+    });
+    c.n_tasks = V * S;
+    hipError_t e;
+    if ((e = upload(c.ind, ind.data(), nnz * 4)) != hipSuccess || (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
+        (e = upload(c.work, pieces.data(), pieces.size() * sizeof(blocked::Piece))) != hipSuccess)
+        return hip_fail(h, e, "CSB upload");
+    data.resize(nnz);
+    rc = upload_blocked_values(h, c.data, data);
+    if (rc != PWM_OK) return rc;
+    c.pieces_per_block = (int32_t)np;
+    c.slabs = (int32_t)S;
+    c.slab_rows = slab;
+    // the CRS arrays are not needed by the CSB kernels
+    h->rs.release();
+    h->ci.release();
+    h->val.release();
+    return PWM_OK;
+}
+
+// ---------------------------------------------------------------- BlockCOH
+
+// BlockCOH.hpp:79-114 (Wikipedia xy2d, x = col, y = row, flips with the full size).
+static int64_t hilbert(int64_t size, int64_t row, int64_t col) {
+    int64_t index = 0;
+    for (int64_t s = size / 2; s > 0; s /= 2) {
+        const int64_t rr = (row & s) > 0, cr = (col & s) > 0;
+        index += s * s * ((3 * cr) ^ rr);
+        if (rr == 0) {
+            if (cr == 1) {
+                col = size - 1 - col;
+                row = size - 1 - row;
+            }
+            std::swap(row, col);
+        }
+    }
+    return index;
+}
+
+// BlockCOH.hpp:325-355: thread clamp and nnz-balanced bands.
+static void coh_bands(int64_t nor, int64_t nnz, const int32_t* rs, int threads, std::vector<int64_t>& start,
+                      std::vector<int64_t>& bnnz) {
+    start.assign(threads, nor);
+    bnnz.assign(threads, 0);
+    const int64_t share = nnz / threads;
+    int64_t sum = 0;
+    int pid = 0;
+    start[0] = 0;
+    for (int64_t i = 0; i < nor; ++i) {
+        const int64_t cnt = rs[i + 1] - rs[i];
+        sum += cnt;
+        if (sum > share && pid != threads - 1) {
+            bnnz[pid++] = sum - cnt;
+            start[pid] = i;
+            sum = cnt;
+        }
+    }
+    bnnz[pid] = sum;
+}
+
+int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
+    const int64_t nor = h->nor, noc = h->noc, nnz = h->nnz;
+    int rc = PWM_OK;
+    check_bf16(h, vals, nnz, &rc);
+    if (rc != PWM_OK) return rc;
+    const int64_t lds_rows = blocked::lds_rows(h->vtype);
+    // thread count: the caller's, else the smallest count (>= 256 bands, so
+    // every CU gets work) whose bands all fit the LDS y-window
+    std::vector<int64_t> bstart, bnnz;
+    int threads = h->threads;
+    auto clamp = [&](int t) {  // BlockCOH.hpp:325-328 (kept >= 1)
+        if (t < 1) t = 1;
+        if (nor / t < 8) t = (int)(nor / 8);
+        return t < 1 ? 1 : t;
+    };
+    if (threads > 0) {
+        threads = clamp(threads);
+        coh_bands(nor, nnz, rs, threads, bstart, bnnz);
+    } else {
+        threads = clamp((int)std::min<int64_t>(256, std::max<int64_t>(1, nor / 8)));
+        for (;;) {
+            coh_bands(nor, nnz, rs, threads, bstart, bnnz);
+            int64_t mx = 0;
+            for (int p = 0; p < threads; ++p) mx = std::max(mx, (p + 1 < threads ? bstart[p + 1] : nor) - bstart[p]);
+            const int next = clamp(threads * 2);
+            if ((mx <= lds_rows && nnz / threads <= 262144) || next == threads) break;
+            threads = next;
+        }
+    }
+    CohDev& c = h->coh;
+    c.bands.assign(threads, CohBandMeta{});
+    int64_t tot_nz = 0;
+    std::vector<CohBandMeta>& B = c.bands;
+    for (int p = 0; p < threads; ++p) {
+        CohBandMeta& m = B[p];
+        m.start = bstart[p];
+        m.rows = (p + 1 < threads ? bstart[p + 1] : nor) - bstart[p];
+        if (m.rows < 0) m.rows = 0;
+        m.nnz = m.rows > 0 ? rs[m.start + m.rows] - rs[m.start] : 0;
+        const int64_t mx = std::max(m.rows, noc);
+        m.beta_bits = std::min(15, 3 + ceil_log4(mx));               // BlockCOH.hpp:119-132
+        const int64_t beta = (int64_t)1 << m.beta_bits;
+        m.H = (int32_t)((noc + beta - 1) / beta);
+        m.V = (int32_t)((m.rows + beta - 1) / beta);
+        m.off_nz = tot_nz;
+        tot_nz += m.nnz;
+    }
+    // Per band: Hilbert order of blocks (hilbert_size/beta grid), ICRS inside
+    // each block, BICRS jumps between blocks.
+    std::vector<uint16_t> col_jump(std::max<int64_t>(nnz, 1)), row_jump(std::max<int64_t>(nnz, 1));
+    std::vector<double> data(std::max<int64_t>(nnz, 1));
+    std::vector<std::vector<int16_t>> rjb(threads), cjb(threads);
+    std::vector<std::vector<int32_t>> bnz(threads);
+    std::vector<std::vector<blocked::CohBlock>> blocks(threads);
+    std::vector<int64_t> nrj(threads, 0);
+    parallel_for(threads, 1, [&](int64_t p0, int64_t p1, int) {
+        std::vector<int64_t> key_of_block, order, rank, cnt;
+        std::vector<int64_t> perm;
+        for (int64_t p = p0; p < p1; ++p) {
+            CohBandMeta& m = B[p];
+            if (m.nnz == 0) continue;
+            const int64_t beta = (int64_t)1 << m.beta_bits;
+            const int64_t mx = std::max(m.rows, noc);
+            int64_t hsize = 1;
+            while (hsize < mx) hsize <<= 1;                               // BlockCOH.hpp:401
+            const int64_t H = m.H, V = m.V, nb = H * V;
+            key_of_block.resize(nb);
+            for (int64_t br = 0; br < V; ++br)
+                for (int64_t bc = 0; bc < H; ++bc) key_of_block[br * H + bc] = hilbert(hsize / beta, br, bc);
+            order.resize(nb);
+            std::iota(order.begin(), order.end(), 0);
+            std::sort(order.begin(), order.end(),
+                      [&](int64_t a, int64_t b) { return key_of_block[a] < key_of_block[b]; });
+            rank.resize(nb);
+            for (int64_t i = 0; i < nb; ++i) rank[order[i]] = i;
+            // stable counting sort of the band's entries by block rank
+            const int64_t e0 = rs[m.start];
+            cnt.assign(nb + 1, 0);
+            for (int64_t r = m.start; r < m.start + m.rows; ++r)
+                for (int64_t e = rs[r]; e < rs[r + 1]; ++e)
+                    cnt[rank[((r - m.start) / beta) * H + ci[e] / beta] + 1]++;
+            for (int64_t i = 0; i < nb; ++i) cnt[i + 1] += cnt[i];
+            perm.resize(m.nnz);
+            std::vector<int64_t> lrow(m.nnz);
+            {
+                std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+                for (int64_t r = m.start; r < m.start + m.rows; ++r)
+                    for (int64_t e = rs[r]; e < rs[r + 1]; ++e) {
+                        const int64_t q = pos[rank[((r - m.start) / beta) * H + ci[e] / beta]]++;
+                        perm[q] = e;
+                        lrow[q] = r - m.start;
+                    }
+            }
+            int64_t rj = 0;
+            int64_t prev_br = -1, prev_bc = -1;
+            for (int64_t q = 0; q < nb; ++q) {
+                const int64_t s = cnt[q], t = cnt[q + 1];
+                if (s == t) continue;
+                const int64_t blk = order[q], br = blk / H, bc = blk % H;
+                blocked::CohBlock cb;
+                cb.entry = (int32_t)s;
+                cb.row_jump = (int32_t)rj;
+                cb.brow = (int32_t)br;
+                cb.bcol = (int32_t)bc;
+                // ICRS (TripletToCRS.hpp:402-452)
+                int64_t pr = -1, pc = -1;
+                for (int64_t i = s; i < t; ++i) {
+                    const int64_t e = perm[i];
+                    const int64_t r = lrow[i] % beta, cc = ci[e] % beta;
+                    data[e0 + i] = vals[e];
+                    if (i == s) {
+                        row_jump[e0 + rj++] = (uint16_t)r;
+                        col_jump[e0 + i] = (uint16_t)cc;
+                    } else if (r != pr) {
+                        col_jump[e0 + i] = (uint16_t)(cc + beta - pc);
+                        row_jump[e0 + rj++] = (uint16_t)(r - pr);
+                    } else {
+                        col_jump[e0 + i] = (uint16_t)(cc - pc);
+                    }
+                    pr = r;
+                    pc = cc;
+                }
+                // block-level BICRS (BlockCOH.hpp:433-439)
+                if (blocks[p].empty()) {
+                    rjb[p].push_back((int16_t)br);
+                    cjb[p].push_back((int16_t)bc);
+                } else if (br == prev_br) {
+                    cjb[p].push_back((int16_t)(bc - prev_bc));
+                } else {
+                    rjb[p].push_back((int16_t)(br - prev_br));
+                    cjb[p].push_back((int16_t)(bc - prev_bc + H));
+                }
+                bnz[p].push_back((int32_t)(t - s));
+                blocks[p].push_back(cb);
+                prev_br = br;
+                prev_bc = bc;
+            }
+            nrj[p] = rj;
+        }
+    });
+    // concatenate: row_jump entries of band p start at its entry offset (a
+    // band has at most nnz row segments), block tables band after band
+    std::vector<blocked::CohBlock> allb;
+    std::vector<blocked::CohBand> table(threads);
+    std::vector<int16_t> all_rjb, all_cjb;
+    std::vector<int32_t> all_bnz, ctab;
+    for (int p = 0; p < threads; ++p) {
+        // chunk c of the kernel decodes entries [c*kCohChunk, (c+1)*kCohChunk):
+        // ctab[c] = block holding its first entry, ctab[chunks] = last block
+        table[p].chunk_table = (int32_t)ctab.size();
+        const int64_t nch = (B[p].nnz + blocked::kCohChunk - 1) / blocked::kCohChunk;
+        int64_t b = 0;
+        for (int64_t c = 0; c < nch; ++c) {
+            const int64_t e = c * blocked::kCohChunk;
+            while (b + 1 < (int64_t)blocks[p].size() && blocks[p][b + 1].entry <= e) ++b;
+            ctab.push_back((int32_t)b);
+        }
+        ctab.push_back((int32_t)std::max<int64_t>(0, (int64_t)blocks[p].size() - 1));
+        CohBandMeta& m = B[p];
+        m.blocks = (int64_t)blocks[p].size();
+        m.row_jumps = nrj[p];
+        m.block_row_jumps = (int64_t)rjb[p].size();
+        m.off_rj = m.off_nz;
+        m.off_blk = (int64_t)allb.size();
+        m.off_rjb = (int64_t)all_rjb.size();
+        table[p].start = (int32_t)m.start;
+        table[p].rows = (int32_t)m.rows;
+        table[p].beta_bits = m.beta_bits;
+        table[p].first_block = (int32_t)m.off_blk;
+        table[p].blocks = (int32_t)m.blocks;
+        table[p].entry = (int32_t)m.off_nz;
+        table[p].nnz = (int32_t)m.nnz;
+        allb.insert(allb.end(), blocks[p].begin(), blocks[p].end());
+        all_rjb.insert(all_rjb.end(), rjb[p].begin(), rjb[p].end());
+        all_cjb.insert(all_cjb.end(), cjb[p].begin(), cjb[p].end());
+        all_bnz.insert(all_bnz.end(), bnz[p].begin(), bnz[p].end());
+    }
+    c.total_blocks = (int64_t)allb.size();
+    hipError_t e;
+    if ((e = upload(c.col_jump, col_jump.data(), nnz * 2)) != hipSuccess ||
+        (e = upload(c.row_jump, row_jump.data(), nnz * 2)) != hipSuccess ||
+        (e = upload(c.row_jump_block, all_rjb.data(), all_rjb.size() * 2)) != hipSuccess ||
+        (e = upload(c.col_jump_block, all_cjb.data(), all_cjb.size() * 2)) != hipSuccess ||
+        (e = upload(c.block_nnz, all_bnz.data(), all_bnz.size() * 4)) != hipSuccess ||
+        (e = upload(c.block_coords, allb.data(), allb.size() * sizeof(blocked::CohBlock))) != hipSuccess ||
+        (e = upload(c.band_table, table.data(), table.size() * sizeof(blocked::CohBand))) != hipSuccess ||
+        (e = upload(c.tasks, ctab.data(), ctab.size() * 4)) != hipSuccess)
+        return hip_fail(h, e, "BlockCOH upload");
+    data.resize(nnz);
+    rc = upload_blocked_values(h, c.data, data);
+    if (rc != PWM_OK) return rc;
+    c.n_tasks = threads;
+    h->rs.release();
+    h->ci.release();
+    h->val.release();
+    return PWM_OK;
+}
+
+int64_t csb_partials(const pwm_matrix* h) { return h->csb.n_tasks; }
+int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks; }
+
+hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
+    blocked::CsbArgs a;
+    a.nor = h->nor;
+    a.bits = h->csb.bits;
+    a.H = h->csb.H;
+    a.slabs = h->csb.slabs;
+    a.slab_rows = h->csb.slab_rows;
+    a.np = h->csb.pieces_per_block;
+    a.ind = h->csb.ind.as<uint32_t>();
+    a.val = h->csb.data.p;
+    a.pieces = h->csb.work.as<blocked::Piece>();
+    a.n_tasks = h->csb.n_tasks;
+    a.vtype = h->vtype;
+    hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream);
+    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)a.n_tasks, *pw, h->stream);
+    return e;
+}
+
+hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
+    blocked::CohArgs a;
+    a.nor = h->nor;
+    a.n_bands = h->coh.n_tasks;
+    a.bands = h->coh.band_table.as<blocked::CohBand>();
+    a.blocks = h->coh.block_coords.as<blocked::CohBlock>();
+    a.chunk_block = h->coh.tasks.as<int32_t>();
+    a.col_jump = h->coh.col_jump.as<uint16_t>();
+    a.row_jump = h->coh.row_jump.as<uint16_t>();
+    a.val = h->coh.data.p;
+    a.vtype = h->vtype;
+    hipError_t e = blocked::launch_coh(a, x, y, pw, h->stream);
+    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)a.n_bands, *pw, h->stream);
+    return e;
+}
+
+}  // namespace pwm_impl
+
+using namespace pwm_impl;
+
+extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nblocks, uint32_t* ind, double* data,
+                              int32_t* blk_ptr) {
+    if (!h || !block_bits || !nblocks) return fail(h, PWM_ERR_INVALID, "pwm_csb_layout: NULL argument");
+    if (h->format != PWM_FORMAT_CSB || !h->loaded) return fail(h, PWM_ERR_STATE, "pwm_csb_layout: not a loaded CSB handle");
+    *block_bits = h->csb.bits;
+    *nblocks = h->csb.H * h->csb.V;
+    if (!ind || !data || !blk_ptr) return PWM_OK;
+    hipError_t e = hipMemcpy(blk_ptr, h->csb.blk_ptr.p, (*nblocks + 1) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h->nnz) e = hipMemcpy(ind, h->csb.ind.p, h->nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h->nnz) {
+        DevBuf tmp;
+        e = tmp.alloc(h->nnz * 8);
+        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(h->csb.data.p, h->vtype, tmp.as<double>(), h->nnz, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(data, tmp.p, h->nnz * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_csb_layout");
+    return PWM_OK;
+}
+
+extern "C" int pwm_blockcoh_band(pwm_matrix* h, int band, int64_t* counts, int32_t* meta, uint16_t* row_jump,
+                                 uint16_t* col_jump, double* data, int16_t* row_jump_block, int16_t* col_jump_block,
+                                 int32_t* block_nnz) {
+    if (!h || !counts || !meta) return fail(h, PWM_ERR_INVALID, "pwm_blockcoh_band: NULL argument");
+    if (h->format != PWM_FORMAT_BLOCKCOH || !h->loaded)
+        return fail(h, PWM_ERR_STATE, "pwm_blockcoh_band: not a loaded BlockCOH handle");
+    if (band < 0 || band >= (int)h->coh.bands.size()) return fail(h, PWM_ERR_INVALID, "pwm_blockcoh_band: no such band");
+    const CohBandMeta& m = h->coh.bands[band];
+    counts[0] = m.nnz;
+    counts[1] = m.blocks;
+    counts[2] = m.row_jumps;
+    counts[3] = m.block_row_jumps;
+    counts[4] = m.start;
+    counts[5] = m.rows;
+    meta[0] = m.beta_bits;
+    meta[1] = m.H;
+    meta[2] = m.V;
+    if (!row_jump || !col_jump || !data || !row_jump_block || !col_jump_block || !block_nnz) return PWM_OK;
+    hipError_t e = hipSuccess;
+    if (m.row_jumps) e = hipMemcpy(row_jump, h->coh.row_jump.as<uint16_t>() + m.off_rj, m.row_jumps * 2, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.nnz) e = hipMemcpy(col_jump, h->coh.col_jump.as<uint16_t>() + m.off_nz, m.nnz * 2, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.block_row_jumps)
+        e = hipMemcpy(row_jump_block, h->coh.row_jump_block.as<int16_t>() + m.off_rjb, m.block_row_jumps * 2, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && m.blocks) {
+        e = hipMemcpy(col_jump_block, h->coh.col_jump_block.as<int16_t>() + m.off_blk, m.blocks * 2, hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(block_nnz, h->coh.block_nnz.as<int32_t>() + m.off_blk, m.blocks * 4, hipMemcpyDeviceToHost);
+    }
+    if (e == hipSuccess && m.nnz) {
+        DevBuf tmp;
+        const size_t vs = value_size(h->vtype);
+        e = tmp.alloc(m.nnz * 8);
+        if (e == hipSuccess)
+            e = pwmk::launch_copy_values_f64((const char*)h->coh.data.p + m.off_nz * vs, h->vtype, tmp.as<double>(), m.nnz,
+                                             h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(data, tmp.p, m.nnz * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_blockcoh_band");
+    return PWM_OK;
+}

@@ -1,25 +1,448 @@
-// CSB and BlockCOH device formats (built in a later step).
-#include "api_internal.hpp"
+// CSB and BlockCOH SpMV kernels for gfx950.
+//
+// Both formats are processed by workgroups that own a slab of rows and keep
+// its y-slice in LDS (FP64 or FP32 atomic adds, ds_add_*): every entry of a
+// slab, wherever its block sits in the block row or band, adds into LDS and
+// the slab is written to y once with coalesced stores.
+//
+// x locality: every workgroup walks its blocks in the same order (CSB: block
+// columns ascending, CSB.hpp:354-363; BlockCOH: the Hilbert order of the
+// blocks, which is the same sequence of block columns for every band whose
+// rows fit one block row).  The workgroups resident on one XCD therefore
+// gather from the same x-slice at about the same time, and that slice
+// (beta * 8 bytes, 128-256 KB) stays in the XCD's 4 MB L2 instead of each
+// gather missing to the Infinity Fabric as in a row-ordered CRS sweep over
+// random columns.
+#include "blocked.hpp"
+#include "kernels.hpp"
 
-namespace pwm_impl {
+namespace blocked {
 
-int build_csb(pwm_matrix* h, const int32_t*, const int32_t*, const double*) {
-    return fail(h, PWM_ERR_UNSUPPORTED, "CSB format not built yet");
-}
-int build_blockcoh(pwm_matrix* h, const int32_t*, const int32_t*, const double*) {
-    return fail(h, PWM_ERR_UNSUPPORTED, "BlockCOH format not built yet");
-}
-hipError_t csb_launch(pwm_matrix*, const void*, void*, const pwmk::PowerArgs*) { return hipErrorNotSupported; }
-hipError_t coh_launch(pwm_matrix*, const void*, void*, const pwmk::PowerArgs*) { return hipErrorNotSupported; }
-int64_t csb_partials(const pwm_matrix*) { return 0; }
-int64_t coh_partials(const pwm_matrix*) { return 0; }
+using pwmk::Arith;
+using pwmk::bf16_t;
+using pwmk::PowerArgs;
+using pwmk::widen;
 
-}  // namespace pwm_impl
+constexpr int kBS = kBlockedBS;
+constexpr int kWaves = kBS / 64;
+constexpr int kWin = 2 * kBS;  // CSB pieces staged per window
 
-extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t*, int64_t*, uint32_t*, double*, int32_t*) {
-    return pwm_impl::fail(h, PWM_ERR_UNSUPPORTED, "CSB format not built yet");
+template <typename Acc>
+struct LdsRows;
+template <> struct LdsRows<double> { static constexpr int v = kLdsRowsF64; };
+template <> struct LdsRows<float> { static constexpr int v = kLdsRowsF32; };
+
+// Block-wide exclusive scan of one int per thread; returns the prefix and
+// writes the total to *total.  `tmp` holds kWaves ints.
+__device__ __forceinline__ int block_excl_scan(int v, int* tmp, int* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(incl, d);
+        if (lane >= d) incl += u;
+    }
+    if (lane == 63) tmp[wave] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        const int t = tmp[w];
+        if (w < wave) before += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
 }
-extern "C" int pwm_blockcoh_band(pwm_matrix* h, int, int64_t*, int32_t*, uint16_t*, uint16_t*, double*, int16_t*,
-                                 int16_t*, int32_t*) {
-    return pwm_impl::fail(h, PWM_ERR_UNSUPPORTED, "BlockCOH format not built yet");
+
+// Block-wide segmented scan.  Each thread brings the sum `v` of its entries
+// after its last segment start and `f` = 1 if it holds a segment start.  The
+// result is the value that flows into the thread's first entry: the sum over
+// earlier threads back to the most recent segment start, plus `carry` if no
+// segment starts before this thread.  `tv`/`tf` hold kWaves values each.
+__device__ __forceinline__ int block_seg_excl(int v, int f, int carry, int* tv, int* tf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int iv = v, ifl = f;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int uv = __shfl_up(iv, d);
+        const int uf = __shfl_up(ifl, d);
+        if (lane >= d) {
+            if (!ifl) iv += uv;
+            ifl |= uf;
+        }
+    }
+    if (lane == 63) {
+        tv[wave] = iv;
+        tf[wave] = ifl;
+    }
+    __syncthreads();
+    // prefix of the earlier waves, seeded with the chunk carry
+    int pv = carry;
+    for (int w = 0; w < wave; ++w) {
+        if (tf[w]) {
+            pv = tv[w];
+        } else {
+            pv += tv[w];
+        }
+    }
+    __syncthreads();
+    // exclusive value of this lane: inclusive of the previous lane in the wave
+    int ev = __shfl_up(iv, 1);
+    int ef = __shfl_up(ifl, 1);
+    if (lane == 0) {
+        ev = 0;
+        ef = 0;
+    }
+    return ef ? ev : ev + pv;
 }
+
+// ---------------------------------------------------------------- CSB
+
+template <typename VT, bool POWER>
+__global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
+                                             typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    constexpr int LR = LdsRows<Acc>::v;
+    __shared__ Acc ylds[LR];
+    __shared__ int32_t p_start[kWin];
+    __shared__ int32_t p_pre[kWin + 1];
+    __shared__ int scan_tmp[kWaves];
+    __shared__ double red[16];
+
+    const X* __restrict__ x = x_arg;
+    X* __restrict__ y = y_arg;
+    double scale = 1.0;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = pwmk::step_scale(pw, k);
+    }
+    const VT* __restrict__ val = (const VT*)a.val;
+    const uint32_t* __restrict__ ind = a.ind;
+    const int tid = threadIdx.x;
+    const int64_t t = blockIdx.x;
+    const int64_t br = t / a.slabs;
+    const int s = (int)(t % a.slabs);
+    const int64_t beta = (int64_t)1 << a.bits;
+    const int h = (int)a.slab_rows;
+    const int slab_base = s * h;
+    for (int i = tid; i < h; i += kBS) ylds[i] = 0;
+
+    const int64_t npieces = a.H * a.np;
+    const Piece* __restrict__ P = a.pieces + t * npieces;
+    for (int64_t w0 = 0; w0 < npieces; w0 += kWin) {
+        const int nw = (int)(npieces - w0 < kWin ? npieces - w0 : kWin);
+        __syncthreads();
+        const int i0 = 2 * tid, i1 = 2 * tid + 1;
+        int len0 = 0, len1 = 0;
+        if (i0 < nw) {
+            const Piece q = P[w0 + i0];
+            p_start[i0] = q.start;
+            len0 = q.end - q.start;
+        }
+        if (i1 < nw) {
+            const Piece q = P[w0 + i1];
+            p_start[i1] = q.start;
+            len1 = q.end - q.start;
+        }
+        int total;
+        const int base = block_excl_scan(len0 + len1, scan_tmp, &total);
+        p_pre[i0] = base;
+        p_pre[i1] = base + len0;
+        if (tid == kBS - 1) p_pre[kWin] = total;
+        __syncthreads();
+        const int T = total;
+        int cur = 0;
+        for (int v = tid; v < T; v += 4 * kBS) {
+            int kk[4], pc[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int vv = v + u * kBS;
+                ok[u] = vv < T;
+                if (ok[u]) {
+                    while (p_pre[cur + 1] <= vv) ++cur;
+                    pc[u] = cur;
+                    kk[u] = p_start[cur] + (vv - p_pre[cur]);
+                } else {
+                    pc[u] = 0;
+                    kk[u] = 0;
+                }
+            }
+            uint32_t w[4];
+            Acc vv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                w[u] = ok[u] ? ind[kk[u]] : 0u;
+                vv[u] = ok[u] ? (Acc)widen(val[kk[u]]) : (Acc)0;
+            }
+            Acc xv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t bc = (w0 + pc[u]) / a.np;
+                xv[u] = ok[u] ? (Acc)x[bc * beta + (w[u] & 0xffffu)] : (Acc)0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (ok[u]) atomicAdd(&ylds[(int)(w[u] >> 16) - slab_base], vv[u] * xv[u]);
+        }
+    }
+    __syncthreads();
+
+    const int64_t row0 = br * beta + slab_base;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = tid; i < h; i += kBS) {
+        const int64_t row = row0 + i;
+        if (row >= a.nor) break;
+        if constexpr (POWER) {
+            const X out = (X)((double)ylds[i] * scale);
+            y[row] = out;
+            const double yd = (double)out, xd = (double)x[row] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        } else {
+            y[row] = (X)ylds[i];
+        }
+    }
+    if constexpr (POWER) {
+        const double ra = pwmk::block_sum<kBS>(yy, red);
+        const double rb = pwmk::block_sum<kBS>(xy, red);
+        const double rc = pwmk::block_sum<kBS>(xx, red);
+        if (tid == 0) {
+            pw.partials[3 * t + 0] = ra;
+            pw.partials[3 * t + 1] = rb;
+            pw.partials[3 * t + 2] = rc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- BlockCOH
+
+// One workgroup per band; bands with more rows than the LDS window are swept
+// once per window of rows.  Each step decodes kCohChunk consecutive entries of
+// the band (ICRS, TripletToCRS.hpp:402-452 / BlockCOH.hpp:235-256) with two
+// segmented scans that restart at every block:
+//   S = running sum of col_jump: column = S mod beta, row changes = S div beta
+//   (a row change adds exactly beta, so S = col + beta * changes);
+//   R = running sum of the row_jump entered at each row change: the row.
+template <typename VT, bool POWER>
+__global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
+                                             typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    constexpr int LR = LdsRows<Acc>::v;
+    __shared__ Acc ylds[LR];
+    __shared__ int32_t bstart[kCohChunk + 2];  // up to kCohChunk+1 blocks and a sentinel
+    __shared__ int tv[kWaves], tf[kWaves];
+    __shared__ int carry_s, carry_r;
+    __shared__ double red[16];
+
+    const X* __restrict__ x = x_arg;
+    X* __restrict__ y = y_arg;
+    double scale = 1.0;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = pwmk::step_scale(pw, k);
+    }
+    const int tid = threadIdx.x;
+    const CohBand band = a.bands[blockIdx.x];
+    const VT* __restrict__ val = (const VT*)a.val + band.entry;
+    const uint16_t* __restrict__ cjmp = a.col_jump + band.entry;
+    const uint16_t* __restrict__ rjmp = a.row_jump + band.entry;
+    const CohBlock* __restrict__ blocks = a.blocks + band.first_block;
+    const int32_t* __restrict__ ctab = a.chunk_block + band.chunk_table;
+    const int bits = band.beta_bits;
+    const int64_t beta = (int64_t)1 << bits;
+    const int nnz = band.nnz;
+    const int nchunks = (nnz + kCohChunk - 1) / kCohChunk;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+
+    for (int wbase = 0; wbase < band.rows; wbase += LR) {
+        const int wrows = band.rows - wbase < LR ? band.rows - wbase : LR;
+        __syncthreads();
+        for (int i = tid; i < wrows; i += kBS) ylds[i] = 0;
+        if (tid == 0) {
+            carry_s = 0;
+            carry_r = 0;
+        }
+        for (int c = 0; c < nchunks; ++c) {
+            const int base = c * kCohChunk;
+            const int b_lo = ctab[c], b_hi = ctab[c + 1];  // blocks touching this chunk
+            const int nbl = b_hi - b_lo + 1;
+            __syncthreads();
+            for (int i = tid; i < nbl; i += kBS) bstart[i] = blocks[b_lo + i].entry;
+            if (tid == 0) bstart[nbl] = 0x7fffffff;
+            __syncthreads();
+            const int e0 = base + tid * kCohIPT;
+            // block of my first entry: last block start <= e0
+            int lo = 0, hi = nbl;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (bstart[mid] <= e0) lo = mid;
+                else hi = mid;
+            }
+            int bj[kCohIPT], cj[kCohIPT], rst[kCohIPT];
+            Acc vv[kCohIPT];
+            int b = lo;
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) {
+                const int e = e0 + j;
+                if (e < nnz) {
+                    while (bstart[b + 1] <= e) ++b;
+                    bj[j] = b;
+                    rst[j] = bstart[b] == e;
+                    cj[j] = cjmp[e];
+                    vv[j] = (Acc)widen(val[e]);
+                } else {
+                    bj[j] = b;
+                    rst[j] = 0;
+                    cj[j] = 0;
+                    vv[j] = 0;
+                }
+            }
+            // scan 1: S
+            int run = 0, anyr = 0;
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) {
+                run = rst[j] ? cj[j] : run + cj[j];
+                anyr |= rst[j];
+            }
+            const int sin = block_seg_excl(run, anyr, carry_s, tv, tf);
+            int S[kCohIPT];
+            run = sin;
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) {
+                run = rst[j] ? cj[j] : run + cj[j];
+                S[j] = run;
+            }
+            // row deltas at every row change
+            CohBlock blk[kCohIPT];
+            int dl[kCohIPT];
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) {
+                const int e = e0 + j;
+                dl[j] = 0;
+                if (e < nnz) {
+                    blk[j] = blocks[b_lo + bj[j]];
+                    const int kk = S[j] >> bits;
+                    const bool newrow = rst[j] || (kk != ((S[j] - cj[j]) >> bits));
+                    if (newrow) dl[j] = rjmp[blk[j].row_jump + kk];
+                } else {
+                    blk[j] = CohBlock{0, 0, 0, 0};
+                }
+            }
+            // scan 2: R
+            run = 0;
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) run = rst[j] ? dl[j] : run + dl[j];
+            const int rin = block_seg_excl(run, anyr, carry_r, tv, tf);
+            run = rin;
+#pragma unroll
+            for (int j = 0; j < kCohIPT; ++j) {
+                run = rst[j] ? dl[j] : run + dl[j];
+                const int e = e0 + j;
+                if (e < nnz) {
+                    const int col = S[j] & (int)(beta - 1);
+                    const int row = (int)(blk[j].brow * beta) + run;  // row inside the band
+                    if (row >= wbase && row < wbase + wrows) {
+                        const Acc xv = (Acc)x[(int64_t)blk[j].bcol * beta + col];
+                        atomicAdd(&ylds[row - wbase], vv[j] * xv);
+                    }
+                }
+                // the entry that closes the chunk leaves its S and R for the next chunk
+                if (e == base + kCohChunk - 1 || e == nnz - 1) {
+                    carry_s = S[j];
+                    carry_r = run;
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < wrows; i += kBS) {
+            const int64_t row = (int64_t)band.start + wbase + i;
+            if constexpr (POWER) {
+                const X out = (X)((double)ylds[i] * scale);
+                y[row] = out;
+                const double yd = (double)out, xd = (double)x[row] * scale;
+                yy += yd * yd;
+                xy += xd * yd;
+                xx += xd * xd;
+            } else {
+                y[row] = (X)ylds[i];
+            }
+        }
+    }
+    if constexpr (POWER) {
+        const double ra = pwmk::block_sum<kBS>(yy, red);
+        const double rb = pwmk::block_sum<kBS>(xy, red);
+        const double rc = pwmk::block_sum<kBS>(xx, red);
+        if (tid == 0) {
+            pw.partials[3 * blockIdx.x + 0] = ra;
+            pw.partials[3 * blockIdx.x + 1] = rb;
+            pw.partials[3 * blockIdx.x + 2] = rc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <typename VT>
+static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+    using X = typename Arith<VT>::X;
+    if (a.n_tasks <= 0) return hipSuccess;
+    if (pw)
+        hipLaunchKernelGGL((k_csb<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
+                           (X*)nullptr, *pw);
+    else
+        hipLaunchKernelGGL((k_csb<VT, false>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
+                           PowerArgs{});
+    return hipGetLastError();
+}
+
+hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+    switch (a.vtype) {
+        case 0: return csb_t<double>(a, x, y, pw, s);
+        case 1: return csb_t<float>(a, x, y, pw, s);
+        case 2: return csb_t<bf16_t>(a, x, y, pw, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename VT>
+static hipError_t coh_t(const CohArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+    using X = typename Arith<VT>::X;
+    if (a.n_bands <= 0) return hipSuccess;
+    if (pw)
+        hipLaunchKernelGGL((k_coh<VT, true>), dim3((unsigned)a.n_bands), dim3(kBS), 0, s, a, (const X*)nullptr,
+                           (X*)nullptr, *pw);
+    else
+        hipLaunchKernelGGL((k_coh<VT, false>), dim3((unsigned)a.n_bands), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
+                           PowerArgs{});
+    return hipGetLastError();
+}
+
+hipError_t launch_coh(const CohArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+    switch (a.vtype) {
+        case 0: return coh_t<double>(a, x, y, pw, s);
+        case 1: return coh_t<float>(a, x, y, pw, s);
+        case 2: return coh_t<bf16_t>(a, x, y, pw, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_blocked_finalize(int vtype, int n_partials, const PowerArgs& pw, hipStream_t s) {
+    pwmk::CrsDev A;
+    A.vtype = vtype;
+    pwmk::LongRows none;
+    return pwmk::launch_crs_finalize(A, n_partials, none, nullptr, &pw, s);
+}
+
+}  // namespace blocked

@@ -52,7 +52,7 @@ struct CohBand {
 };
 
 struct CohArgs {
-    int64_t nor = 0;
+    int64_t nor = 0, noc = 0;
     int64_t n_bands = 0;
     const CohBand* bands = nullptr;
     const CohBlock* blocks = nullptr;

@@ -481,6 +481,7 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
 hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
     blocked::CohArgs a;
     a.nor = h->nor;
+    a.noc = h->noc;
     a.n_bands = h->coh.n_tasks;
     a.bands = h->coh.band_table.as<blocked::CohBand>();
     a.blocks = h->coh.block_coords.as<blocked::CohBlock>();

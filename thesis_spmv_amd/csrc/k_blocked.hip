@@ -335,7 +335,11 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
                     blk[j] = blocks[b_lo + bj[j]];
                     const int kk = S[j] >> bits;
                     const bool newrow = rst[j] || (kk != ((S[j] - cj[j]) >> bits));
-                    if (newrow) dl[j] = rjmp[blk[j].row_jump + kk];
+                    // a band holds at most nnz row segments: the clamp keeps a
+                    // decode error an error in y, never a read outside the band
+                    int ri = blk[j].row_jump + kk;
+                    ri = ri < nnz ? ri : nnz - 1;
+                    if (newrow) dl[j] = rjmp[ri];
                 } else {
                     blk[j] = CohBlock{0, 0, 0, 0};
                 }
@@ -354,7 +358,9 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
                     const int col = S[j] & (int)(beta - 1);
                     const int row = (int)(blk[j].brow * beta) + run;  // row inside the band
                     if (row >= wbase && row < wbase + wrows) {
-                        const Acc xv = (Acc)x[(int64_t)blk[j].bcol * beta + col];
+                        int64_t xi = (int64_t)blk[j].bcol * beta + col;
+                        xi = xi < a.noc ? xi : a.noc - 1;
+                        const Acc xv = (Acc)x[xi];
                         atomicAdd(&ylds[row - wbase], vv[j] * xv);
                     }
                 }

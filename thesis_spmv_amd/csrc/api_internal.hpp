@@ -93,13 +93,15 @@ struct CohDev {
     DevBuf block_coords;   // decoded (block_row, block_col, entry offset, row_jump offset) per block
     DevBuf band_table;     // per band: start, rows, beta_bits, first block, blocks
     DevBuf tasks;          // per band: chunk -> first block table
-    int64_t n_tasks = 0, total_blocks = 0;
+    DevBuf task_list;      // CohTask per workgroup
+    DevBuf atomic_bands;   // bands accumulated with global atomics
+    int64_t n_tasks = 0, total_blocks = 0, n_atomic = 0;
     void clear() {
         bands.clear();
         for (DevBuf* b : {&row_jump, &col_jump, &data, &row_jump_block, &col_jump_block, &block_nnz, &block_coords,
-                          &band_table, &tasks})
+                          &band_table, &tasks, &task_list, &atomic_bands})
             b->release();
-        n_tasks = total_blocks = 0;
+        n_tasks = total_blocks = n_atomic = 0;
     }
 };
 

@@ -25,7 +25,8 @@ struct CsbArgs {
     int64_t H = 0;           // block columns
     int32_t slabs = 1;       // slabs per block row
     int64_t slab_rows = 0;   // beta / slabs
-    int32_t np = 1;          // pieces per (block, slab)
+    int32_t np = 1;          // pieces per (block, slab), a power of two
+    int32_t np_shift = 0;    // log2(np)
     const uint32_t* ind = nullptr;
     const void* val = nullptr;
     const Piece* pieces = nullptr;  // [task][bc][piece]
@@ -51,9 +52,22 @@ struct CohBand {
     int32_t chunk_table;  // offset of the band's chunk -> first-block table
 };
 
+// Unit of work of the BlockCOH kernel: chunks [chunk0, chunk0+nchunks) of one
+// band.  A band whose rows fit the LDS window and whose entries fit one task
+// accumulates in LDS (atomic = 0); larger bands are cut into several tasks
+// that add into y with global atomics (atomic = 1), each starting from the
+// decode state (carry_s, carry_r) the build computed for its first entry.
+constexpr int kCohTaskChunks = 16;
+struct CohTask {
+    int32_t band, chunk0, nchunks, atomic;
+    int32_t carry_s, carry_r;
+};
+
 struct CohArgs {
     int64_t nor = 0, noc = 0;
     int64_t n_bands = 0;
+    int64_t n_tasks = 0;
+    const CohTask* tasks = nullptr;
     const CohBand* bands = nullptr;
     const CohBlock* blocks = nullptr;
     const int32_t* chunk_block = nullptr;  // per band: block holding entry c*kCohChunk, c = 0..chunks
@@ -64,7 +78,8 @@ struct CohArgs {
 };
 
 hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
-hipError_t launch_coh(const CohArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
+hipError_t launch_coh(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,
+                      const pwmk::PowerArgs* pw, hipStream_t s);
 // FP64 reduction of the per-workgroup power-method partials (one workgroup).
 hipError_t launch_blocked_finalize(int vtype, int n_partials, const pwmk::PowerArgs& pw, hipStream_t s);
 

@@ -274,20 +274,12 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         if (nor / t < 8) t = (int)(nor / 8);
         return t < 1 ? 1 : t;
     };
-    if (threads > 0) {
-        threads = clamp(threads);
-        coh_bands(nor, nnz, rs, threads, bstart, bnnz);
-    } else {
-        threads = clamp((int)std::min<int64_t>(256, std::max<int64_t>(1, nor / 8)));
-        for (;;) {
-            coh_bands(nor, nnz, rs, threads, bstart, bnnz);
-            int64_t mx = 0;
-            for (int p = 0; p < threads; ++p) mx = std::max(mx, (p + 1 < threads ? bstart[p + 1] : nor) - bstart[p]);
-            const int next = clamp(threads * 2);
-            if ((mx <= lds_rows && nnz / threads <= 262144) || next == threads) break;
-            threads = next;
-        }
-    }
+    // the caller's thread count, else about 16K entries per band (and at
+    // least one band per CU); bands the LDS window or one task cannot hold
+    // are split into atomic tasks below
+    threads = threads > 0 ? clamp(threads)
+                          : clamp((int)std::min<int64_t>(INT32_MAX, std::max<int64_t>(256, nnz / 16384)));
+    coh_bands(nor, nnz, rs, threads, bstart, bnnz);
     CohDev& c = h->coh;
     c.bands.assign(threads, CohBandMeta{});
     int64_t tot_nz = 0;
@@ -437,6 +429,44 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         all_bnz.insert(all_bnz.end(), bnz[p].begin(), bnz[p].end());
     }
     c.total_blocks = (int64_t)allb.size();
+    // work list: one LDS task per band that fits, atomic tasks of at most
+    // kCohTaskChunks chunks otherwise, each with the decode state (S, R) of
+    // the entry before its first one
+    std::vector<blocked::CohTask> tasks;
+    std::vector<int32_t> atomic_bands;
+    for (int p = 0; p < threads; ++p) {
+        const CohBandMeta& m = B[p];
+        const int64_t nch = (m.nnz + blocked::kCohChunk - 1) / blocked::kCohChunk;
+        if (m.rows <= lds_rows && nch <= blocked::kCohTaskChunks) {
+            if (m.rows > 0) tasks.push_back({p, 0, (int32_t)nch, 0, 0, 0});
+            continue;
+        }
+        atomic_bands.push_back(p);
+        if (nch == 0) continue;
+        const int64_t beta = (int64_t)1 << m.beta_bits;
+        const uint16_t* cj = col_jump.data() + m.off_nz;
+        const uint16_t* rj = row_jump.data() + m.off_nz;
+        const std::vector<blocked::CohBlock>& bl = blocks[p];
+        int64_t S = 0, R = 0, b = 0;
+        for (int64_t c0 = 0; c0 < nch; c0 += blocked::kCohTaskChunks) {
+            // decode up to the entry before c0*kCohChunk
+            const int64_t stop = c0 * blocked::kCohChunk;
+            for (int64_t e = (c0 == 0 ? 0 : (c0 - blocked::kCohTaskChunks) * blocked::kCohChunk); e < stop; ++e) {
+                while (b + 1 < (int64_t)bl.size() && bl[b + 1].entry <= e) ++b;
+                if (bl[b].entry == e) {
+                    S = cj[e];
+                    R = rj[bl[b].row_jump];
+                } else {
+                    S += cj[e];
+                    if ((S >> m.beta_bits) != ((S - cj[e]) >> m.beta_bits)) R += rj[bl[b].row_jump + (S >> m.beta_bits)];
+                }
+            }
+            (void)beta;
+            tasks.push_back({p, (int32_t)c0, (int32_t)std::min<int64_t>(blocked::kCohTaskChunks, nch - c0), 1,
+                             (int32_t)S, (int32_t)R});
+        }
+    }
+    c.n_atomic = (int64_t)atomic_bands.size();
     hipError_t e;
     if ((e = upload(c.col_jump, col_jump.data(), nnz * 2)) != hipSuccess ||
         (e = upload(c.row_jump, row_jump.data(), nnz * 2)) != hipSuccess ||
@@ -445,12 +475,14 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         (e = upload(c.block_nnz, all_bnz.data(), all_bnz.size() * 4)) != hipSuccess ||
         (e = upload(c.block_coords, allb.data(), allb.size() * sizeof(blocked::CohBlock))) != hipSuccess ||
         (e = upload(c.band_table, table.data(), table.size() * sizeof(blocked::CohBand))) != hipSuccess ||
-        (e = upload(c.tasks, ctab.data(), ctab.size() * 4)) != hipSuccess)
+        (e = upload(c.tasks, ctab.data(), ctab.size() * 4)) != hipSuccess ||
+        (e = upload(c.task_list, tasks.data(), tasks.size() * sizeof(blocked::CohTask))) != hipSuccess ||
+        (e = upload(c.atomic_bands, atomic_bands.data(), atomic_bands.size() * 4)) != hipSuccess)
         return hip_fail(h, e, "BlockCOH upload");
     data.resize(nnz);
     rc = upload_blocked_values(h, c.data, data);
     if (rc != PWM_OK) return rc;
-    c.n_tasks = threads;
+    c.n_tasks = (int64_t)tasks.size();
     h->rs.release();
     h->ci.release();
     h->val.release();
@@ -458,7 +490,7 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
 }
 
 int64_t csb_partials(const pwm_matrix* h) { return h->csb.n_tasks; }
-int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks; }
+int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks + h->coh.n_atomic; }
 
 hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
     blocked::CsbArgs a;
@@ -468,6 +500,8 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.slabs = h->csb.slabs;
     a.slab_rows = h->csb.slab_rows;
     a.np = h->csb.pieces_per_block;
+    a.np_shift = 0;
+    while ((1 << a.np_shift) < a.np) ++a.np_shift;
     a.ind = h->csb.ind.as<uint32_t>();
     a.val = h->csb.data.p;
     a.pieces = h->csb.work.as<blocked::Piece>();
@@ -482,7 +516,9 @@ hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     blocked::CohArgs a;
     a.nor = h->nor;
     a.noc = h->noc;
-    a.n_bands = h->coh.n_tasks;
+    a.n_bands = (int64_t)h->coh.bands.size();
+    a.n_tasks = h->coh.n_tasks;
+    a.tasks = h->coh.task_list.as<blocked::CohTask>();
     a.bands = h->coh.band_table.as<blocked::CohBand>();
     a.blocks = h->coh.block_coords.as<blocked::CohBlock>();
     a.chunk_block = h->coh.tasks.as<int32_t>();
@@ -490,8 +526,9 @@ hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.row_jump = h->coh.row_jump.as<uint16_t>();
     a.val = h->coh.data.p;
     a.vtype = h->vtype;
-    hipError_t e = blocked::launch_coh(a, x, y, pw, h->stream);
-    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)a.n_bands, *pw, h->stream);
+    hipError_t e = blocked::launch_coh(a, h->coh.atomic_bands.as<int32_t>(), (int)h->coh.n_atomic, x, y, pw, h->stream);
+    if (e == hipSuccess && pw)
+        e = blocked::launch_blocked_finalize(h->vtype, (int)coh_partials(h), *pw, h->stream);
     return e;
 }
 

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
             Acc xv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int64_t bc = (w0 + pc[u]) / a.np;
+                const int64_t bc = (w0 + pc[u]) >> a.np_shift;
                 xv[u] = ok[u] ? (Acc)x[bc * beta + (w[u] & 0xffffu)] : (Acc)0;
             }
 #pragma unroll
@@ -224,13 +224,14 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
 
 // ---------------------------------------------------------------- BlockCOH
 
-// One workgroup per band; bands with more rows than the LDS window are swept
-// once per window of rows.  Each step decodes kCohChunk consecutive entries of
-// the band (ICRS, TripletToCRS.hpp:402-452 / BlockCOH.hpp:235-256) with two
-// segmented scans that restart at every block:
+// One workgroup per task (CohTask): chunks of kCohChunk consecutive entries
+// of one band, decoded (ICRS, TripletToCRS.hpp:402-452 / BlockCOH.hpp:235-256)
+// with two segmented scans that restart at every block:
 //   S = running sum of col_jump: column = S mod beta, row changes = S div beta
 //   (a row change adds exactly beta, so S = col + beta * changes);
 //   R = running sum of the row_jump entered at each row change: the row.
+// LDS tasks own their band (rows <= LDS window) and write it once; atomic
+// tasks add into y (zeroed by k_coh_zero) with global atomics.
 template <typename VT, bool POWER>
 __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                              typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
@@ -254,7 +255,8 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
         scale = pwmk::step_scale(pw, k);
     }
     const int tid = threadIdx.x;
-    const CohBand band = a.bands[blockIdx.x];
+    const CohTask task = a.tasks[blockIdx.x];
+    const CohBand band = a.bands[task.band];
     const VT* __restrict__ val = (const VT*)a.val + band.entry;
     const uint16_t* __restrict__ cjmp = a.col_jump + band.entry;
     const uint16_t* __restrict__ rjmp = a.row_jump + band.entry;
@@ -263,127 +265,131 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
     const int bits = band.beta_bits;
     const int64_t beta = (int64_t)1 << bits;
     const int nnz = band.nnz;
-    const int nchunks = (nnz + kCohChunk - 1) / kCohChunk;
-    double yy = 0.0, xy = 0.0, xx = 0.0;
+    const bool lds = task.atomic == 0;
 
-    for (int wbase = 0; wbase < band.rows; wbase += LR) {
-        const int wrows = band.rows - wbase < LR ? band.rows - wbase : LR;
+    if (lds)
+        for (int i = tid; i < band.rows; i += kBS) ylds[i] = 0;
+    if (tid == 0) {
+        carry_s = task.carry_s;
+        carry_r = task.carry_r;
+    }
+    for (int c = task.chunk0; c < task.chunk0 + task.nchunks; ++c) {
+        const int base = c * kCohChunk;
+        const int b_lo = ctab[c], b_hi = ctab[c + 1];  // blocks touching this chunk
+        const int nbl = b_hi - b_lo + 1;
         __syncthreads();
-        for (int i = tid; i < wrows; i += kBS) ylds[i] = 0;
-        if (tid == 0) {
-            carry_s = 0;
-            carry_r = 0;
+        for (int i = tid; i < nbl; i += kBS) bstart[i] = blocks[b_lo + i].entry;
+        if (tid == 0) bstart[nbl] = 0x7fffffff;
+        __syncthreads();
+        const int e0 = base + tid * kCohIPT;
+        // block of my first entry: last block start <= e0
+        int lo = 0, hi = nbl;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (bstart[mid] <= e0) lo = mid;
+            else hi = mid;
         }
-        for (int c = 0; c < nchunks; ++c) {
-            const int base = c * kCohChunk;
-            const int b_lo = ctab[c], b_hi = ctab[c + 1];  // blocks touching this chunk
-            const int nbl = b_hi - b_lo + 1;
-            __syncthreads();
-            for (int i = tid; i < nbl; i += kBS) bstart[i] = blocks[b_lo + i].entry;
-            if (tid == 0) bstart[nbl] = 0x7fffffff;
-            __syncthreads();
-            const int e0 = base + tid * kCohIPT;
-            // block of my first entry: last block start <= e0
-            int lo = 0, hi = nbl;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (bstart[mid] <= e0) lo = mid;
-                else hi = mid;
+        int bj[kCohIPT], cj[kCohIPT], rst[kCohIPT];
+        Acc vv[kCohIPT];
+        int b = lo;
+#pragma unroll
+        for (int j = 0; j < kCohIPT; ++j) {
+            const int e = e0 + j;
+            if (e < nnz) {
+                while (bstart[b + 1] <= e) ++b;
+                bj[j] = b;
+                rst[j] = bstart[b] == e;
+                cj[j] = cjmp[e];
+                vv[j] = (Acc)widen(val[e]);
+            } else {
+                bj[j] = b;
+                rst[j] = 0;
+                cj[j] = 0;
+                vv[j] = 0;
             }
-            int bj[kCohIPT], cj[kCohIPT], rst[kCohIPT];
-            Acc vv[kCohIPT];
-            int b = lo;
+        }
+        // scan 1: S
+        int run = 0, anyr = 0;
 #pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) {
-                const int e = e0 + j;
-                if (e < nnz) {
-                    while (bstart[b + 1] <= e) ++b;
-                    bj[j] = b;
-                    rst[j] = bstart[b] == e;
-                    cj[j] = cjmp[e];
-                    vv[j] = (Acc)widen(val[e]);
-                } else {
-                    bj[j] = b;
-                    rst[j] = 0;
-                    cj[j] = 0;
-                    vv[j] = 0;
-                }
+        for (int j = 0; j < kCohIPT; ++j) {
+            run = rst[j] ? cj[j] : run + cj[j];
+            anyr |= rst[j];
+        }
+        const int sin = block_seg_excl(run, anyr, carry_s, tv, tf);
+        int S[kCohIPT];
+        run = sin;
+#pragma unroll
+        for (int j = 0; j < kCohIPT; ++j) {
+            run = rst[j] ? cj[j] : run + cj[j];
+            S[j] = run;
+        }
+        // row deltas at every row change
+        CohBlock blk[kCohIPT];
+        int dl[kCohIPT];
+#pragma unroll
+        for (int j = 0; j < kCohIPT; ++j) {
+            const int e = e0 + j;
+            dl[j] = 0;
+            if (e < nnz) {
+                blk[j] = blocks[b_lo + bj[j]];
+                const int kk = S[j] >> bits;
+                const bool newrow = rst[j] || (kk != ((S[j] - cj[j]) >> bits));
+                // a band holds at most nnz row segments: the clamp keeps a
+                // decode error an error in y, never a read outside the band
+                int ri = blk[j].row_jump + kk;
+                ri = ri < nnz ? ri : nnz - 1;
+                if (newrow) dl[j] = rjmp[ri];
+            } else {
+                blk[j] = CohBlock{0, 0, 0, 0};
             }
-            // scan 1: S
-            int run = 0, anyr = 0;
+        }
+        // scan 2: R
+        run = 0;
 #pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) {
-                run = rst[j] ? cj[j] : run + cj[j];
-                anyr |= rst[j];
-            }
-            const int sin = block_seg_excl(run, anyr, carry_s, tv, tf);
-            int S[kCohIPT];
-            run = sin;
+        for (int j = 0; j < kCohIPT; ++j) run = rst[j] ? dl[j] : run + dl[j];
+        const int rin = block_seg_excl(run, anyr, carry_r, tv, tf);
+        run = rin;
 #pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) {
-                run = rst[j] ? cj[j] : run + cj[j];
-                S[j] = run;
-            }
-            // row deltas at every row change
-            CohBlock blk[kCohIPT];
-            int dl[kCohIPT];
-#pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) {
-                const int e = e0 + j;
-                dl[j] = 0;
-                if (e < nnz) {
-                    blk[j] = blocks[b_lo + bj[j]];
-                    const int kk = S[j] >> bits;
-                    const bool newrow = rst[j] || (kk != ((S[j] - cj[j]) >> bits));
-                    // a band holds at most nnz row segments: the clamp keeps a
-                    // decode error an error in y, never a read outside the band
-                    int ri = blk[j].row_jump + kk;
-                    ri = ri < nnz ? ri : nnz - 1;
-                    if (newrow) dl[j] = rjmp[ri];
-                } else {
-                    blk[j] = CohBlock{0, 0, 0, 0};
-                }
-            }
-            // scan 2: R
-            run = 0;
-#pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) run = rst[j] ? dl[j] : run + dl[j];
-            const int rin = block_seg_excl(run, anyr, carry_r, tv, tf);
-            run = rin;
-#pragma unroll
-            for (int j = 0; j < kCohIPT; ++j) {
-                run = rst[j] ? dl[j] : run + dl[j];
-                const int e = e0 + j;
-                if (e < nnz) {
-                    const int col = S[j] & (int)(beta - 1);
-                    const int row = (int)(blk[j].brow * beta) + run;  // row inside the band
-                    if (row >= wbase && row < wbase + wrows) {
-                        int64_t xi = (int64_t)blk[j].bcol * beta + col;
-                        xi = xi < a.noc ? xi : a.noc - 1;
-                        const Acc xv = (Acc)x[xi];
-                        atomicAdd(&ylds[row - wbase], vv[j] * xv);
+        for (int j = 0; j < kCohIPT; ++j) {
+            run = rst[j] ? dl[j] : run + dl[j];
+            const int e = e0 + j;
+            if (e < nnz) {
+                const int col = S[j] & (int)(beta - 1);
+                const int row = (int)(blk[j].brow * beta) + run;  // row inside the band
+                if (row >= 0 && row < band.rows) {
+                    int64_t xi = (int64_t)blk[j].bcol * beta + col;
+                    xi = xi < a.noc ? xi : a.noc - 1;
+                    const Acc p = vv[j] * (Acc)x[xi];
+                    if (lds) {
+                        atomicAdd(&ylds[row], p);
+                    } else {
+                        // atomic tasks add the scaled product: y rows of big
+                        // bands are final once every task has finished
+                        atomicAdd(&y[(int64_t)band.start + row], POWER ? (X)((double)p * scale) : (X)p);
                     }
                 }
-                // the entry that closes the chunk leaves its S and R for the next chunk
-                if (e == base + kCohChunk - 1 || e == nnz - 1) {
-                    carry_s = S[j];
-                    carry_r = run;
-                }
+            }
+            // the entry that closes the chunk leaves its S and R for the next chunk
+            if (e == base + kCohChunk - 1 || e == nnz - 1) {
+                carry_s = S[j];
+                carry_r = run;
             }
         }
-        __syncthreads();
-        for (int i = tid; i < wrows; i += kBS) {
-            const int64_t row = (int64_t)band.start + wbase + i;
-            if constexpr (POWER) {
-                const X out = (X)((double)ylds[i] * scale);
-                y[row] = out;
-                const double yd = (double)out, xd = (double)x[row] * scale;
-                yy += yd * yd;
-                xy += xd * yd;
-                xx += xd * xd;
-            } else {
-                y[row] = (X)ylds[i];
-            }
+    }
+    __syncthreads();
+    if (!lds) return;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = tid; i < band.rows; i += kBS) {
+        const int64_t row = (int64_t)band.start + i;
+        if constexpr (POWER) {
+            const X out = (X)((double)ylds[i] * scale);
+            y[row] = out;
+            const double yd = (double)out, xd = (double)x[row] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        } else {
+            y[row] = (X)ylds[i];
         }
     }
     if constexpr (POWER) {
@@ -395,6 +401,60 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
             pw.partials[3 * blockIdx.x + 1] = rb;
             pw.partials[3 * blockIdx.x + 2] = rc;
         }
+    }
+}
+
+// Zero the rows of the bands that atomic tasks accumulate into (one workgroup
+// per such band), and in power mode clear the atomic tasks' partial slots.
+template <typename VT, bool POWER>
+__global__ __launch_bounds__(kBS) void k_coh_zero(CohArgs a, const int32_t* __restrict__ atomic_bands,
+                                                  typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    X* y = y_arg;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+    }
+    const CohBand band = a.bands[atomic_bands[blockIdx.x]];
+    for (int i = threadIdx.x; i < band.rows; i += kBS) y[(int64_t)band.start + i] = (X)0;
+    if constexpr (POWER)
+        for (int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x; t < a.n_tasks; t += (int64_t)gridDim.x * kBS)
+            if (a.tasks[t].atomic) {
+                pw.partials[3 * t + 0] = 0.0;
+                pw.partials[3 * t + 1] = 0.0;
+                pw.partials[3 * t + 2] = 0.0;
+            }
+}
+
+// Power mode: |y|^2, x^T y, x^T x over the rows of atomic bands, into the
+// partial slots after the tasks' (slot n_tasks + i for atomic band i).
+template <typename VT>
+__global__ __launch_bounds__(kBS) void k_coh_stats(CohArgs a, const int32_t* __restrict__ atomic_bands, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    __shared__ double red[16];
+    if (pw.ctrl[1]) return;
+    const int k = pw.ctrl[0];
+    const X* x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+    const X* y = (const X*)((k & 1) ? pw.buf0 : pw.buf1);
+    const double scale = pwmk::step_scale(pw, k);
+    const CohBand band = a.bands[atomic_bands[blockIdx.x]];
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = threadIdx.x; i < band.rows; i += kBS) {
+        const int64_t row = (int64_t)band.start + i;
+        const double yd = (double)y[row], xd = (double)x[row] * scale;
+        yy += yd * yd;
+        xy += xd * yd;
+        xx += xd * xd;
+    }
+    const double ra = pwmk::block_sum<kBS>(yy, red);
+    const double rb = pwmk::block_sum<kBS>(xy, red);
+    const double rc = pwmk::block_sum<kBS>(xx, red);
+    if (threadIdx.x == 0) {
+        const int64_t slot = a.n_tasks + blockIdx.x;
+        pw.partials[3 * slot + 0] = ra;
+        pw.partials[3 * slot + 1] = rb;
+        pw.partials[3 * slot + 2] = rc;
     }
 }
 
@@ -423,23 +483,34 @@ hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const PowerArgs*
 }
 
 template <typename VT>
-static hipError_t coh_t(const CohArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+static hipError_t coh_t(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,
+                        const PowerArgs* pw, hipStream_t s) {
     using X = typename Arith<VT>::X;
-    if (a.n_bands <= 0) return hipSuccess;
-    if (pw)
-        hipLaunchKernelGGL((k_coh<VT, true>), dim3((unsigned)a.n_bands), dim3(kBS), 0, s, a, (const X*)nullptr,
+    if (a.n_tasks <= 0) return hipSuccess;
+    if (pw) {
+        if (n_atomic > 0)
+            hipLaunchKernelGGL((k_coh_zero<VT, true>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, (X*)nullptr,
+                               *pw);
+        hipLaunchKernelGGL((k_coh<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
                            (X*)nullptr, *pw);
-    else
-        hipLaunchKernelGGL((k_coh<VT, false>), dim3((unsigned)a.n_bands), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
+        if (n_atomic > 0)
+            hipLaunchKernelGGL((k_coh_stats<VT>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, *pw);
+    } else {
+        if (n_atomic > 0)
+            hipLaunchKernelGGL((k_coh_zero<VT, false>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, (X*)y,
+                               PowerArgs{});
+        hipLaunchKernelGGL((k_coh<VT, false>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
                            PowerArgs{});
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_coh(const CohArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+hipError_t launch_coh(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,
+                      const PowerArgs* pw, hipStream_t s) {
     switch (a.vtype) {
-        case 0: return coh_t<double>(a, x, y, pw, s);
-        case 1: return coh_t<float>(a, x, y, pw, s);
-        case 2: return coh_t<bf16_t>(a, x, y, pw, s);
+        case 0: return coh_t<double>(a, atomic_bands, n_atomic, x, y, pw, s);
+        case 1: return coh_t<float>(a, atomic_bands, n_atomic, x, y, pw, s);
+        case 2: return coh_t<bf16_t>(a, atomic_bands, n_atomic, x, y, pw, s);
     }
     return hipErrorInvalidValue;
 }

@@ -67,14 +67,18 @@ struct CsbDev {
     int32_t slab_bits = 0;
     int32_t slabs = 1, pieces_per_block = 1;
     int64_t slab_rows = 0;
+    int32_t groups = 1;          // block-column groups (one per XCD)
+    DevBuf ypart;                // groups x nor partial sums
     void clear() {
         bits = 0;
         H = V = 0;
         balanced = 1;
+        groups = 1;
         ind.release();
         data.release();
         blk_ptr.release();
         work.release();
+        ypart.release();
         n_tasks = 0;
         slab_bits = 0;
     }

@@ -29,8 +29,15 @@ struct CsbArgs {
     int32_t np_shift = 0;    // log2(np)
     const uint32_t* ind = nullptr;
     const void* val = nullptr;
-    const Piece* pieces = nullptr;  // [task][bc][piece]
-    int64_t n_tasks = 0;
+    const Piece* pieces = nullptr;  // [slab task][bc][piece]
+    int64_t n_tasks = 0;            // slab tasks x column groups
+    // Column groups: task t works on slab t / groups and block columns
+    // [g*H/groups, (g+1)*H/groups) with g = t % groups.  MI355X dispatches
+    // workgroup t to XCD t % 8, so with 8 groups each XCD gathers only from
+    // its own eighth of x.  With groups > 1 the slab sums go to
+    // ypart[g * nor + row] and k_csb_reduce adds the groups in order.
+    int32_t groups = 1;
+    void* ypart = nullptr;          // accumulator type, groups * nor
     int vtype = 0;
 };
 
@@ -78,6 +85,8 @@ struct CohArgs {
 };
 
 hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
+// workgroups of the CSB column-group reduction (= its power-method partials)
+int csb_reduce_blocks(int64_t nor);
 hipError_t launch_coh(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,
                       const pwmk::PowerArgs* pw, hipStream_t s);
 // FP64 reduction of the per-workgroup power-method partials (one workgroup).

@@ -201,8 +201,12 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
                 }
             }
     });
-    c.n_tasks = V * S;
+    // one block-column group per XCD (fewer when there are fewer block columns)
+    c.groups = (int32_t)std::min<int64_t>(8, H);
+    c.n_tasks = V * S * c.groups;
     hipError_t e;
+    if (c.groups > 1 && (e = c.ypart.alloc((size_t)c.groups * nor * vector_size(h->vtype))) != hipSuccess)
+        return hip_fail(h, e, "CSB partial sums");
     if ((e = upload(c.ind, ind.data(), nnz * 4)) != hipSuccess || (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
         (e = upload(c.work, pieces.data(), pieces.size() * sizeof(blocked::Piece))) != hipSuccess)
         return hip_fail(h, e, "CSB upload");
@@ -489,7 +493,9 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
     return PWM_OK;
 }
 
-int64_t csb_partials(const pwm_matrix* h) { return h->csb.n_tasks; }
+int64_t csb_partials(const pwm_matrix* h) {
+    return h->csb.groups > 1 ? blocked::csb_reduce_blocks(h->nor) : h->csb.n_tasks;
+}
 int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks + h->coh.n_atomic; }
 
 hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
@@ -506,9 +512,11 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.val = h->csb.data.p;
     a.pieces = h->csb.work.as<blocked::Piece>();
     a.n_tasks = h->csb.n_tasks;
+    a.groups = h->csb.groups;
+    a.ypart = h->csb.ypart.p;
     a.vtype = h->vtype;
     hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream);
-    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)a.n_tasks, *pw, h->stream);
+    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);
     return e;
 }
 

@@ -125,7 +125,8 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     const VT* __restrict__ val = (const VT*)a.val;
     const uint32_t* __restrict__ ind = a.ind;
     const int tid = threadIdx.x;
-    const int64_t t = blockIdx.x;
+    const int g = (int)(blockIdx.x % a.groups);
+    const int64_t t = blockIdx.x / a.groups;  // slab task
     const int64_t br = t / a.slabs;
     const int s = (int)(t % a.slabs);
     const int64_t beta = (int64_t)1 << a.bits;
@@ -133,8 +134,9 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     const int slab_base = s * h;
     for (int i = tid; i < h; i += kBS) ylds[i] = 0;
 
-    const int64_t npieces = a.H * a.np;
-    const Piece* __restrict__ P = a.pieces + t * npieces;
+    const int64_t bc_lo = g * a.H / a.groups, bc_hi = (g + 1) * a.H / a.groups;
+    const int64_t npieces = (bc_hi - bc_lo) * a.np;
+    const Piece* __restrict__ P = a.pieces + t * a.H * a.np + bc_lo * a.np;
     for (int64_t w0 = 0; w0 < npieces; w0 += kWin) {
         const int nw = (int)(npieces - w0 < kWin ? npieces - w0 : kWin);
         __syncthreads();
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
             Acc xv[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int64_t bc = (w0 + pc[u]) >> a.np_shift;
+                const int64_t bc = bc_lo + ((w0 + pc[u]) >> a.np_shift);
                 xv[u] = ok[u] ? (Acc)x[bc * beta + (w[u] & 0xffffu)] : (Acc)0;
             }
 #pragma unroll
@@ -195,6 +197,15 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     __syncthreads();
 
     const int64_t row0 = br * beta + slab_base;
+    if (a.groups > 1) {  // partial slab sums; k_csb_reduce finishes y
+        Acc* yp = (Acc*)a.ypart + (int64_t)g * a.nor;
+        for (int i = tid; i < h; i += kBS) {
+            const int64_t row = row0 + i;
+            if (row >= a.nor) break;
+            yp[row] = ylds[i];
+        }
+        return;
+    }
     double yy = 0.0, xy = 0.0, xx = 0.0;
     for (int i = tid; i < h; i += kBS) {
         const int64_t row = row0 + i;
@@ -220,6 +231,57 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
             pw.partials[3 * t + 2] = rc;
         }
     }
+}
+
+// y[r] = sum over the column groups of ypart[g][r], groups in order (and the
+// power-method scale and partial sums).
+template <typename VT, bool POWER>
+__global__ __launch_bounds__(kBS) void k_csb_reduce(CsbArgs a, typename Arith<VT>::X* __restrict__ y_arg,
+                                                    PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    __shared__ double red[16];
+    X* __restrict__ y = y_arg;
+    const X* __restrict__ x = nullptr;
+    double scale = 1.0;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = pwmk::step_scale(pw, k);
+    }
+    const Acc* __restrict__ yp = (const Acc*)a.ypart;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * kBS + threadIdx.x; r < a.nor; r += (int64_t)gridDim.x * kBS) {
+        Acc s = 0;
+        for (int g = 0; g < a.groups; ++g) s += yp[(int64_t)g * a.nor + r];
+        if constexpr (POWER) {
+            const X out = (X)((double)s * scale);
+            y[r] = out;
+            const double yd = (double)out, xd = (double)x[r] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        } else {
+            y[r] = (X)s;
+        }
+    }
+    if constexpr (POWER) {
+        const double ra = pwmk::block_sum<kBS>(yy, red);
+        const double rb = pwmk::block_sum<kBS>(xy, red);
+        const double rc = pwmk::block_sum<kBS>(xx, red);
+        if (threadIdx.x == 0) {
+            pw.partials[3 * blockIdx.x + 0] = ra;
+            pw.partials[3 * blockIdx.x + 1] = rb;
+            pw.partials[3 * blockIdx.x + 2] = rc;
+        }
+    }
+}
+
+int csb_reduce_blocks(int64_t nor) {
+    const int64_t b = (nor + kBS - 1) / kBS;
+    return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);
 }
 
 // ---------------------------------------------------------------- BlockCOH
@@ -470,6 +532,13 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     else
         hipLaunchKernelGGL((k_csb<VT, false>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
                            PowerArgs{});
+    if (a.groups > 1) {
+        const unsigned rb = (unsigned)csb_reduce_blocks(a.nor);
+        if (pw)
+            hipLaunchKernelGGL((k_csb_reduce<VT, true>), dim3(rb), dim3(kBS), 0, s, a, (X*)nullptr, *pw);
+        else
+            hipLaunchKernelGGL((k_csb_reduce<VT, false>), dim3(rb), dim3(kBS), 0, s, a, (X*)y, PowerArgs{});
+    }
     return hipGetLastError();
 }
 

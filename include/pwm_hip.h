@@ -65,7 +65,7 @@ typedef struct {
     int64_t nor, noc, nnz;     /* SparseMatrix fields (Matrix/SparseMatrix.hpp:17-23) */
     int32_t format, value_type;
     int32_t threads;           /* partitions actually used (tiles, bands) */
-    int32_t group_size;        /* CRS vector kernel: lanes per row */
+    int32_t group_size;        /* CRS vector kernel: lanes per row; CSB: block-column groups */
     int64_t device_bytes;      /* HBM held by the handle */
     int64_t long_rows;         /* rows routed to the long-row path */
     int32_t block_bits;        /* CSB beta = 2^block_bits (0 when not CSB) */
@@ -114,6 +114,8 @@ int pwm_info(const pwm_matrix* h, pwm_info_t* out);
 /* Options (addition).  Keys: "group_size" (CRS vector kernel lanes per row,
  * 0 = automatic), "long_row" (CRS: rows longer than this use the long-row
  * path), "items_per_thread" (merge path), "block_bits" (CSB beta override),
+ * "csb_groups" (CSB block-column groups pinned to XCDs, 0 = time 1/8/16 at
+ * load and keep the fastest),
  * "autotune" (1 = time the candidate kernels on the next call and keep the
  * fastest).  Must be set before the matrix is loaded unless noted. */
 int pwm_set_option(pwm_matrix* h, const char* key, int64_t value);

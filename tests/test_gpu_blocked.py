@@ -126,7 +126,9 @@ def test_spmv_random_and_stencil(cls):
     for A in cases:
         x = pw.gen_uniform(A.noc, 0)
         ref = oracle.crs_mv(A, x)
-        for opt in ({}, {"block_bits": 7}) if cls is pw.CSBHip else ({}, ):
+        csb_opts = ({}, {"block_bits": 7}, {"csb_groups": 1}, {"csb_groups": 3}, {"csb_groups": 8},
+                    {"csb_groups": 16}, {"block_bits": 7, "csb_groups": 16})
+        for opt in csb_opts if cls is pw.CSBHip else ({}, ):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)
                 y = np.zeros(A.nor)
@@ -197,10 +199,12 @@ def test_power_iteration_state(cls, it):
     A.data[:] = np.abs(A.data)
     x0 = pw.gen_uniform(A.nor, 3)
     xr, yr = oracle.crs_power_iteration(A, x0, it)
-    m = _load(cls, A)
-    x, y = x0.copy(), np.zeros(A.nor)
-    m.powerIteration(x, y, it)
-    assert _relerr(x, xr) <= 1e-12 and _relerr(y, yr) <= 1e-12
+    for opt in (({"csb_groups": 1}, {"csb_groups": 8, "block_bits": 9}, {"csb_groups": 16, "block_bits": 8})
+                if cls is pw.CSBHip else ({}, {"long_chunk": 256})[:1]):
+        m = _load(cls, A, **opt)
+        x, y = x0.copy(), np.zeros(A.nor)
+        m.powerIteration(x, y, it)
+        assert _relerr(x, xr) <= 1e-12 and _relerr(y, yr) <= 1e-12, opt
 
 
 @pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)

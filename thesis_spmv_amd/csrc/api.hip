@@ -256,6 +256,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "block_bits") {
         if (value < 0 || value > 16) return fail(h, PWM_ERR_INVALID, "block_bits must be 0..16");
         h->csb_bits_override = (int32_t)value;
+    } else if (k == "csb_groups") {
+        if (value < 0 || value > 64) return fail(h, PWM_ERR_INVALID, "csb_groups must be 0..64");
+        h->csb_groups = (int32_t)value;
     } else if (k == "autotune") {
         h->autotune = value != 0;
     } else {
@@ -452,7 +455,7 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
     out->nnz = h->nnz;
     out->format = h->format;
     out->value_type = h->vtype;
-    out->group_size = h->group;
+    out->group_size = h->format == PWM_FORMAT_CSB ? h->csb.groups : h->group;
     out->long_rows = h->lr.n_rows;
     out->threads = h->format == PWM_FORMAT_CRS_MERGE ? (int32_t)h->mp.P
                    : h->format == PWM_FORMAT_BLOCKCOH ? (int32_t)h->coh.bands.size()

@@ -139,6 +139,7 @@ struct pwm_matrix {
 
     // blocked formats
     int32_t csb_bits_override = 0;
+    int32_t csb_groups = 0;  // 0 = autotune
     pwm_impl::CsbDev csb;
     pwm_impl::CohDev coh;
 

@@ -201,11 +201,20 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
                 }
             }
     });
-    // one block-column group per XCD (fewer when there are fewer block columns)
-    c.groups = (int32_t)std::min<int64_t>(8, H);
+    // block-column groups: the caller's count, else the fastest of 1, 8 and
+    // 16 (timed below, once the arrays are on the device)
+    std::vector<int32_t> cands;
+    if (h->csb_groups > 0)
+        cands.push_back((int32_t)std::min<int64_t>(h->csb_groups, H));
+    else
+        for (int64_t gcount : {(int64_t)1, (int64_t)8, (int64_t)16})
+            if (std::find(cands.begin(), cands.end(), (int32_t)std::min(gcount, H)) == cands.end())
+                cands.push_back((int32_t)std::min(gcount, H));
+    const int32_t gmax = *std::max_element(cands.begin(), cands.end());
+    c.groups = cands[0];
     c.n_tasks = V * S * c.groups;
     hipError_t e;
-    if (c.groups > 1 && (e = c.ypart.alloc((size_t)c.groups * nor * vector_size(h->vtype))) != hipSuccess)
+    if (gmax > 1 && (e = c.ypart.alloc((size_t)gmax * nor * vector_size(h->vtype))) != hipSuccess)
         return hip_fail(h, e, "CSB partial sums");
     if ((e = upload(c.ind, ind.data(), nnz * 4)) != hipSuccess || (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
         (e = upload(c.work, pieces.data(), pieces.size() * sizeof(blocked::Piece))) != hipSuccess)
@@ -216,6 +225,39 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     c.pieces_per_block = (int32_t)np;
     c.slabs = (int32_t)S;
     c.slab_rows = slab;
+    if (cands.size() > 1) {
+        // autotune: a few timed SpMVs per candidate on x = 1
+        DevBuf tx, ty;
+        const size_t vs = vector_size(h->vtype);
+        if ((e = tx.alloc(noc * vs)) != hipSuccess || (e = ty.alloc(nor * vs)) != hipSuccess)
+            return hip_fail(h, e, "CSB autotune vectors");
+        (void)pwmk::launch_fill(tx.p, noc, h->vtype == PWM_VALUES_F64, 1.0, h->stream);
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        float best = 1e30f;
+        int32_t best_g = cands[0];
+        for (int32_t gc : cands) {
+            c.groups = gc;
+            c.n_tasks = V * S * gc;
+            for (int w = 0; w < 2; ++w) (void)csb_launch(h, tx.p, ty.p, nullptr);
+            (void)hipEventRecord(e0, h->stream);
+            for (int r = 0; r < 5; ++r) (void)csb_launch(h, tx.p, ty.p, nullptr);
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                best_g = gc;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c.groups = best_g;
+        c.n_tasks = V * S * best_g;
+        if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(h, e, "CSB autotune");
+    }
     // the CRS arrays are not needed by the CSB kernels
     h->rs.release();
     h->ci.release();

@@ -125,8 +125,20 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     const VT* __restrict__ val = (const VT*)a.val;
     const uint32_t* __restrict__ ind = a.ind;
     const int tid = threadIdx.x;
-    const int g = (int)(blockIdx.x % a.groups);
-    const int64_t t = blockIdx.x / a.groups;  // slab task
+    // workgroup -> (slab task, column group).  With a multiple of 8 groups the
+    // launch runs in rounds of 8 groups so that XCD (b % 8) keeps one group
+    // per round: b = round * 8 * slabs_total + slab * 8 + xcd.
+    int g;
+    int64_t t;
+    if (a.groups % 8 == 0) {
+        const int64_t per_round = (a.n_tasks / a.groups) * 8;
+        const int64_t round = blockIdx.x / per_round, rem = blockIdx.x % per_round;
+        t = rem / 8;
+        g = (int)(round * 8 + rem % 8);
+    } else {
+        g = (int)(blockIdx.x % a.groups);
+        t = blockIdx.x / a.groups;
+    }
     const int64_t br = t / a.slabs;
     const int s = (int)(t % a.slabs);
     const int64_t beta = (int64_t)1 << a.bits;

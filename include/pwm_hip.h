@@ -189,10 +189,39 @@ int pwm_gen_powerlaw(int64_t n, double alpha, double xm, int64_t cap, uint64_t s
                      int64_t* nnz);
 int pwm_gen_stencil9(int64_t side, int64_t extra, uint64_t seed, int32_t** row_start, int32_t** col_ind,
                      double** vals, int64_t* nnz);
+/* Symmetric positive definite power-law matrix (batch workload): the
+ * power-law row law, half of each row drawn off the diagonal and mirrored,
+ * values +-U(0,1] (duplicates summed), diagonal = row absolute sum + 1. */
+int pwm_gen_spd_powerlaw(int64_t n, uint64_t seed, int32_t** row_start, int32_t** col_ind, double** vals,
+                         int64_t* nnz);
 /* x ~ U[-1,1] from a 64-bit Mersenne twister with the given seed:
  * (u >> 11) * 2^-53 * 2 - 1. */
 int pwm_gen_uniform(int64_t n, uint64_t seed, double* out);
 void pwm_free(void* p);
+
+/* ---- batched power method (addition) ----------------------------------
+ * Many independent power methods (the reference's powerIteration,
+ * Matrix/CRS.hpp:118-130, plus the Rayleigh-quotient stopping rule of
+ * pwm_power_method_device) solved in one persistent launch: each workgroup
+ * takes whole instances from a queue, largest first.  value_type F32 or
+ * BF16 (FP32 vectors and row sums) or F64; norms and Rayleigh quotients are
+ * FP64.  Instances are square.  Not re-entrant per batch. */
+typedef struct pwm_batch pwm_batch;
+int pwm_batch_create(int value_type, pwm_batch** out);
+int pwm_batch_destroy(pwm_batch* b);
+const char* pwm_batch_last_error(const pwm_batch* b);
+int pwm_batch_add_crs(pwm_batch* b, int64_t n, const int32_t* row_start, const int32_t* col_ind, const double* vals,
+                      int32_t* id);
+/* 2D Poisson instance generated on the device (Util/Poisson.hpp:32-83). */
+int pwm_batch_add_poisson(pwm_batch* b, int64_t m, int64_t n, int32_t* id);
+/* Start vectors x0[i] = U[-1,1) hashed from (seed, id, i); stop at relative
+ * eigenvalue change <= tol (tol < 0: never) or max_it iterations.  Outputs
+ * (per instance, in id order) may be NULL; *ms = device time of the launch. */
+int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_t seed, double* lambdas, int64_t* iterations,
+                    double* ms);
+int pwm_batch_start_vector(pwm_batch* b, int32_t id, uint64_t seed, double* out);
+int pwm_batch_vector(pwm_batch* b, int32_t id, double* out, double* ynorm);
+int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_t* nnz);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,23 @@ def test_gen_powerlaw_invariants():
     assert 0.5 * (7.3333 / 100) ** 1.8 < frac < 2 * (7.3333 / 100) ** 1.8
 
 
+def test_gen_spd_powerlaw_is_spd():
+    import scipy.sparse as sp
+    n = 3000
+    rs, ci, dv = pw.gen_spd_powerlaw(n, seed=4)
+    A = sp.csr_matrix((dv, ci, rs), shape=(n, n))
+    assert abs(A - A.T).max() == 0
+    d = A.diagonal()
+    off = np.asarray(abs(A).sum(axis=1)).ravel() - np.abs(d)
+    assert np.all(d > 0) and np.all(d >= off + 1 - 1e-9)
+    assert np.all(np.diff(ci[rs[0]:rs[1]]) > 0)
+    assert 10 < A.nnz / n < 30
+    B = A[:400, :400].toarray()
+    np.linalg.cholesky(B)  # a principal submatrix of an SPD matrix is SPD
+    rs2, ci2, dv2 = pw.gen_spd_powerlaw(n, seed=4)
+    assert np.array_equal(ci, ci2) and np.array_equal(dv, dv2)
+
+
 def test_gen_stencil9_structure():
     side, extra = 37, 2
     rs, ci, dv = pw.gen_stencil9(side, extra, 3)

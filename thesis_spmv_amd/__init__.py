@@ -68,8 +68,19 @@ _SIGS = {
                          C.POINTER(VP), C.POINTER(I64)],
     "pwm_gen_stencil9": [I64, I64, C.c_uint64, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(I64)],
     "pwm_gen_uniform": [I64, C.c_uint64, VP],
+    "pwm_gen_spd_powerlaw": [I64, C.c_uint64, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(I64)],
     "pwm_free": [VP],
+    "pwm_batch_create": [C.c_int, C.POINTER(VP)],
+    "pwm_batch_destroy": [VP],
+    "pwm_batch_last_error": [VP],
+    "pwm_batch_add_crs": [VP, I64, VP, VP, VP, C.POINTER(I32)],
+    "pwm_batch_add_poisson": [VP, I64, I64, C.POINTER(I32)],
+    "pwm_batch_solve": [VP, I64, F64, C.c_uint64, VP, VP, C.POINTER(F64)],
+    "pwm_batch_start_vector": [VP, I32, C.c_uint64, VP],
+    "pwm_batch_vector": [VP, I32, VP, C.POINTER(F64)],
+    "pwm_batch_size": [VP, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
 }
+_CHAR_RESULT = {"pwm_last_error", "pwm_batch_last_error"}
 
 
 def lib():
@@ -82,7 +93,7 @@ def lib():
         for name, args in _SIGS.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = C.c_char_p if name == "pwm_last_error" else (None if name == "pwm_free" else C.c_int)
+            fn.restype = C.c_char_p if name in _CHAR_RESULT else (None if name == "pwm_free" else C.c_int)
         _lib = L
     return _lib
 
@@ -346,6 +357,72 @@ def gen_stencil9(side, extra=2, seed=3):
     _check(lib().pwm_gen_stencil9(int(side), int(extra), int(seed), C.byref(rs), C.byref(ci), C.byref(dv),
                                   C.byref(nnz)))
     return _take_crs(int(side) * int(side), rs, ci, dv, nnz)
+
+
+def gen_spd_powerlaw(n, seed=4):
+    """Symmetric positive definite power-law CRS (diagonal = row |sum| + 1)."""
+    rs, ci, dv, nnz = VP(), VP(), VP(), I64()
+    _check(lib().pwm_gen_spd_powerlaw(int(n), int(seed), C.byref(rs), C.byref(ci), C.byref(dv), C.byref(nnz)))
+    return _take_crs(int(n), rs, ci, dv, nnz)
+
+
+class Batch:
+    """Many power methods in one persistent launch (pwm_batch_* in pwm_hip.h)."""
+
+    def __init__(self, values="f32"):
+        self.h = VP()
+        _check(lib().pwm_batch_create(VALUES[values], C.byref(self.h)))
+        self.values = values
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pwm_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PwmError(rc, lib().pwm_batch_last_error(self.h).decode())
+
+    def add_crs(self, row_start, col_ind, data):
+        rs = np.ascontiguousarray(row_start, np.int32)
+        ci = np.ascontiguousarray(col_ind, np.int32)
+        dv = np.ascontiguousarray(data, np.float64)
+        i = I32()
+        self._chk(lib().pwm_batch_add_crs(self.h, len(rs) - 1, _ptr(rs), _ptr(ci), _ptr(dv), C.byref(i)))
+        return i.value
+
+    def add_poisson(self, m, n):
+        i = I32()
+        self._chk(lib().pwm_batch_add_poisson(self.h, int(m), int(n), C.byref(i)))
+        return i.value
+
+    def size(self):
+        a, b, c = I32(), I64(), I64()
+        self._chk(lib().pwm_batch_size(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def solve(self, max_it, tol=-1.0, seed=0):
+        n = self.size()[0]
+        lam = np.zeros(n)
+        its = np.zeros(n, np.int64)
+        ms = F64()
+        self._chk(lib().pwm_batch_solve(self.h, int(max_it), float(tol), int(seed), _ptr(lam), _ptr(its),
+                                        C.byref(ms)))
+        return lam, its, ms.value
+
+    def start_vector(self, i, n, seed=0):
+        out = np.zeros(int(n))
+        self._chk(lib().pwm_batch_start_vector(self.h, int(i), int(seed), _ptr(out)))
+        return out
+
+    def vector(self, i, n):
+        out = np.zeros(int(n))
+        yn = F64()
+        self._chk(lib().pwm_batch_vector(self.h, int(i), _ptr(out), C.byref(yn)))
+        return out, yn.value
 
 
 def gen_uniform(n, seed=0):

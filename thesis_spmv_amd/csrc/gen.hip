@@ -141,6 +141,108 @@ extern "C" int pwm_gen_powerlaw(int64_t n, double alpha, double xm, int64_t cap,
     return PWM_OK;
 }
 
+// Symmetric positive definite power-law matrix: Pareto row lengths (as
+// pwm_gen_powerlaw, capped at n-1), half of each row's entries drawn
+// uniformly off the diagonal and mirrored, values +-U(0,1] (duplicates
+// summed), then the diagonal set to the row's absolute sum + 1 (strict
+// diagonal dominance with a positive diagonal, hence SPD).
+extern "C" int pwm_gen_spd_powerlaw(int64_t n, uint64_t seed, int32_t** row_start, int32_t** col_ind, double** vals,
+                                    int64_t* nnz) {
+    if (n <= 1 || n > (1 << 26) || !row_start || !col_ind || !vals || !nnz)
+        return pwm_impl::fail(nullptr, PWM_ERR_INVALID, "pwm_gen_spd_powerlaw: bad argument");
+    struct E {
+        int32_t r, c;
+        double v;
+    };
+    std::vector<std::vector<E>> local(16);  // one list per parallel_for worker
+    // draw the upper/lower halves per row, keep mirrors per worker
+    pwm_impl::parallel_for(n, 2048, [&](int64_t b, int64_t e, int w) {
+        std::vector<int32_t> cols;
+        const std::vector<int32_t> none;
+        for (int64_t r = b; r < e; ++r) {
+            uint64_t st = row_stream(seed, r, 5);
+            double L = floor(7.3333 * pow(u01_open(st), -1.0 / 1.8));
+            L = std::min(L, (double)std::min<int64_t>(50000, n - 1));
+            const int64_t half = (int64_t)(L + 1) / 2;
+            std::vector<int32_t> ex(1, (int32_t)r);
+            sample_columns(st, n, half, ex, cols);
+            for (int32_t c : cols) {
+                const double mag = u01_open(st);
+                const double v = (splitmix64(st) & 1) ? mag : -mag;
+                local[w].push_back({(int32_t)r, c, v});
+                local[w].push_back({c, (int32_t)r, v});
+            }
+        }
+    });
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (auto& L : local)
+        for (const E& x : L) cnt[x.r + 1]++;
+    for (int64_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+    std::vector<E> all(cnt[n]);
+    {
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (auto& L : local)
+            for (const E& x : L) all[pos[x.r]++] = x;
+    }
+    local.clear();
+    // per row: sort by column, merge duplicates, add the diagonal
+    std::vector<int64_t> rowlen(n, 0);
+    pwm_impl::parallel_for(n, 2048, [&](int64_t b, int64_t e, int) {
+        for (int64_t r = b; r < e; ++r) {
+            E* s = all.data() + cnt[r];
+            E* t = all.data() + cnt[r + 1];
+            std::sort(s, t, [](const E& p, const E& q) { return p.c < q.c; });
+            int64_t m = 0;
+            for (E* p = s; p < t; ++p) {
+                if (m > 0 && s[m - 1].c == p->c) s[m - 1].v += p->v;
+                else s[m++] = *p;
+            }
+            rowlen[r] = m + 1;  // + diagonal
+        }
+    });
+    std::vector<int32_t> rs(n + 1, 0);
+    for (int64_t r = 0; r < n; ++r) {
+        if ((int64_t)rs[r] + rowlen[r] > INT32_MAX) return pwm_impl::fail(nullptr, PWM_ERR_INVALID, "spd_powerlaw: nnz >= 2^31");
+        rs[r + 1] = rs[r] + (int32_t)rowlen[r];
+    }
+    const int64_t Z = rs[n];
+    int32_t* ci = (int32_t*)malloc(sizeof(int32_t) * Z);
+    double* dv = (double*)malloc(sizeof(double) * Z);
+    if (!ci || !dv) {
+        free(ci);
+        free(dv);
+        return pwm_impl::fail(nullptr, PWM_ERR_NOMEM, "pwm_gen_spd_powerlaw: host allocation");
+    }
+    pwm_impl::parallel_for(n, 2048, [&](int64_t b, int64_t e, int) {
+        for (int64_t r = b; r < e; ++r) {
+            const E* s = all.data() + cnt[r];
+            const int64_t m = rowlen[r] - 1;
+            double asum = 0.0;
+            for (int64_t k = 0; k < m; ++k) asum += fabs(s[k].v);
+            int64_t o = rs[r];
+            bool placed = false;
+            for (int64_t k = 0; k < m; ++k) {
+                if (!placed && s[k].c > r) {
+                    ci[o] = (int32_t)r;
+                    dv[o++] = asum + 1.0;
+                    placed = true;
+                }
+                ci[o] = s[k].c;
+                dv[o++] = s[k].v;
+            }
+            if (!placed) {
+                ci[o] = (int32_t)r;
+                dv[o++] = asum + 1.0;
+            }
+        }
+    });
+    *row_start = dup_out(rs);
+    *col_ind = ci;
+    *vals = dv;
+    *nnz = Z;
+    return PWM_OK;
+}
+
 extern "C" int pwm_gen_stencil9(int64_t side, int64_t extra, uint64_t seed, int32_t** row_start, int32_t** col_ind,
                                 double** vals, int64_t* nnz) {
     if (side <= 0 || extra < 0 || side * side > INT32_MAX - 1 || !row_start || !col_ind || !vals || !nnz)

@@ -31,6 +31,10 @@ __device__ __forceinline__ int dpp_i32(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL>
+__device__ __forceinline__ int dpp(int v) {
+    return dpp_i32<CTRL>(v);
+}
+template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
     return __int_as_float(dpp_i32<CTRL>(__float_as_int(v)));
 }
@@ -44,6 +48,7 @@ __device__ __forceinline__ double dpp(double v) {
 
 // ds_swizzle in bit-mask mode: lane i reads lane (i ^ 16) inside each
 // 32-lane half (and_mask 0x1f, or_mask 0, xor_mask 0x10).
+__device__ __forceinline__ int swz16(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x401F); }
 __device__ __forceinline__ float swz16(float v) {
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
 }

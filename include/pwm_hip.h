@@ -202,10 +202,12 @@ void pwm_free(void* p);
 /* ---- batched power method (addition) ----------------------------------
  * Many independent power methods (the reference's powerIteration,
  * Matrix/CRS.hpp:118-130, plus the Rayleigh-quotient stopping rule of
- * pwm_power_method_device) solved in one persistent launch: each workgroup
- * takes whole instances from a queue, largest first.  value_type F32 or
- * BF16 (FP32 vectors and row sums) or F64; norms and Rayleigh quotients are
- * FP64.  Instances are square.  Not re-entrant per batch. */
+ * pwm_power_method_device) solved together: every step is one launch over
+ * the merge-path tiles of all unfinished instances plus one launch that
+ * reduces each instance's norms (FP64) and applies its stop test; chunks of
+ * steps are replayed as one hipGraph.  Each instance's result is
+ * bit-identical to solving it alone.  value_type F32 or BF16 (FP32 vectors
+ * and row sums) or F64.  Instances are square.  Not re-entrant per batch. */
 typedef struct pwm_batch pwm_batch;
 int pwm_batch_create(int value_type, pwm_batch** out);
 int pwm_batch_destroy(pwm_batch* b);
@@ -222,6 +224,46 @@ int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_t seed, dou
 int pwm_batch_start_vector(pwm_batch* b, int32_t id, uint64_t seed, double* out);
 int pwm_batch_vector(pwm_batch* b, int32_t id, double* out, double* ynorm);
 int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_t* nnz);
+/* "items_per_thread" (4, 8, 16: tile = 256 x items merge items),
+ * "chunk" (steps per graph replay, 1..4096), "graph" (0: plain launches). */
+int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value);
+/* Kernel launches, graph instantiations and work-list builds so far. */
+int pwm_batch_stats(const pwm_batch* b, int64_t* launches, int64_t* graphs, int64_t* lists);
+
+/* ---- request server (addition) ----------------------------------------
+ * Continuous batching of power-method jobs (stop at relative eigenvalue
+ * change <= tol, or after max_it steps) and SpMV chains (tol < 0: exactly
+ * max_it normalised SpMVs) on registered matrices.  All device memory of a
+ * server (resident matrices, job vectors) comes from one region of
+ * `budget_bytes`; a matrix is made resident when its first job is admitted
+ * and stays resident, while unreferenced it may be evicted (least recently
+ * used first) to admit another job.  pwm_server_step admits queued jobs in
+ * arrival order, runs `chunk` steps of every running job and retires the
+ * finished ones; a job's result does not depend on what else is in flight.
+ * Start vectors x0[i] = U[-1,1) hashed from (seed, i).  Single-threaded: calls
+ * on one server must not overlap. */
+typedef struct pwm_server pwm_server;
+int pwm_server_create(int value_type, int64_t budget_bytes, int32_t max_jobs, pwm_server** out);
+int pwm_server_destroy(pwm_server* s);
+const char* pwm_server_last_error(const pwm_server* s);
+/* "chunk" (steps per pwm_server_step, default 8), "graph" (0: plain
+ * launches), "items_per_thread" (4, 8, 16; only with nothing resident). */
+int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
+#define PWM_SERVER_BORROW 1 /* keep the caller's arrays (alive until removed) instead of a copy */
+int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* row_start, const int32_t* col_ind,
+                          const double* vals, int32_t flags, int32_t* mat);
+/* 2D Poisson matrix, generated on the device when it becomes resident. */
+int pwm_server_add_poisson(pwm_server* s, int64_t m, int64_t n, int32_t* mat);
+int pwm_server_remove_matrix(pwm_server* s, int32_t mat);
+int pwm_server_matrix_resident(const pwm_server* s, int32_t mat, int32_t* resident);
+int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, double tol, uint64_t seed, int64_t* job);
+int pwm_server_step(pwm_server* s, int32_t* running, int32_t* finished);
+/* Finished job: eigenvalue estimate, |y| of the last step, steps taken and
+ * (vec non-NULL, n doubles) the normalised final iterate.  PWM_ERR_STATE
+ * while the job is queued or running; afterwards the job is forgotten. */
+int pwm_server_result(pwm_server* s, int64_t job, double* lambda, double* ynorm, int64_t* iters, double* vec);
+int pwm_server_stats(const pwm_server* s, int64_t* used_bytes, int32_t* resident, int32_t* queued, int32_t* running,
+                     int64_t* finished);
 
 #ifdef __cplusplus
 }

@@ -91,8 +91,8 @@ def test_converged_eigenvalues_closed_form():
         assert its[i] < 200000 and abs(lam[i] - exact) <= 1e-9 * exact, (m, n, its[i], lam[i], exact)
 
 
-def test_queue_with_more_instances_than_workgroups():
-    """More instances than resident workgroups: each workgroup takes several."""
+def test_many_small_instances():
+    """More instances than the chip has compute units, most of one tile."""
     b = pw.Batch("f32")
     rng = np.random.default_rng(2)
     sizes = [(int(rng.integers(2, 9)), int(rng.integers(2, 9))) for _ in range(1300)]
@@ -107,6 +107,74 @@ def test_queue_with_more_instances_than_workgroups():
         xr, yr = oracle.crs_power_iteration(A, x0, 29)
         ref = _rq(A, yr)
         assert abs(lam[i] - ref) <= 1e-5 * abs(ref)
+
+
+def _mixed_batch(values, mats, **opt):
+    b = pw.Batch(values, **opt)
+    for A in mats:
+        b.add_crs(A.row_start, A.col_ind, A.data)
+    return b
+
+
+@pytest.mark.parametrize("values", ["f64", "f32"])
+def test_alone_equals_in_the_mix(values):
+    """An instance's tiles are aligned to its own start and its reductions run
+    in a fixed order, so its result is bit-identical alone and in a batch."""
+    rng = np.random.default_rng(7)
+    mats = _instances(rng)
+    b = _mixed_batch(values, mats)
+    lam, its, _ = b.solve(300, 1e-9, seed=21)
+    assert len(set(its.tolist())) > 1  # instances stop at different steps
+    for i in (0, 3, 5, len(mats) - 1):
+        # the start vector hashes (seed, instance id, row): keep the id by
+        # placing 2x2 Poisson instances before it (instance 0 runs truly alone)
+        solo = pw.Batch(values)
+        for _ in range(i):
+            solo.add_poisson(2, 2)
+        solo.add_crs(mats[i].row_start, mats[i].col_ind, mats[i].data)
+        lam1, its1, _ = solo.solve(300, 1e-9, seed=21)
+        assert its1[i] == its[i] and lam1[i] == lam[i], (i, lam1[i], lam[i])
+        v_mix, yn_mix = b.vector(i, mats[i].nor)
+        v_solo, yn_solo = solo.vector(i, mats[i].nor)
+        assert np.array_equal(v_mix, v_solo) and yn_mix == yn_solo
+
+
+def test_graph_and_chunking_do_not_change_results():
+    rng = np.random.default_rng(8)
+    mats = _instances(rng)
+    ref = None
+    for opt in ({"graph": 1, "chunk": 32}, {"graph": 0, "chunk": 32}, {"graph": 1, "chunk": 7},
+                {"graph": 0, "chunk": 1}):
+        b = _mixed_batch("f32", mats, **opt)
+        out = []
+        for rep in range(2):  # graph replays of a second solve too
+            lam, its, _ = b.solve(150, 1e-8, seed=4)
+            vecs = [b.vector(i, A.nor)[0] for i, A in enumerate(mats)]
+            out.append((lam, its, vecs))
+        st = b.stats()
+        print(f"options {opt}: stats {st}", flush=True)
+        if opt["graph"]:
+            assert st["graphs"] >= 1
+        for lam, its, vecs in out:
+            if ref is None:
+                ref = (lam, its, vecs)
+            assert np.array_equal(lam, ref[0]) and np.array_equal(its, ref[1])
+            for v, w in zip(vecs, ref[2]):
+                assert np.array_equal(v, w)
+
+
+@pytest.mark.parametrize("ipt", [4, 16])
+def test_tile_size_variants(ipt):
+    rng = np.random.default_rng(9)
+    mats = _instances(rng)
+    b = _mixed_batch("f64", mats, items_per_thread=ipt)
+    K = 40
+    lam, its, _ = b.solve(K, -1.0, seed=2)
+    for i, A in enumerate(mats):
+        x0 = b.start_vector(i, A.nor, seed=2)
+        xr, yr = oracle.crs_power_iteration(A, x0, K - 1)
+        ref = _rq(A, xr if (K - 1) % 2 == 0 else yr)
+        assert abs(lam[i] - ref) <= 1e-10 * abs(ref), (ipt, i)
 
 
 def test_errors():

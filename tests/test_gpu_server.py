@@ -1,0 +1,167 @@
+"""GPU tests of the request server (pwm_server_*): results against the FP64
+oracle (CRS::powerIteration, Matrix/CRS.hpp:118-130), bit-identical results
+alone and in a mix of concurrently running jobs, admission and eviction
+under a device-memory budget, and error handling."""
+import numpy as np
+import pytest
+
+import oracle
+
+pw = pytest.importorskip("thesis_spmv_amd")
+pytestmark = pytest.mark.gpu
+
+
+def _rq(A, v):
+    return float(v @ oracle.crs_mv(A, v) / (v @ v))
+
+
+def _drain(srv, jobs, limit=100000):
+    out = {}
+    for _ in range(limit):
+        srv.step()
+        for j in list(jobs):
+            try:
+                out[j] = srv.result(j, jobs[j])
+                del jobs[j]
+            except pw.PwmError as e:
+                assert e.code == -6  # still running
+        if not jobs:
+            return out
+    raise AssertionError("jobs did not finish")
+
+
+def _powerlaw(rng, n, long_rows=()):
+    lens = rng.integers(0, 14, n)
+    for r, L in long_rows:
+        lens[r] = L
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(L), replace=False)) for L in lens]).astype(np.int32)
+    return oracle.CRSMatrix(n, n, rs, ci, rng.random(int(rs[-1])) + 0.1)  # Perron root: a real gap
+
+
+@pytest.mark.parametrize("values,tol_lam,tol_vec", [("f64", 1e-10, 1e-12), ("f32", 2e-6, 1e-4)])
+def test_chains_and_solves_against_oracle(values, tol_lam, tol_vec):
+    rng = np.random.default_rng(1)
+    srv = pw.Server(values, budget_bytes=256 << 20, max_jobs=64)
+    mats = {srv.add_poisson(30, 20): oracle.poisson_crs(30, 20)}
+    A = _powerlaw(rng, 5000, long_rows=[(10, 4000), (4000, 2500)])
+    mats[srv.add_matrix(A.row_start, A.col_ind, A.data)] = A
+    jobs, spec = {}, {}
+    for k, (mat, L) in enumerate([(0, 1), (0, 7), (1, 1), (1, 33), (0, 64), (1, 2)]):
+        j = srv.submit(mat, L, -1.0, seed=100 + k)
+        jobs[j] = mats[mat].nor
+        spec[j] = (mat, L, 100 + k)
+    res = _drain(srv, jobs)
+    for j, (lam, yn, its, vec) in res.items():
+        mat, L, seed = spec[j]
+        A = mats[mat]
+        assert its == L
+        x0 = pw.server_start_vector(A.nor, seed)
+        if values != "f64":
+            x0 = x0.astype(np.float32).astype(np.float64)
+        # L steps: y_L = A^L x0 / prod|y_k|, normalised; lambda_L = RQ of
+        # the iterate before the last step
+        xr, yr = oracle.crs_power_iteration(A, x0, L - 1) if L > 1 else (x0, None)
+        prev = xr if (L - 1) % 2 == 0 else yr
+        ref_lam = _rq(A, prev)
+        assert abs(lam - ref_lam) <= tol_lam * abs(ref_lam), (j, lam, ref_lam)
+        y = oracle.crs_mv(A, prev / (np.linalg.norm(prev) if L > 1 else 1.0))
+        assert abs(yn - np.linalg.norm(y)) <= tol_lam * 10 * np.linalg.norm(y)
+        assert np.max(np.abs(vec - y / np.linalg.norm(y))) <= tol_vec
+
+
+def test_power_solves_converge_to_closed_form():
+    srv = pw.Server("f64", budget_bytes=64 << 20, max_jobs=8)
+    sizes = [(6, 5), (12, 9)]
+    ids = [srv.add_poisson(m, n) for m, n in sizes]
+    jobs = {srv.submit(i, 200000, 1e-13, seed=3): m * n for i, (m, n) in zip(ids, sizes)}
+    res = _drain(srv, dict(jobs))
+    for (j, _), (m, n) in zip(jobs.items(), sizes):
+        lam, _, its, _ = res[j]
+        exact = 4 + 2 * np.cos(np.pi / (m + 1)) + 2 * np.cos(np.pi / (n + 1))
+        assert its < 200000 and abs(lam - exact) <= 1e-9 * exact
+
+
+@pytest.mark.parametrize("values", ["f32", "f64"])
+def test_alone_equals_in_the_mix(values):
+    """A job's result is bit-identical whether it runs alone or joins while
+    other jobs are running and others arrive after it."""
+    rng = np.random.default_rng(5)
+    A = _powerlaw(rng, 3000, long_rows=[(5, 3000)])
+    B = _powerlaw(rng, 800)
+
+    def fresh():
+        srv = pw.Server(values, budget_bytes=128 << 20, max_jobs=32, chunk=3)
+        a = srv.add_matrix(A.row_start, A.col_ind, A.data)
+        b = srv.add_matrix(B.row_start, B.col_ind, B.data)
+        p = srv.add_poisson(40, 25)
+        return srv, a, b, p
+
+    srv, a, b, p = fresh()
+    alone = _drain(srv, {srv.submit(a, 500, 1e-9, seed=77): A.nor})
+    (lam0, yn0, it0, v0), = alone.values()
+
+    srv, a, b, p = fresh()
+    others = {srv.submit(b, 40, -1.0, seed=1): B.nor, srv.submit(p, 300, 1e-10, seed=2): 1000}
+    running_seen = []
+    for _ in range(2):
+        running_seen.append(srv.step()[0])
+    target = srv.submit(a, 500, 1e-9, seed=77)  # joins while others run
+    more = {srv.submit(b, 9, -1.0, seed=3): B.nor, srv.submit(a, 12, -1.0, seed=4): A.nor}
+    res = _drain(srv, {**others, target: A.nor, **more})
+    lam1, yn1, it1, v1 = res[target]
+    assert max(running_seen) >= 2
+    assert it1 == it0 and lam1 == lam0 and yn1 == yn0 and np.array_equal(v1, v0)
+
+
+def test_budget_admission_and_eviction():
+    # three 200x200 Poisson matrices (1.75 MB each in FP32, job vectors
+    # 0.32 MB) under a budget that holds only two of them
+    srv = pw.Server("f32", budget_bytes=4 << 20, max_jobs=16)
+    ids = [srv.add_poisson(200, 200) for _ in range(3)]
+    refs = {}
+    for k, i in enumerate(ids):
+        j = srv.submit(i, 20, -1.0, seed=k)
+        refs[j] = _drain(srv, {j: 40000})[j]
+        assert srv.resident(i)
+    st = srv.stats()
+    assert st["resident"] == 2 and st["used_bytes"] <= 4 << 20
+    assert not srv.resident(ids[0])  # least recently used, evicted
+    # the evicted matrix comes back (regenerated) with identical results
+    j = srv.submit(ids[0], 20, -1.0, seed=0)
+    again = _drain(srv, {j: 40000})[j]
+    first = refs[min(refs)]
+    assert again[0] == first[0] and np.array_equal(again[3], first[3])
+    # many jobs at once: they queue and run as memory frees up
+    jobs = {srv.submit(ids[k % 3], 15, -1.0, seed=k): 40000 for k in range(9)}
+    res = _drain(srv, dict(jobs))
+    assert len(res) == 9
+
+
+def test_errors():
+    srv = pw.Server("bf16", budget_bytes=1 << 20, max_jobs=4)
+    with pytest.raises(pw.PwmError) as e:
+        srv.add_matrix([0, 1, 2], [0, 1], [1.0, 0.1])  # 0.1 is not exact in BF16
+    assert e.value.code == -1
+    with pytest.raises(pw.PwmError) as e:
+        srv.add_poisson(1000, 1000)  # larger than the whole budget
+    assert e.value.code == -7
+    p = srv.add_poisson(10, 10)
+    with pytest.raises(pw.PwmError):
+        srv.submit(p, 0)
+    with pytest.raises(pw.PwmError):
+        srv.submit(99, 10)
+    j = srv.submit(p, 50)
+    with pytest.raises(pw.PwmError) as e:
+        srv.result(j)
+    assert e.value.code == -6
+    with pytest.raises(pw.PwmError) as e:
+        srv.remove_matrix(p)  # a job is queued on it
+    assert e.value.code == -6
+    res = _drain(srv, {j: 100})
+    assert res[j][2] == 50
+    srv.remove_matrix(p)
+    with pytest.raises(pw.PwmError):
+        srv.submit(p, 10)
+    with pytest.raises(pw.PwmError):
+        srv.result(j)  # already collected

@@ -79,8 +79,22 @@ _SIGS = {
     "pwm_batch_start_vector": [VP, I32, C.c_uint64, VP],
     "pwm_batch_vector": [VP, I32, VP, C.POINTER(F64)],
     "pwm_batch_size": [VP, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
+    "pwm_batch_set_option": [VP, C.c_char_p, I64],
+    "pwm_batch_stats": [VP, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)],
+    "pwm_server_create": [C.c_int, I64, I32, C.POINTER(VP)],
+    "pwm_server_destroy": [VP],
+    "pwm_server_last_error": [VP],
+    "pwm_server_set_option": [VP, C.c_char_p, I64],
+    "pwm_server_add_matrix": [VP, I64, VP, VP, VP, I32, C.POINTER(I32)],
+    "pwm_server_add_poisson": [VP, I64, I64, C.POINTER(I32)],
+    "pwm_server_remove_matrix": [VP, I32],
+    "pwm_server_matrix_resident": [VP, I32, C.POINTER(I32)],
+    "pwm_server_submit": [VP, I32, I64, F64, C.c_uint64, C.POINTER(I64)],
+    "pwm_server_step": [VP, C.POINTER(I32), C.POINTER(I32)],
+    "pwm_server_result": [VP, I64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64), VP],
+    "pwm_server_stats": [VP, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I64)],
 }
-_CHAR_RESULT = {"pwm_last_error", "pwm_batch_last_error"}
+_CHAR_RESULT = {"pwm_last_error", "pwm_batch_last_error", "pwm_server_last_error"}
 
 
 def lib():
@@ -367,12 +381,25 @@ def gen_spd_powerlaw(n, seed=4):
 
 
 class Batch:
-    """Many power methods in one persistent launch (pwm_batch_* in pwm_hip.h)."""
+    """Many power methods stepped together (pwm_batch_* in pwm_hip.h).
 
-    def __init__(self, values="f32"):
+    Options: items_per_thread (4, 8, 16), chunk (steps per graph replay),
+    graph (False: plain launches)."""
+
+    def __init__(self, values="f32", **options):
         self.h = VP()
         _check(lib().pwm_batch_create(VALUES[values], C.byref(self.h)))
         self.values = values
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def set_option(self, key, value):
+        self._chk(lib().pwm_batch_set_option(self.h, key.encode(), int(value)))
+
+    def stats(self):
+        a, b, c = I64(), I64(), I64()
+        self._chk(lib().pwm_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"launches": a.value, "graphs": b.value, "list_builds": c.value}
 
     def __del__(self):
         try:
@@ -423,6 +450,103 @@ class Batch:
         yn = F64()
         self._chk(lib().pwm_batch_vector(self.h, int(i), _ptr(out), C.byref(yn)))
         return out, yn.value
+
+
+class Server:
+    """Continuous batching of power-method jobs and SpMV chains under a device
+    memory budget (pwm_server_* in pwm_hip.h).
+
+    Borrowed matrices (borrow=True) keep references to the caller's arrays;
+    the Python object holds them alive until remove_matrix."""
+
+    def __init__(self, values="f32", budget_bytes=64 << 30, max_jobs=1024, **options):
+        self.h = VP()
+        _check(lib().pwm_server_create(VALUES[values], int(budget_bytes), int(max_jobs), C.byref(self.h)))
+        self.values = values
+        self._sizes = {}
+        self._held = {}
+        for k, v in options.items():
+            self.set_option(k, v)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pwm_server_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise PwmError(rc, lib().pwm_server_last_error(self.h).decode())
+
+    def set_option(self, key, value):
+        self._chk(lib().pwm_server_set_option(self.h, key.encode(), int(value)))
+
+    def add_matrix(self, row_start, col_ind, data, borrow=False):
+        rs = np.ascontiguousarray(row_start, np.int32)
+        ci = np.ascontiguousarray(col_ind, np.int32)
+        dv = np.ascontiguousarray(data, np.float64)
+        m = I32()
+        self._chk(lib().pwm_server_add_matrix(self.h, len(rs) - 1, _ptr(rs), _ptr(ci), _ptr(dv), 1 if borrow else 0,
+                                              C.byref(m)))
+        if borrow:
+            self._held[m.value] = (rs, ci, dv)
+        self._sizes[m.value] = len(rs) - 1
+        return m.value
+
+    def add_poisson(self, m, n):
+        k = I32()
+        self._chk(lib().pwm_server_add_poisson(self.h, int(m), int(n), C.byref(k)))
+        self._sizes[k.value] = int(m) * int(n)
+        return k.value
+
+    def remove_matrix(self, mat):
+        self._chk(lib().pwm_server_remove_matrix(self.h, int(mat)))
+        self._held.pop(mat, None)
+
+    def resident(self, mat):
+        r = I32()
+        self._chk(lib().pwm_server_matrix_resident(self.h, int(mat), C.byref(r)))
+        return bool(r.value)
+
+    def submit(self, mat, max_it, tol=-1.0, seed=0):
+        j = I64()
+        self._chk(lib().pwm_server_submit(self.h, int(mat), int(max_it), float(tol), int(seed), C.byref(j)))
+        return j.value
+
+    def step(self):
+        """One admission + chunk of steps; returns (running, finished_now)."""
+        a, f = I32(), I32()
+        self._chk(lib().pwm_server_step(self.h, C.byref(a), C.byref(f)))
+        return a.value, f.value
+
+    def result(self, job, n=None):
+        """(lambda, ynorm, iterations, vector or None); raises while unfinished."""
+        lam, yn, its = F64(), F64(), I64()
+        vec = None if n is None else np.zeros(int(n))
+        self._chk(lib().pwm_server_result(self.h, int(job), C.byref(lam), C.byref(yn), C.byref(its),
+                                          None if vec is None else _ptr(vec)))
+        return lam.value, yn.value, its.value, vec
+
+    def stats(self):
+        u, r, q, a, f = I64(), I32(), I32(), I32(), I64()
+        self._chk(lib().pwm_server_stats(self.h, C.byref(u), C.byref(r), C.byref(q), C.byref(a), C.byref(f)))
+        return {"used_bytes": u.value, "resident": r.value, "queued": q.value, "running": a.value,
+                "finished": f.value}
+
+
+def server_start_vector(n, seed):
+    """The start vector of a server job with `seed` (host restatement of the
+    device hash with stream id 0; FP32 servers round it to float)."""
+    i = np.arange(int(n), dtype=np.uint64)
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) * np.uint64(0x9e3779b97f4a7c15) + i + np.uint64(0x632be59bd9b4e019)) & M
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)) & M
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)) & M
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
 
 
 def gen_uniform(n, seed=0):

@@ -1,0 +1,498 @@
+// Iteration engine: kernels and host-side slot, list and graph management.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "api_internal.hpp"
+#include "engine.hpp"
+#include "merge_tile.hpp"
+
+namespace pwmk {
+
+int32_t engine_tiles_for(int64_t n, int64_t nnz, int ipt) {
+    const int64_t cap = (int64_t)kEngBS * ipt;
+    return (int32_t)((n + nnz + cap - 1) / cap);
+}
+
+// searchPathOnDiag (SOA_Implementations/CRS_Merge.hpp:42-62) at diagonal d.
+__device__ __forceinline__ void eng_search(const int32_t* __restrict__ rs, int64_t n, int64_t nnz, int64_t d,
+                                           int32_t& a, int32_t& b) {
+    int64_t lo = d > nnz ? d - nnz : 0;
+    int64_t hi = d < n ? d : n;
+    while (lo < hi) {
+        const int64_t piv = (lo + hi) >> 1;
+        if ((int64_t)rs[piv + 1] <= d - piv - 1)
+            lo = piv + 1;
+        else
+            hi = piv;
+    }
+    a = (int32_t)(lo < n ? lo : n);
+    b = (int32_t)(d - lo);
+}
+
+// Tile t covers diagonals [t*cap, min((t+1)*cap, n+nnz)) of the matrix's merge.
+__global__ void k_eng_plan(const int32_t* __restrict__ rs, int64_t n, int64_t nnz, int64_t cap, int32_t nt,
+                           ETile* __restrict__ tiles) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int64_t total = n + nnz;
+    const int64_t d0 = t * cap < total ? t * cap : total;
+    const int64_t d1 = d0 + cap < total ? d0 + cap : total;
+    ETile T;
+    eng_search(rs, n, nnz, d0, T.a0, T.b0);
+    eng_search(rs, n, nnz, d1, T.a1, T.b1);
+    tiles[t] = T;
+}
+
+hipError_t engine_plan_tiles(const int32_t* rs, int64_t n, int64_t nnz, int ipt, ETile* tiles, hipStream_t s) {
+    const int32_t nt = engine_tiles_for(n, nnz, ipt);
+    hipLaunchKernelGGL(k_eng_plan, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, rs, n, nnz,
+                       (int64_t)kEngBS * ipt, nt, tiles);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ uint64_t eng_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// x0[i] = U[-1,1) from a hash of (seed, stream_id, i).
+template <typename X>
+__global__ void k_eng_start(X* __restrict__ x, int64_t n, uint64_t seed, uint64_t sid) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t h = eng_mix64(seed * 0x9e3779b97f4a7c15ull + (sid << 32) + (uint64_t)i + 0x632be59bd9b4e019ull);
+        x[i] = (X)((double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+    }
+}
+
+template <typename X>
+__global__ void k_eng_scale(X* __restrict__ v, int64_t n, double s) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = (X)((double)v[i] * s);
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096)); }
+
+hipError_t engine_start_vector(void* x, int64_t n, int vtype, uint64_t seed, uint64_t sid, hipStream_t s) {
+    if (vtype == VT_F64)
+        hipLaunchKernelGGL(k_eng_start<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)x, n, seed, sid);
+    else
+        hipLaunchKernelGGL(k_eng_start<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)x, n, seed, sid);
+    return hipGetLastError();
+}
+
+hipError_t engine_normalize(void* v, int64_t n, int vtype, double inv_norm, hipStream_t s) {
+    if (vtype == VT_F64)
+        hipLaunchKernelGGL(k_eng_scale<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)v, n, inv_norm);
+    else
+        hipLaunchKernelGGL(k_eng_scale<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)v, n, inv_norm);
+    return hipGetLastError();
+}
+
+// One workgroup per (slot, tile) of the work list: the tile's rows of
+// y = (A x) * scale, its carry and its FP64 partials.
+template <typename VT, int ITEMS>
+__global__ __launch_bounds__(kEngBS) void k_eng_tiles(EArgs a) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    __shared__ MergeTileSmem<Acc, kEngBS, ITEMS> sm;
+    __shared__ double red[16];
+    const int nw = a.ctl->n_work;
+    if ((int)blockIdx.x >= nw) return;
+    const int2 w = a.work[xcd_remap(blockIdx.x, nw)];
+    const int s = w.x, t = w.y;
+    if (a.st[s].done) return;
+    const EJob& J = a.jobs[s];
+    const int64_t it = a.st[s].it;
+    const double scale = a.st[s].scale;
+    const X* __restrict__ x = (const X*)((it & 1) ? J.buf1 : J.buf0);
+    X* __restrict__ y = (X*)((it & 1) ? J.buf0 : J.buf1);
+    const ETile T = J.tiles[t];
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    const Acc c = merge_tile<VT, kEngBS, ITEMS, true>(J.rs, J.ci, (const VT*)J.val, x, y, T.a0, T.a1, T.b0, T.b1,
+                                                      scale, sm, yy, xy, xx);
+    const double ra = block_sum<kEngBS>(yy, red);
+    const double rb = block_sum<kEngBS>(xy, red);
+    const double rc = block_sum<kEngBS>(xx, red);
+    if (threadIdx.x == 0) {
+        J.partials[3 * (int64_t)t + 0] = ra;
+        J.partials[3 * (int64_t)t + 1] = rb;
+        J.partials[3 * (int64_t)t + 2] = rc;
+        ((Acc*)J.carry)[t] = c;
+    }
+}
+
+// One workgroup per active slot: carries, norms, lambda and the stop test.
+// A run of tiles t..q-1 carrying into the same row r is summed in tile
+// order and added to y[r] once; the tiles' partials are summed in a fixed
+// thread/tile order, so the result does not depend on scheduling.
+template <typename VT>
+__global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    __shared__ double red[16];
+    if ((int)blockIdx.x >= a.ctl->n_slots) return;
+    const int s = a.slots[blockIdx.x];
+    if (a.st[s].done) return;
+    const EJob& J = a.jobs[s];
+    const int64_t it = a.st[s].it;
+    const double scale = a.st[s].scale;
+    const X* __restrict__ x = (const X*)((it & 1) ? J.buf1 : J.buf0);
+    X* __restrict__ y = (X*)((it & 1) ? J.buf0 : J.buf1);
+    const Acc* __restrict__ cv = (const Acc*)J.carry;
+    const ETile* __restrict__ tiles = J.tiles;
+    const int nt = J.n_tiles;
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int t = threadIdx.x; t < nt; t += kEngBS) {
+        yy += J.partials[3 * (int64_t)t + 0];
+        xy += J.partials[3 * (int64_t)t + 1];
+        xx += J.partials[3 * (int64_t)t + 2];
+        const int32_t r = tiles[t].a1;
+        if (r >= J.n || (t > 0 && tiles[t - 1].a1 == r)) continue;
+        Acc sum = 0;
+        int q = t;
+        for (; q < nt && tiles[q].a1 == r; ++q) sum += cv[q];
+        // tile q completes row r; when row r starts at or after that tile's
+        // first nonzero the carries are empty and tile q counted the row
+        if (q >= nt || J.rs[r] >= tiles[q].b0) continue;
+        const X out = (X)((double)y[r] + (double)sum * scale);
+        y[r] = out;
+        const double yd = (double)out, xd = (double)x[r] * scale;
+        yy += yd * yd;
+        xy += xd * yd;
+        xx += xd * xd;
+    }
+    const double YY = block_sum<kEngBS>(yy, red);
+    const double XY = block_sum<kEngBS>(xy, red);
+    const double XX = block_sum<kEngBS>(xx, red);
+    if (threadIdx.x == 0) {
+        EState& S = a.st[s];
+        const double lam = XY / XX;
+        const int64_t it1 = it + 1;
+        int done = it1 >= J.max_it;
+        if (J.tol >= 0.0 && it1 >= 2 && fabs(lam - S.lambda) <= J.tol * fabs(lam)) done = 1;
+        S.it = it1;
+        S.lambda = lam;
+        S.yy = YY;
+        S.scale = 1.0 / sqrt(YY);
+        S.done = done;
+    }
+}
+
+template <typename VT, int ITEMS>
+static hipError_t step_t(const EArgs& a, int gw, int gs, hipStream_t s) {
+    hipLaunchKernelGGL((k_eng_tiles<VT, ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a);
+    hipLaunchKernelGGL((k_eng_finish<VT>), dim3(gs), dim3(kEngBS), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename VT>
+static hipError_t step_v(const EArgs& a, int ipt, int gw, int gs, hipStream_t s) {
+    switch (ipt) {
+        case 4: return step_t<VT, 4>(a, gw, gs, s);
+        case 8: return step_t<VT, 8>(a, gw, gs, s);
+        case 16: return step_t<VT, 16>(a, gw, gs, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t engine_step(const EArgs& a, int vtype, int ipt, int gw, int gs, hipStream_t s) {
+    switch (vtype) {
+        case VT_F64: return step_v<double>(a, ipt, gw, gs, s);
+        case VT_F32: return step_v<float>(a, ipt, gw, gs, s);
+        case VT_BF16: return step_v<bf16_t>(a, ipt, gw, gs, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace pwmk
+
+namespace pwm_impl {
+
+// ---------------------------------------------------------------- arena
+
+Arena::~Arena() {
+    if (base_) (void)hipFree(base_);
+}
+
+hipError_t Arena::reserve(size_t bytes) {
+    bytes = round(std::max<size_t>(bytes, 256));
+    if (base_ && cap_ >= bytes) {
+        reset();
+        return hipSuccess;
+    }
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = 0;
+    const hipError_t e = hipMalloc(&base_, bytes);
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        free_.clear();
+        used_ = 0;
+        return e;
+    }
+    cap_ = bytes;
+    reset();
+    return hipSuccess;
+}
+
+void Arena::reset() {
+    free_.clear();
+    if (cap_) free_[0] = cap_;
+    used_ = 0;
+}
+
+bool Arena::alloc(size_t bytes, size_t* off) {
+    bytes = round(std::max<size_t>(bytes, 1));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        *off = it->first;
+        const size_t rest = it->second - bytes, at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        used_ += bytes;
+        return true;
+    }
+    return false;
+}
+
+void Arena::release(size_t off, size_t bytes) {
+    bytes = round(std::max<size_t>(bytes, 1));
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- engine
+
+Engine::~Engine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (graph_) (void)hipGraphExecDestroy(graph_);
+    for (void* p : {d_jobs_, d_state_, d_ctl_, d_work_, d_slots_})
+        if (p) (void)hipFree(p);
+    for (void* p : {(void*)h_jobs_, (void*)h_state_, (void*)h_ctl_, (void*)h_work_, (void*)h_slots_})
+        if (p) (void)hipHostFree(p);
+}
+
+int Engine::init(int vtype, int max_slots, hipStream_t stream, std::string& err) {
+    vtype_ = vtype;
+    max_slots_ = std::max(max_slots, 1);
+    stream_ = stream;
+    hipError_t e;
+    const size_t ns = max_slots_;
+    if ((e = hipHostMalloc((void**)&h_jobs_, ns * sizeof(pwmk::EJob), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&h_state_, ns * sizeof(pwmk::EState), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&h_ctl_, sizeof(pwmk::ECtl), hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc((void**)&h_slots_, ns * 4, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc(&d_jobs_, ns * sizeof(pwmk::EJob))) != hipSuccess ||
+        (e = hipMalloc(&d_state_, ns * sizeof(pwmk::EState))) != hipSuccess ||
+        (e = hipMalloc(&d_ctl_, sizeof(pwmk::ECtl))) != hipSuccess || (e = hipMalloc(&d_slots_, ns * 4)) != hipSuccess) {
+        err = std::string("engine allocation: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
+    }
+    memset(h_jobs_, 0, ns * sizeof(pwmk::EJob));
+    memset(h_state_, 0, ns * sizeof(pwmk::EState));
+    memset(h_ctl_, 0, sizeof(pwmk::ECtl));
+    in_use_.assign(ns, 0);
+    running_.assign(ns, 0);
+    fresh_.assign(ns, 0);
+    return PWM_OK;
+}
+
+int Engine::set_ipt(int v) {
+    if (v != 4 && v != 8 && v != 16) return PWM_ERR_INVALID;
+    for (int s = 0; s < max_slots_; ++s)
+        if (in_use_[s]) return PWM_ERR_STATE;
+    ipt_ = v;
+    return PWM_OK;
+}
+
+int Engine::add(const pwmk::EJob& j) {
+    for (int s = 0; s < max_slots_; ++s) {
+        if (in_use_[s]) continue;
+        h_jobs_[s] = j;
+        pwmk::EState st;
+        memset(&st, 0, sizeof(st));
+        st.scale = 1.0;
+        h_state_[s] = st;
+        in_use_[s] = 1;
+        running_[s] = 1;
+        fresh_[s] = 1;
+        ++n_active_;
+        lists_dirty_ = true;
+        return s;
+    }
+    return -1;
+}
+
+void Engine::release(int slot) {
+    if (slot < 0 || slot >= max_slots_ || !in_use_[slot]) return;
+    if (running_[slot]) {
+        running_[slot] = 0;
+        --n_active_;
+        lists_dirty_ = true;
+    }
+    in_use_[slot] = 0;
+}
+
+void* Engine::last_buffer(int slot) const {
+    const pwmk::EJob& j = h_jobs_[slot];
+    return (h_state_[slot].it & 1) ? j.buf1 : j.buf0;
+}
+
+int Engine::build_lists(std::string& err) {
+    int64_t nw = 0;
+    int ns = 0;
+    for (int s = 0; s < max_slots_; ++s)
+        if (running_[s]) {
+            nw += h_jobs_[s].n_tiles;
+            ++ns;
+        }
+    if (nw > INT32_MAX / 2) {
+        err = "engine: too many tiles in flight";
+        return PWM_ERR_INVALID;
+    }
+    if ((size_t)nw > work_cap_alloc_) {
+        // grow the work list buffers (device and pinned mirror)
+        const size_t cap = std::max<size_t>((size_t)(nw + nw / 4), 4096);
+        if (d_work_) (void)hipFree(d_work_);
+        if (h_work_) (void)hipHostFree(h_work_);
+        d_work_ = nullptr;
+        h_work_ = nullptr;
+        work_cap_alloc_ = 0;
+        hipError_t e;
+        if ((e = hipMalloc(&d_work_, cap * sizeof(int2))) != hipSuccess ||
+            (e = hipHostMalloc((void**)&h_work_, cap * sizeof(int2), hipHostMallocDefault)) != hipSuccess) {
+            err = std::string("engine work list: ") + hipGetErrorString(e);
+            return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
+        }
+        work_cap_alloc_ = cap;
+        if (graph_) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+    }
+    int64_t k = 0;
+    int j = 0;
+    for (int s = 0; s < max_slots_; ++s)
+        if (running_[s]) {
+            for (int32_t t = 0; t < h_jobs_[s].n_tiles; ++t) h_work_[k++] = make_int2(s, t);
+            h_slots_[j++] = s;
+        }
+    n_work_ = (int)nw;
+    n_slots_ = ns;
+    n_stale_work_ = 0;
+    h_ctl_->n_work = n_work_;
+    h_ctl_->n_slots = n_slots_;
+    // grids: resized (and the graph rebuilt) only when a list outgrows its
+    // grid or falls below a quarter of it; spare blocks return at once
+    const int gw = std::max(1, (int)std::min<int64_t>((int64_t)work_cap_alloc_, ((int64_t)n_work_ * 5 / 4 + 1023) / 1024 * 1024));
+    if (n_work_ > grid_work_ || n_work_ < grid_work_ / 4) grid_work_ = std::max(gw, n_work_);
+    const int gs = std::min(max_slots_, std::max(64, (n_slots_ * 5 / 4 + 63) / 64 * 64));
+    if (n_slots_ > grid_slots_ || n_slots_ < grid_slots_ / 4) grid_slots_ = std::max(gs, n_slots_);
+    hipError_t e = hipSuccess;
+    if (n_work_ > 0) e = hipMemcpyAsync(d_work_, h_work_, (size_t)n_work_ * sizeof(int2), hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess && n_slots_ > 0)
+        e = hipMemcpyAsync(d_slots_, h_slots_, (size_t)n_slots_ * 4, hipMemcpyHostToDevice, stream_);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_ctl_, h_ctl_, sizeof(pwmk::ECtl), hipMemcpyHostToDevice, stream_);
+    if (e != hipSuccess) {
+        err = std::string("engine list upload: ") + hipGetErrorString(e);
+        return PWM_ERR_HIP;
+    }
+    ++list_builds;
+    lists_dirty_ = false;
+    return PWM_OK;
+}
+
+int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
+    finished.clear();
+    if (n_active_ == 0) return PWM_OK;
+    if (steps < 1) steps = 1;
+    hipError_t e = hipSuccess;
+    // new jobs: descriptor and step-0 state
+    for (int s = 0; s < max_slots_ && e == hipSuccess; ++s)
+        if (fresh_[s]) {
+            e = hipMemcpyAsync((pwmk::EJob*)d_jobs_ + s, h_jobs_ + s, sizeof(pwmk::EJob), hipMemcpyHostToDevice, stream_);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync((pwmk::EState*)d_state_ + s, h_state_ + s, sizeof(pwmk::EState), hipMemcpyHostToDevice,
+                                   stream_);
+            fresh_[s] = 0;
+        }
+    if (e != hipSuccess) {
+        err = std::string("engine job upload: ") + hipGetErrorString(e);
+        return PWM_ERR_HIP;
+    }
+    if (lists_dirty_) {
+        const int rc = build_lists(err);
+        if (rc != PWM_OK) return rc;
+    }
+    pwmk::EArgs a;
+    a.jobs = (const pwmk::EJob*)d_jobs_;
+    a.st = (pwmk::EState*)d_state_;
+    a.work = (const int2*)d_work_;
+    a.slots = (const int32_t*)d_slots_;
+    a.ctl = (const pwmk::ECtl*)d_ctl_;
+    if (use_graph) {
+        if (graph_ && (graph_steps_ != steps || graph_gw_ != grid_work_ || graph_gs_ != grid_slots_)) {
+            (void)hipGraphExecDestroy(graph_);
+            graph_ = nullptr;
+        }
+        if (!graph_) {
+            hipGraph_t g = nullptr;
+            e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+            hipError_t le = hipSuccess;
+            for (int i = 0; i < steps && e == hipSuccess && le == hipSuccess; ++i)
+                le = pwmk::engine_step(a, vtype_, ipt_, grid_work_, grid_slots_, stream_);
+            if (e == hipSuccess) e = hipStreamEndCapture(stream_, &g);
+            if (e == hipSuccess) e = le;
+            if (e == hipSuccess) e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                graph_ = nullptr;
+                err = std::string("engine graph capture: ") + hipGetErrorString(e);
+                return PWM_ERR_HIP;
+            }
+            graph_steps_ = steps;
+            graph_gw_ = grid_work_;
+            graph_gs_ = grid_slots_;
+            ++graph_builds;
+        }
+        e = hipGraphLaunch(graph_, stream_);
+    } else {
+        for (int i = 0; i < steps && e == hipSuccess; ++i)
+            e = pwmk::engine_step(a, vtype_, ipt_, grid_work_, grid_slots_, stream_);
+    }
+    launches += 2 * steps;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_state_, d_state_, (size_t)max_slots_ * sizeof(pwmk::EState), hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) {
+        err = std::string("engine step: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
+    }
+    for (int s = 0; s < max_slots_; ++s)
+        if (running_[s] && h_state_[s].done) {
+            finished.push_back(s);
+            running_[s] = 0;
+            --n_active_;
+            n_stale_work_ += h_jobs_[s].n_tiles;
+        }
+    // drop finished jobs' tiles once they make up a tenth of the list
+    if ((int64_t)n_stale_work_ * 10 > (int64_t)n_work_) lists_dirty_ = true;
+    return PWM_OK;
+}
+
+}  // namespace pwm_impl

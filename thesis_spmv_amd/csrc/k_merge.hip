@@ -21,6 +21,7 @@
 // :136-140): each run of equal carry rows is summed in tile order and added
 // once, so the result does not depend on scheduling.
 #include "kernels.hpp"
+#include "merge_tile.hpp"
 
 namespace pwmk {
 
@@ -52,12 +53,7 @@ __global__ __launch_bounds__(kMBS) void k_merge_spmv(CrsDev A, MergePlan mp, con
                                                      typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
-    constexpr int CAP = kMBS * ITEMS;
-    __shared__ int32_t s_rowend[CAP];
-    __shared__ Acc s_prod[CAP];
-    __shared__ Acc s_out[CAP];
-    __shared__ int32_t s_key[kMBS];
-    __shared__ Acc s_val[kMBS];
+    __shared__ MergeTileSmem<Acc, kMBS, ITEMS> sm;
     __shared__ double red[16];
 
     const X* __restrict__ x = x_arg;
@@ -70,113 +66,20 @@ __global__ __launch_bounds__(kMBS) void k_merge_spmv(CrsDev A, MergePlan mp, con
         y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
         scale = step_scale(pw, k);
     }
-    const int32_t* __restrict__ rs = A.rs;
-    const int32_t* __restrict__ ci = A.ci;
-    const VT* __restrict__ val = (const VT*)A.val;
-    const int tid = threadIdx.x;
-
     const int64_t p = xcd_remap(blockIdx.x, mp.P);
-    const int64_t a0 = mp.split_a[p], a1 = mp.split_a[p + 1];
-    const int64_t b0 = mp.split_b[p], b1 = mp.split_b[p + 1];
-    const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
-
-    for (int i = tid; i < na; i += kMBS) s_rowend[i] = rs[a0 + 1 + i];
-    {
-        int i = tid;
-        for (; i + 3 * kMBS < nb; i += 4 * kMBS) {
-            const int64_t k = b0 + i;
-            const int32_t j0 = ci[k], j1 = ci[k + kMBS], j2 = ci[k + 2 * kMBS], j3 = ci[k + 3 * kMBS];
-            const Acc v0 = widen(val[k]), v1 = widen(val[k + kMBS]), v2 = widen(val[k + 2 * kMBS]),
-                      v3 = widen(val[k + 3 * kMBS]);
-            s_prod[i] = v0 * (Acc)x[j0];
-            s_prod[i + kMBS] = v1 * (Acc)x[j1];
-            s_prod[i + 2 * kMBS] = v2 * (Acc)x[j2];
-            s_prod[i + 3 * kMBS] = v3 * (Acc)x[j3];
-        }
-        for (; i < nb; i += kMBS) s_prod[i] = widen(val[b0 + i]) * (Acc)x[ci[b0 + i]];
-    }
-    __syncthreads();
-
-    // this thread's sub-range of the tile's merge items
-    const int items = na + nb;
-    const int d0 = tid * ITEMS < items ? tid * ITEMS : items;
-    const int d1 = d0 + ITEMS < items ? d0 + ITEMS : items;
-    int lo = d0 > nb ? d0 - nb : 0;
-    int hi = d0 < na ? d0 : na;
-    while (lo < hi) {
-        const int piv = (lo + hi) >> 1;
-        if ((int64_t)s_rowend[piv] <= b0 + d0 - piv - 1)
-            lo = piv + 1;
-        else
-            hi = piv;
-    }
-    int ai = lo, bi = d0 - lo;
-    const int a_start = ai;
-    Acc run = 0;
-    int first = -1;
-    Acc first_val = 0;
-    for (int d = d0; d < d1; ++d) {
-        if (ai < na && (int64_t)s_rowend[ai] <= b0 + bi) {
-            if (first < 0) {
-                first = ai;
-                first_val = run;
-            } else {
-                s_out[ai] = run;
-            }
-            run = 0;
-            ++ai;
-        } else {
-            run += s_prod[bi];
-            ++bi;
-        }
-    }
-
-    // segmented inclusive scan of (row, partial) over threads; keys ascend
-    s_key[tid] = ai;
-    s_val[tid] = run;
-    __syncthreads();
-#pragma unroll
-    for (int off = 1; off < kMBS; off <<= 1) {
-        Acc add = 0;
-        const bool take = tid >= off && s_key[tid - off] == s_key[tid];
-        if (take) add = s_val[tid - off];
-        __syncthreads();
-        if (take) s_val[tid] = add + s_val[tid];
-        __syncthreads();
-    }
-    if (first >= 0) {
-        const Acc carry_in = (tid > 0 && s_key[tid - 1] == a_start) ? s_val[tid - 1] : (Acc)0;
-        s_out[first] = carry_in + first_val;
-    }
-    __syncthreads();
-
-    // row a0 is unfinished here when it began in an earlier tile
-    const bool head_open = na > 0 && (int64_t)rs[a0] < b0;
+    const int64_t a1 = mp.split_a[p + 1];
     double yy = 0.0, xy = 0.0, xx = 0.0;
-    for (int i = tid; i < na; i += kMBS) {
-        const Acc v = s_out[i];
-        if constexpr (POWER) {
-            const X out = (X)((double)v * scale);
-            y[a0 + i] = out;
-            if (!(i == 0 && head_open)) {
-                const double yd = (double)out, xd = (double)x[a0 + i] * scale;
-                yy += yd * yd;
-                xy += xd * yd;
-                xx += xd * xd;
-            }
-        } else {
-            y[a0 + i] = (X)v;
-        }
-    }
-    if (tid == kMBS - 1) {
+    const Acc carry = merge_tile<VT, kMBS, ITEMS, POWER>(A.rs, A.ci, (const VT*)A.val, x, y, mp.split_a[p], a1,
+                                                         mp.split_b[p], mp.split_b[p + 1], scale, sm, yy, xy, xx);
+    if (threadIdx.x == kMBS - 1) {
         mp.carry_row[p] = (int32_t)a1;
-        ((Acc*)mp.carry_val)[p] = s_val[kMBS - 1];
+        ((Acc*)mp.carry_val)[p] = carry;
     }
     if constexpr (POWER) {
         const double ra = block_sum<kMBS>(yy, red);
         const double rb = block_sum<kMBS>(xy, red);
         const double rc = block_sum<kMBS>(xx, red);
-        if (tid == 0) {
+        if (threadIdx.x == 0) {
             pw.partials[3 * p + 0] = ra;
             pw.partials[3 * p + 1] = rb;
             pw.partials[3 * p + 2] = rc;

@@ -1,0 +1,485 @@
+// Request server: SpMV-chain and power-method jobs on resident matrices,
+// batched continuously on the iteration engine (engine.hpp).
+//
+// pwm_server_step() admits queued jobs in arrival order while an engine slot
+// and device memory are free (memory comes from one arena of `budget`
+// bytes; matrices no running job uses are evicted least recently used first
+// to make room), runs one chunk of steps of every running job, and retires
+// the jobs that stopped.  Jobs that arrive while others run join at the next
+// chunk; a job's arithmetic depends only on its own matrix tiles, so its
+// result is bit-identical whatever else is in flight.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "engine.hpp"
+
+namespace {
+
+struct SrvMatrix {
+    int64_t n = 0, nnz = 0;
+    int kind = 0;  // 0: CRS from the host, 1: Poisson generated on the device
+    int64_t pm = 0, pn = 0;
+    bool borrowed = false;
+    const int32_t* rs_p = nullptr;  // host arrays (owned copies or the caller's)
+    const int32_t* ci_p = nullptr;
+    const double* val_p = nullptr;
+    std::vector<int32_t> rs, ci;
+    std::vector<double> val;
+    bool alive = true;
+    bool resident = false;
+    size_t off = 0, bytes = 0;
+    size_t o_ci = 0, o_val = 0, o_tiles = 0;  // inside the block
+    int32_t n_tiles = 0;
+    int refs = 0;
+    uint64_t last_use = 0;
+};
+
+struct SrvJob {
+    int32_t mat = 0;
+    int64_t max_it = 0;
+    double tol = -1.0;
+    uint64_t seed = 0;
+    int state = 0;  // 0 queued, 1 running, 2 finished
+    int slot = -1;
+    size_t off = 0, bytes = 0;
+    bool has_block = false;
+    double lambda = 0.0, ynorm = 0.0;
+    int64_t iters = 0;
+    void* last = nullptr;
+};
+
+}  // namespace
+
+struct pwm_server {
+    int vtype = PWM_VALUES_F32;
+    int ipt = 8;
+    int chunk = 8;
+    int use_graph = 1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    pwm_impl::Arena arena;
+    pwm_impl::Engine eng;
+    std::vector<SrvMatrix> mats;
+    std::map<int64_t, SrvJob> jobs;
+    std::deque<int64_t> queue;
+    std::vector<int64_t> slot_job;
+    int64_t next_job = 0, finished_total = 0;
+    uint64_t clock = 0;
+};
+
+static int sfail(pwm_server* s, int code, const std::string& m) {
+    if (s)
+        s->err = m;
+    else
+        pwm_impl::set_global_error(m);
+    return code;
+}
+static int shp(pwm_server* s, hipError_t e, const char* what) {
+    return sfail(s, e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static size_t acc_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : 4; }
+
+static void matrix_layout(const pwm_server* s, SrvMatrix& M) {
+    using pwm_impl::Arena;
+    const size_t vs = pwm_impl::value_size(s->vtype);
+    M.n_tiles = pwmk::engine_tiles_for(M.n, M.nnz, s->ipt);
+    M.o_ci = Arena::round((M.n + 1) * 4);
+    M.o_val = M.o_ci + Arena::round(M.nnz * 4);
+    M.o_tiles = M.o_val + Arena::round(M.nnz * vs);
+    M.bytes = M.o_tiles + Arena::round((size_t)M.n_tiles * sizeof(pwmk::ETile));
+}
+
+static size_t job_bytes(const pwm_server* s, const SrvMatrix& M) {
+    using pwm_impl::Arena;
+    return Arena::round(2 * M.n * pwm_impl::vector_size(s->vtype)) + Arena::round((size_t)M.n_tiles * 24) +
+           Arena::round((size_t)M.n_tiles * acc_size(s->vtype));
+}
+
+extern "C" int pwm_server_create(int value_type, int64_t budget_bytes, int32_t max_jobs, pwm_server** out) {
+    if (!out) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_create: out is NULL");
+    *out = nullptr;
+    if (value_type < PWM_VALUES_F64 || value_type > PWM_VALUES_BF16)
+        return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_create: unknown value type");
+    if (budget_bytes < (1 << 20) || max_jobs < 1 || max_jobs > 65536)
+        return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_create: budget must be >= 1 MiB and max_jobs in [1, 65536]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return sfail(nullptr, PWM_ERR_HIP, "pwm_server_create: no HIP device available");
+    pwm_server* s = new pwm_server();
+    s->vtype = value_type;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = s->arena.reserve((size_t)budget_bytes);
+    if (e != hipSuccess) {
+        const int rc = e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        return sfail(nullptr, rc, std::string("pwm_server_create: ") + hipGetErrorString(e));
+    }
+    std::string err;
+    const int rc = s->eng.init(value_type, max_jobs, s->stream, err);
+    if (rc != PWM_OK) {
+        (void)hipStreamDestroy(s->stream);
+        delete s;
+        return sfail(nullptr, rc, err);
+    }
+    s->eng.chunk = s->chunk;
+    s->slot_job.assign(max_jobs, -1);
+    *out = s;
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_destroy(pwm_server* s) {
+    if (!s) return PWM_OK;
+    hipStream_t st = s->stream;
+    if (st) (void)hipStreamSynchronize(st);
+    delete s;  // engine and arena free their device memory here
+    if (st) (void)hipStreamDestroy(st);
+    return PWM_OK;
+}
+
+extern "C" const char* pwm_server_last_error(const pwm_server* s) { return s ? s->err.c_str() : pwm_last_error(nullptr); }
+
+extern "C" int pwm_server_set_option(pwm_server* s, const char* key, int64_t value) {
+    if (!s || !key) return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: NULL");
+    const std::string k(key);
+    if (k == "chunk") {
+        if (value < 1 || value > 4096) return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: chunk must be in [1, 4096]");
+        s->chunk = (int)value;
+        s->eng.chunk = s->chunk;
+        return PWM_OK;
+    }
+    if (k == "graph") {
+        s->use_graph = value != 0;
+        s->eng.use_graph = s->use_graph != 0;
+        return PWM_OK;
+    }
+    if (k == "items_per_thread") {
+        if (value != 4 && value != 8 && value != 16)
+            return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: items_per_thread must be 4, 8 or 16");
+        for (const SrvMatrix& M : s->mats)
+            if (M.alive && M.resident) return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: matrices are resident");
+        if (s->eng.set_ipt((int)value) != PWM_OK) return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: jobs in flight");
+        s->ipt = (int)value;
+        for (SrvMatrix& M : s->mats)
+            if (M.alive) matrix_layout(s, M);
+        return PWM_OK;
+    }
+    return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: unknown option " + k);
+}
+
+static int add_matrix_common(pwm_server* s, SrvMatrix&& M, int32_t* mat) {
+    matrix_layout(s, M);
+    if (M.bytes + job_bytes(s, M) > s->arena.capacity())
+        return sfail(s, PWM_ERR_BUDGET, "pwm_server_add_matrix: matrix and one job exceed the device budget");
+    if (mat) *mat = (int32_t)s->mats.size();
+    s->mats.push_back(std::move(M));
+    return PWM_OK;
+}
+
+// flags bit 0 (PWM_SERVER_BORROW): keep the caller's arrays instead of a copy.
+extern "C" int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* rs, const int32_t* ci, const double* vals,
+                                     int32_t flags, int32_t* mat) {
+    if (!s || !rs || n <= 0 || n > INT32_MAX - 1) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: bad argument");
+    const int64_t nnz = rs[n];
+    if (rs[0] != 0 || nnz < 0 || (nnz > 0 && (!ci || !vals)) || n + nnz > INT32_MAX)
+        return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: bad row_start or NULL arrays");
+    for (int64_t r = 0; r < n; ++r)
+        if (rs[r + 1] < rs[r]) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: row_start must be non-decreasing");
+    for (int64_t k = 0; k < nnz; ++k) {
+        if (ci[k] < 0 || ci[k] >= n) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: column index out of range");
+        if (s->vtype == PWM_VALUES_BF16) {
+            const float f = (float)vals[k];
+            uint32_t bits;
+            memcpy(&bits, &f, 4);
+            if ((double)f != vals[k] || (bits & 0xffffu))
+                return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: BF16 storage needs values exact in BF16");
+        }
+    }
+    SrvMatrix M;
+    M.n = n;
+    M.nnz = nnz;
+    if (flags & 1) {
+        M.borrowed = true;
+        M.rs_p = rs;
+        M.ci_p = ci;
+        M.val_p = vals;
+    } else {
+        M.rs.assign(rs, rs + n + 1);
+        M.ci.assign(ci, ci + nnz);
+        M.val.assign(vals, vals + nnz);
+        M.rs_p = M.rs.data();
+        M.ci_p = M.ci.data();
+        M.val_p = M.val.data();
+    }
+    return add_matrix_common(s, std::move(M), mat);
+}
+
+// 2D Poisson matrix generated on the device when it becomes resident
+// (Util/Poisson.hpp:32-83).
+extern "C" int pwm_server_add_poisson(pwm_server* s, int64_t m, int64_t n, int32_t* mat) {
+    if (!s || m <= 0 || n <= 0 || 6 * m * n > INT32_MAX) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_poisson: bad size");
+    SrvMatrix M;
+    M.kind = 1;
+    M.pm = m;
+    M.pn = n;
+    M.n = m * n;
+    M.nnz = n * (m + 2 * (m - 1)) + 2 * (n - 1) * m;
+    return add_matrix_common(s, std::move(M), mat);
+}
+
+static void evict(pwm_server* s, SrvMatrix& M) {
+    if (!M.resident) return;
+    s->arena.release(M.off, M.bytes);
+    M.resident = false;
+}
+
+// Remove a matrix: its device copy is freed and its id is never reused.
+extern "C" int pwm_server_remove_matrix(pwm_server* s, int32_t mat) {
+    if (!s || mat < 0 || mat >= (int32_t)s->mats.size() || !s->mats[mat].alive)
+        return sfail(s, PWM_ERR_INVALID, "pwm_server_remove_matrix: unknown matrix");
+    SrvMatrix& M = s->mats[mat];
+    for (const auto& kv : s->jobs)
+        if (kv.second.mat == mat && kv.second.state != 2)
+            return sfail(s, PWM_ERR_STATE, "pwm_server_remove_matrix: jobs on this matrix are queued or running");
+    evict(s, M);
+    M.alive = false;
+    M.rs = {};
+    M.ci = {};
+    M.val = {};
+    M.rs_p = nullptr;
+    M.ci_p = nullptr;
+    M.val_p = nullptr;
+    return PWM_OK;
+}
+
+// Allocate from the arena, evicting unreferenced resident matrices (least
+// recently used first, never `keep`) until the request fits.
+static bool alloc_evicting(pwm_server* s, size_t bytes, size_t* off, int keep) {
+    for (;;) {
+        if (s->arena.alloc(bytes, off)) return true;
+        int victim = -1;
+        for (int i = 0; i < (int)s->mats.size(); ++i) {
+            const SrvMatrix& M = s->mats[i];
+            if (i == keep || !M.resident || M.refs > 0) continue;
+            if (victim < 0 || M.last_use < s->mats[victim].last_use) victim = i;
+        }
+        if (victim < 0) return false;
+        evict(s, s->mats[victim]);
+    }
+}
+
+static int make_resident(pwm_server* s, int mi) {
+    SrvMatrix& M = s->mats[mi];
+    char* base = (char*)s->arena.at(M.off);
+    int32_t* rs = (int32_t*)base;
+    hipError_t e = hipSuccess;
+    if (M.kind == 1) {
+        e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, s->vtype, s->stream);
+    } else {
+        e = hipMemcpyAsync(rs, M.rs_p, (M.n + 1) * 4, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess && M.nnz) e = hipMemcpyAsync(base + M.o_ci, M.ci_p, M.nnz * 4, hipMemcpyHostToDevice, s->stream);
+        if (e == hipSuccess && M.nnz) {
+            if (s->vtype == PWM_VALUES_F64) {
+                e = hipMemcpyAsync(base + M.o_val, M.val_p, M.nnz * 8, hipMemcpyHostToDevice, s->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            } else {
+                // narrow on the host (FP32, or BF16 = the upper half of the
+                // exact FP32 value), then one copy
+                std::vector<float> f32;
+                std::vector<uint16_t> b16;
+                const void* src;
+                size_t nb;
+                if (s->vtype == PWM_VALUES_F32) {
+                    f32.resize(M.nnz);
+                    for (int64_t k = 0; k < M.nnz; ++k) f32[k] = (float)M.val_p[k];
+                    src = f32.data();
+                    nb = M.nnz * 4;
+                } else {
+                    b16.resize(M.nnz);
+                    for (int64_t k = 0; k < M.nnz; ++k) {
+                        const float f = (float)M.val_p[k];
+                        uint32_t u;
+                        memcpy(&u, &f, 4);
+                        b16[k] = (uint16_t)(u >> 16);
+                    }
+                    src = b16.data();
+                    nb = M.nnz * 2;
+                }
+                e = hipMemcpyAsync(base + M.o_val, src, nb, hipMemcpyHostToDevice, s->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);  // the staging dies here
+            }
+        }
+    }
+    if (e == hipSuccess)
+        e = pwmk::engine_plan_tiles(rs, M.n, M.nnz, s->ipt, (pwmk::ETile*)(base + M.o_tiles), s->stream);
+    if (e != hipSuccess) return shp(s, e, "matrix upload");
+    M.resident = true;
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, double tol, uint64_t seed, int64_t* job) {
+    if (!s || mat < 0 || mat >= (int32_t)s->mats.size() || !s->mats[mat].alive)
+        return sfail(s, PWM_ERR_INVALID, "pwm_server_submit: unknown matrix");
+    if (max_it < 1) return sfail(s, PWM_ERR_INVALID, "pwm_server_submit: max_it must be >= 1");
+    SrvJob J;
+    J.mat = mat;
+    J.max_it = max_it;
+    J.tol = tol;
+    J.seed = seed;
+    const int64_t id = s->next_job++;
+    s->jobs[id] = J;
+    s->queue.push_back(id);
+    if (job) *job = id;
+    return PWM_OK;
+}
+
+// Admit queued jobs in arrival order; stop at the first that does not fit.
+static int admit(pwm_server* s) {
+    const size_t xs = pwm_impl::vector_size(s->vtype);
+    while (!s->queue.empty()) {
+        if (s->eng.active() >= s->eng.max_slots()) break;
+        const int64_t id = s->queue.front();
+        SrvJob& J = s->jobs[id];
+        SrvMatrix& M = s->mats[J.mat];
+        if (!M.resident) {
+            if (!alloc_evicting(s, M.bytes, &M.off, J.mat)) break;
+            const int rc = make_resident(s, J.mat);
+            if (rc != PWM_OK) {
+                s->arena.release(M.off, M.bytes);
+                return rc;
+            }
+        }
+        const size_t jb = job_bytes(s, M);
+        if (!alloc_evicting(s, jb, &J.off, J.mat)) break;
+        J.bytes = jb;
+        J.has_block = true;
+        char* base = (char*)s->arena.at(J.off);
+        const char* mb = (const char*)s->arena.at(M.off);
+        const size_t o_part = pwm_impl::Arena::round(2 * M.n * xs);
+        const size_t o_carry = o_part + pwm_impl::Arena::round((size_t)M.n_tiles * 24);
+        hipError_t e = pwmk::engine_start_vector(base, M.n, s->vtype, J.seed, 0, s->stream);
+        if (e != hipSuccess) return shp(s, e, "job start vector");
+        pwmk::EJob j;
+        memset(&j, 0, sizeof(j));
+        j.rs = (const int32_t*)mb;
+        j.ci = (const int32_t*)(mb + M.o_ci);
+        j.val = mb + M.o_val;
+        j.buf0 = base;
+        j.buf1 = base + M.n * xs;
+        j.tiles = (const pwmk::ETile*)(mb + M.o_tiles);
+        j.partials = (double*)(base + o_part);
+        j.carry = base + o_carry;
+        j.n = M.n;
+        j.max_it = J.max_it;
+        j.tol = J.tol;
+        j.n_tiles = M.n_tiles;
+        const int slot = s->eng.add(j);
+        if (slot < 0) {  // cannot happen: a slot was free above
+            s->arena.release(J.off, J.bytes);
+            J.has_block = false;
+            break;
+        }
+        J.slot = slot;
+        J.state = 1;
+        s->slot_job[slot] = id;
+        M.refs++;
+        M.last_use = ++s->clock;
+        s->queue.pop_front();
+    }
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
+    int rc = admit(s);
+    if (rc != PWM_OK) return rc;
+    if (s->eng.active() == 0 && !s->queue.empty())
+        return sfail(s, PWM_ERR_BUDGET,
+                     "pwm_server_step: the next job does not fit the budget with nothing running "
+                     "(collect finished results to free their memory)");
+    std::vector<int> fin;
+    if (s->eng.active() > 0) {
+        std::string err;
+        if ((rc = s->eng.run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
+    }
+    hipError_t e = hipSuccess;
+    for (int slot : fin) {
+        const int64_t id = s->slot_job[slot];
+        SrvJob& J = s->jobs[id];
+        const pwmk::EState& S = s->eng.state(slot);
+        J.lambda = S.lambda;
+        J.ynorm = sqrt(S.yy);
+        J.iters = S.it;
+        J.last = s->eng.last_buffer(slot);
+        J.state = 2;
+        if (e == hipSuccess) e = pwmk::engine_normalize(J.last, s->mats[J.mat].n, s->vtype, 1.0 / J.ynorm, s->stream);
+        s->eng.release(slot);
+        s->slot_job[slot] = -1;
+        s->mats[J.mat].refs--;
+        ++s->finished_total;
+    }
+    if (e == hipSuccess && !fin.empty()) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return shp(s, e, "job retirement");
+    if (active) *active = s->eng.active();
+    if (finished) *finished = (int32_t)fin.size();
+    return PWM_OK;
+}
+
+// Results of a finished job; `vec` (n doubles, may be NULL) receives the
+// normalised final iterate.  The job and its device memory are released.
+extern "C" int pwm_server_result(pwm_server* s, int64_t job, double* lambda, double* ynorm, int64_t* iters, double* vec) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_result: NULL");
+    auto it = s->jobs.find(job);
+    if (it == s->jobs.end()) return sfail(s, PWM_ERR_INVALID, "pwm_server_result: unknown job");
+    SrvJob& J = it->second;
+    if (J.state != 2) return sfail(s, PWM_ERR_STATE, "pwm_server_result: job has not finished");
+    if (lambda) *lambda = J.lambda;
+    if (ynorm) *ynorm = J.ynorm;
+    if (iters) *iters = J.iters;
+    hipError_t e = hipSuccess;
+    if (vec) {
+        const int64_t n = s->mats[J.mat].n;
+        if (s->vtype == PWM_VALUES_F64) {
+            e = hipMemcpyAsync(vec, J.last, n * 8, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        } else {
+            std::vector<float> f(n);
+            e = hipMemcpyAsync(f.data(), J.last, n * 4, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+            for (int64_t i = 0; i < n; ++i) vec[i] = f[i];
+        }
+    }
+    if (J.has_block) s->arena.release(J.off, J.bytes);
+    s->jobs.erase(it);
+    if (e != hipSuccess) return shp(s, e, "pwm_server_result");
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_stats(const pwm_server* s, int64_t* used_bytes, int32_t* resident, int32_t* queued,
+                                int32_t* running, int64_t* finished) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_stats: NULL");
+    if (used_bytes) *used_bytes = (int64_t)s->arena.used();
+    if (resident) {
+        int r = 0;
+        for (const SrvMatrix& M : s->mats) r += M.resident;
+        *resident = r;
+    }
+    if (queued) *queued = (int32_t)s->queue.size();
+    if (running) *running = s->eng.active();
+    if (finished) *finished = s->finished_total;
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_matrix_resident(const pwm_server* s, int32_t mat, int32_t* resident) {
+    if (!s || mat < 0 || mat >= (int32_t)s->mats.size() || !resident)
+        return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_matrix_resident: bad argument");
+    *resident = s->mats[mat].resident ? 1 : 0;
+    return PWM_OK;
+}

@@ -225,7 +225,9 @@ int pwm_batch_start_vector(pwm_batch* b, int32_t id, uint64_t seed, double* out)
 int pwm_batch_vector(pwm_batch* b, int32_t id, double* out, double* ynorm);
 int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_t* nnz);
 /* "items_per_thread" (4, 8, 16: tile = 256 x items merge items),
- * "chunk" (steps per graph replay, 1..4096), "graph" (0: plain launches). */
+ * "chunk" (steps per graph replay, 1..4096), "graph" (0: plain launches),
+ * "persistent" (0: one workgroup per tile instead of resident workgroups
+ * that prefetch their next tile). */
 int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value);
 /* Kernel launches, graph instantiations and work-list builds so far. */
 int pwm_batch_stats(const pwm_batch* b, int64_t* launches, int64_t* graphs, int64_t* lists);
@@ -247,7 +249,8 @@ int pwm_server_create(int value_type, int64_t budget_bytes, int32_t max_jobs, pw
 int pwm_server_destroy(pwm_server* s);
 const char* pwm_server_last_error(const pwm_server* s);
 /* "chunk" (steps per pwm_server_step, default 8), "graph" (0: plain
- * launches), "items_per_thread" (4, 8, 16; only with nothing resident). */
+ * launches), "persistent" (as for pwm_batch_set_option), "items_per_thread"
+ * (4, 8, 16; only with nothing resident). */
 int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
 #define PWM_SERVER_BORROW 1 /* keep the caller's arrays (alive until removed) instead of a copy */
 int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* row_start, const int32_t* col_ind,

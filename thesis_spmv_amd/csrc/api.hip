@@ -30,6 +30,12 @@ hipError_t DevBuf::alloc(size_t n) {
     bytes = n;
     return hipSuccess;
 }
+hipError_t alloc_padded(DevBuf& b, int64_t n, size_t elem) {
+    hipError_t e = b.alloc((size_t)(n + kPadEntries) * elem);
+    if (e == hipSuccess) e = hipMemset((char*)b.p + (size_t)n * elem, 0, (size_t)kPadEntries * elem);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);  // before any handle stream reads it
+    return e;
+}
 void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
@@ -274,7 +280,7 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
 // ---------------------------------------------------------------- loading
 
 static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
-    hipError_t e = h->val.alloc(std::max<int64_t>(nnz, 1) * value_size(h->vtype));
+    hipError_t e = alloc_padded(h->val, nnz, value_size(h->vtype));
     if (e != hipSuccess) return hip_fail(h, e, "value allocation");
     if (nnz == 0) return PWM_OK;
     if (h->vtype == PWM_VALUES_F64) {
@@ -334,7 +340,7 @@ static int load_crs_host(pwm_matrix* h, int64_t nor, int64_t noc, const int32_t*
     h->noc = noc;
     h->nnz = nnz;
     hipError_t e;
-    if ((e = h->rs.alloc((nor + 1) * 4)) != hipSuccess || (e = h->ci.alloc(std::max<int64_t>(nnz, 1) * 4)) != hipSuccess)
+    if ((e = h->rs.alloc((nor + 1) * 4)) != hipSuccess || (e = alloc_padded(h->ci, nnz, 4)) != hipSuccess)
         return hip_fail(h, e, "CRS allocation");
     if ((e = hipMemcpy(h->rs.p, rs, (nor + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (nnz && (e = hipMemcpy(h->ci.p, ci, nnz * 4, hipMemcpyHostToDevice)) != hipSuccess))
@@ -423,8 +429,8 @@ extern "C" int pwm_generate_poisson(pwm_matrix* h, int64_t m, int64_t n, int par
     h->nor = h->noc = N;
     h->nnz = Z;
     hipError_t e;
-    if ((e = h->rs.alloc((N + 1) * 4)) != hipSuccess || (e = h->ci.alloc(Z * 4)) != hipSuccess ||
-        (e = h->val.alloc(Z * value_size(h->vtype))) != hipSuccess)
+    if ((e = h->rs.alloc((N + 1) * 4)) != hipSuccess || (e = alloc_padded(h->ci, Z, 4)) != hipSuccess ||
+        (e = alloc_padded(h->val, Z, value_size(h->vtype))) != hipSuccess)
         return hip_fail(h, e, "Poisson allocation");
     e = pwmk::launch_fill_poisson(m, n, h->rs.as<int32_t>(), h->ci.as<int32_t>(), h->val.p, h->vtype, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

@@ -56,6 +56,12 @@ struct DevBuf {
     template <typename T> T* as() const { return (T*)p; }
 };
 
+// col_ind / value buffers that merge-path tiles read end in kPadEntries zero
+// entries: tiles load aligned groups of 4 entries.
+constexpr int64_t kPadEntries = 4;
+// n entries of `elem` bytes plus the zeroed pad.
+hipError_t alloc_padded(DevBuf& b, int64_t n, size_t elem);
+
 // CSB device layout (SOA_Implementations/CSB.hpp:34-59).
 struct CsbDev {
     int32_t bits = 0;

@@ -33,6 +33,7 @@ struct BatchInst {
     // arena offsets
     size_t o_rs = 0, o_ci = 0, o_val = 0, o_tiles = 0, o_part = 0, o_carry = 0, o_vec = 0;
     int32_t n_tiles = 0;
+    std::vector<pwmk::ETile> tiles;  // host copy of the plan
     int slot = -1;
 };
 
@@ -49,6 +50,7 @@ struct pwm_batch {
     int ipt = 8;
     int chunk = 32;
     int use_graph = 1;
+    int persistent = 0;
     pwm_impl::Arena arena;
     pwm_impl::Engine* eng = nullptr;
     std::vector<double> lambda, ynorm;
@@ -165,6 +167,10 @@ extern "C" int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value
         b->use_graph = value != 0;
         return PWM_OK;
     }
+    if (k == "persistent") {
+        b->persistent = value != 0;
+        return PWM_OK;
+    }
     return bfail(b, PWM_ERR_INVALID, "pwm_batch_set_option: unknown option " + k);
 }
 
@@ -176,7 +182,8 @@ static int batch_upload(pwm_batch* b) {
     size_t total = 0;
     for (BatchInst& I : b->inst) {
         I.n_tiles = pwmk::engine_tiles_for(I.n, I.nnz, b->ipt);
-        total += Arena::round((I.n + 1) * 4) + Arena::round(I.nnz * 4) + Arena::round(I.nnz * vs) +
+        const int64_t zp = I.nnz + pwm_impl::kPadEntries;  // tiles read aligned groups of 4
+        total += Arena::round((I.n + 1) * 4) + Arena::round(zp * 4) + Arena::round(zp * vs) +
                  Arena::round((size_t)I.n_tiles * sizeof(pwmk::ETile)) + Arena::round((size_t)I.n_tiles * 24) +
                  Arena::round((size_t)I.n_tiles * acc) + Arena::round(2 * I.n * xs);
     }
@@ -188,11 +195,16 @@ static int batch_upload(pwm_batch* b) {
     if (max_stage > 0 && (e = tmp.alloc(max_stage * 8)) != hipSuccess) return bhip(b, e, "batch staging");
     Arena& A = b->arena;
     for (BatchInst& I : b->inst) {
-        bool ok = A.alloc((I.n + 1) * 4, &I.o_rs) && A.alloc(I.nnz * 4, &I.o_ci) && A.alloc(I.nnz * vs, &I.o_val) &&
+        const int64_t zp = I.nnz + pwm_impl::kPadEntries;
+        bool ok = A.alloc((I.n + 1) * 4, &I.o_rs) && A.alloc(zp * 4, &I.o_ci) && A.alloc(zp * vs, &I.o_val) &&
                   A.alloc((size_t)I.n_tiles * sizeof(pwmk::ETile), &I.o_tiles) && A.alloc((size_t)I.n_tiles * 24, &I.o_part) &&
                   A.alloc((size_t)I.n_tiles * acc, &I.o_carry) && A.alloc(2 * I.n * xs, &I.o_vec);
         if (!ok) return bfail(b, PWM_ERR_NOMEM, "batch arena exhausted");
         int32_t* rs = (int32_t*)A.at(I.o_rs);
+        e = hipMemsetAsync((char*)A.at(I.o_ci) + I.nnz * 4, 0, pwm_impl::kPadEntries * 4, b->stream);
+        if (e == hipSuccess)
+            e = hipMemsetAsync((char*)A.at(I.o_val) + I.nnz * vs, 0, pwm_impl::kPadEntries * vs, b->stream);
+        if (e != hipSuccess) return bhip(b, e, "batch padding");
         if (I.kind == 0) {
             e = hipMemcpy(rs, I.rs.data(), (I.n + 1) * 4, hipMemcpyHostToDevice);
             if (e == hipSuccess && I.nnz) e = hipMemcpy(A.at(I.o_ci), I.ci.data(), I.nnz * 4, hipMemcpyHostToDevice);
@@ -205,6 +217,11 @@ static int batch_upload(pwm_batch* b) {
         }
         if (e == hipSuccess)
             e = pwmk::engine_plan_tiles(rs, I.n, I.nnz, b->ipt, (pwmk::ETile*)A.at(I.o_tiles), b->stream);
+        I.tiles.resize(I.n_tiles);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(I.tiles.data(), A.at(I.o_tiles), (size_t)I.n_tiles * sizeof(pwmk::ETile),
+                               hipMemcpyDeviceToHost, b->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch upload");
     }
     if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bhip(b, e, "batch upload");
@@ -239,6 +256,7 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
     if (E.ipt() != b->ipt && (rc = E.set_ipt(b->ipt)) != PWM_OK) return bfail(b, rc, "pwm_batch_solve: items_per_thread");
     E.chunk = b->chunk;
     E.use_graph = b->use_graph != 0;
+    E.persistent = b->persistent != 0;
     const size_t xs = pwm_impl::vector_size(b->vtype);
     pwm_impl::Arena& A = b->arena;
     hipError_t e = hipSuccess;
@@ -264,10 +282,11 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
         j.partials = (double*)A.at(I.o_part);
         j.carry = A.at(I.o_carry);
         j.n = I.n;
+        j.nnz = I.nnz;
         j.max_it = max_it;
         j.tol = tol;
         j.n_tiles = I.n_tiles;
-        I.slot = E.add(j);
+        I.slot = E.add(j, I.tiles.data());
     }
     std::vector<int> fin;
     std::string err;

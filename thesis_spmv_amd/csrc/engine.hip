@@ -10,9 +10,14 @@
 
 namespace pwmk {
 
-int32_t engine_tiles_for(int64_t n, int64_t nnz, int ipt) {
+static int64_t plan_step(int ipt) {
     const int64_t cap = (int64_t)kEngBS * ipt;
-    return (int32_t)((n + nnz + cap - 1) / cap);
+    return cap - cap / 4;
+}
+
+int32_t engine_tiles_for(int64_t n, int64_t nnz, int ipt) {
+    const int64_t step = plan_step(ipt);
+    return (int32_t)((n + nnz + step - 1) / step);
 }
 
 // searchPathOnDiag (SOA_Implementations/CRS_Merge.hpp:42-62) at diagonal d.
@@ -31,24 +36,35 @@ __device__ __forceinline__ void eng_search(const int32_t* __restrict__ rs, int64
     b = (int32_t)(d - lo);
 }
 
-// Tile t covers diagonals [t*cap, min((t+1)*cap, n+nnz)) of the matrix's merge.
-__global__ void k_eng_plan(const int32_t* __restrict__ rs, int64_t n, int64_t nnz, int64_t cap, int32_t nt,
-                           ETile* __restrict__ tiles) {
+// Cut p of the merge path: the path point at diagonal min(p*step, n+nnz),
+// moved back to the start of its row when that row has <= snap nonzeros.
+__device__ __forceinline__ void eng_cut(const int32_t* __restrict__ rs, int64_t n, int64_t nnz, int64_t p,
+                                        int64_t step, int64_t snap, int32_t& a, int32_t& b) {
+    const int64_t total = n + nnz;
+    eng_search(rs, n, nnz, p * step < total ? p * step : total, a, b);
+    if (a < n && (int64_t)rs[a + 1] - rs[a] <= snap) b = rs[a];
+}
+
+// Tile t runs from cut t to cut t+1 (at most step + snap = 256*ipt items).
+__global__ void k_eng_plan(const int32_t* __restrict__ rs, int64_t n, int64_t nnz, int64_t step, int64_t snap,
+                           int32_t nt, ETile* __restrict__ tiles) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
-    const int64_t total = n + nnz;
-    const int64_t d0 = t * cap < total ? t * cap : total;
-    const int64_t d1 = d0 + cap < total ? d0 + cap : total;
     ETile T;
-    eng_search(rs, n, nnz, d0, T.a0, T.b0);
-    eng_search(rs, n, nnz, d1, T.a1, T.b1);
+    eng_cut(rs, n, nnz, t, step, snap, T.a0, T.b0);
+    eng_cut(rs, n, nnz, t + 1, step, snap, T.a1, T.b1);
+    int stream = T.b0 == rs[T.a0] && T.b1 == rs[T.a1];
+    for (int32_t r = T.a0; stream && r < T.a1; ++r) stream = rs[r + 1] - rs[r] <= kStreamRow;
+    T.stream = stream;
+    T.pad[0] = T.pad[1] = T.pad[2] = 0;
     tiles[t] = T;
 }
 
 hipError_t engine_plan_tiles(const int32_t* rs, int64_t n, int64_t nnz, int ipt, ETile* tiles, hipStream_t s) {
     const int32_t nt = engine_tiles_for(n, nnz, ipt);
-    hipLaunchKernelGGL(k_eng_plan, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, rs, n, nnz,
-                       (int64_t)kEngBS * ipt, nt, tiles);
+    const int64_t step = plan_step(ipt), snap = (int64_t)kEngBS * ipt / 4;
+    hipLaunchKernelGGL(k_eng_plan, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, rs, n, nnz, step, snap, nt,
+                       tiles);
     return hipGetLastError();
 }
 
@@ -91,36 +107,117 @@ hipError_t engine_normalize(void* v, int64_t n, int vtype, double inv_norm, hipS
     return hipGetLastError();
 }
 
-// One workgroup per (slot, tile) of the work list: the tile's rows of
-// y = (A x) * scale, its carry and its FP64 partials.
+// The step-dependent view of a job: x, y, scale (x and y swap every step).
+template <typename X>
+struct EView {
+    const X* x;
+    X* y;
+    double scale;
+    double* partials;
+    void* carry;
+};
+
+template <typename X>
+__device__ __forceinline__ bool eng_view(const EJob* __restrict__ jobs, const EState* __restrict__ st, int s,
+                                         EView<X>& v) {
+    if (s < 0 || st[s].done) return false;
+    const EJob& J = jobs[s];
+    const int64_t it = st[s].it;
+    v.x = (const X*)((it & 1) ? J.buf1 : J.buf0);
+    v.y = (X*)((it & 1) ? J.buf0 : J.buf1);
+    v.scale = st[s].scale;
+    v.partials = J.partials;
+    v.carry = J.carry;
+    return true;
+}
+
+// Stage (after tile_load), barrier, compute, carry and FP64 partials of one
+// tile.  `after_stage` runs right after the staging barrier.
+template <typename VT, int ITEMS, typename F>
+__device__ __forceinline__ void eng_tile(const EWork& W, const EView<typename Arith<VT>::X>& v,
+                                         const TileRegs<VT, ITEMS>& R,
+                                         MergeTileSmem<typename Arith<VT>::A, kEngBS, ITEMS>& sm, double* red,
+                                         F&& after_stage) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    const int na = W.a1 - W.a0;
+    X xr[ITEMS];
+    if (W.stream) stream_xrows<VT, kEngBS, ITEMS>(v.x, W.a0, na, xr);
+    tile_stage<VT, kEngBS, ITEMS>(R, W.rs, v.x, W.a0, na, W.b0, W.b1, sm);
+    __syncthreads();
+    after_stage();
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    Acc c = 0;
+    if (W.stream)
+        stream_compute<VT, kEngBS, ITEMS>(v.y, xr, W.a0, W.a1, W.b0, v.scale, sm, yy, xy, xx);
+    else
+        c = merge_compute<VT, kEngBS, ITEMS, true>(W.rs, v.x, v.y, W.a0, W.a1, W.b0, W.b1, v.scale, sm, yy, xy, xx);
+    if (threadIdx.x == kEngBS - 1) G((Acc*)v.carry)[W.tile] = c;
+    block_sum3<kEngBS>(yy, xy, xx, red);  // its closing barrier also frees the LDS tile
+    if (threadIdx.x == 0) {
+        const auto p = G(v.partials + 3 * (int64_t)W.tile);
+        p[0] = yy;
+        p[1] = xy;
+        p[2] = xx;
+    }
+}
+
+// One workgroup per entry of the work list.  Entry b runs on XCD b % 8
+// (workgroups are dispatched round-robin over the XCDs); the host laid the
+// list out for that.
+// The tile kernels take their read-only arrays as restrict parameters, so the
+// compiler reads the uniform work entry, job and state with scalar loads.
 template <typename VT, int ITEMS>
-__global__ __launch_bounds__(kEngBS) void k_eng_tiles(EArgs a) {
+__global__ __launch_bounds__(kEngBS) void k_eng_tiles(const EWork* __restrict__ work, const EJob* __restrict__ jobs,
+                                                      const EState* __restrict__ st, const ECtl* __restrict__ ctl) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
     __shared__ MergeTileSmem<Acc, kEngBS, ITEMS> sm;
-    __shared__ double red[16];
-    const int nw = a.ctl->n_work;
-    if ((int)blockIdx.x >= nw) return;
-    const int2 w = a.work[xcd_remap(blockIdx.x, nw)];
-    const int s = w.x, t = w.y;
-    if (a.st[s].done) return;
-    const EJob& J = a.jobs[s];
-    const int64_t it = a.st[s].it;
-    const double scale = a.st[s].scale;
-    const X* __restrict__ x = (const X*)((it & 1) ? J.buf1 : J.buf0);
-    X* __restrict__ y = (X*)((it & 1) ? J.buf0 : J.buf1);
-    const ETile T = J.tiles[t];
-    double yy = 0.0, xy = 0.0, xx = 0.0;
-    const Acc c = merge_tile<VT, kEngBS, ITEMS, true>(J.rs, J.ci, (const VT*)J.val, x, y, T.a0, T.a1, T.b0, T.b1,
-                                                      scale, sm, yy, xy, xx);
-    const double ra = block_sum<kEngBS>(yy, red);
-    const double rb = block_sum<kEngBS>(xy, red);
-    const double rc = block_sum<kEngBS>(xx, red);
-    if (threadIdx.x == 0) {
-        J.partials[3 * (int64_t)t + 0] = ra;
-        J.partials[3 * (int64_t)t + 1] = rb;
-        J.partials[3 * (int64_t)t + 2] = rc;
-        ((Acc*)J.carry)[t] = c;
+    __shared__ double red[48];
+    if ((int)blockIdx.x >= ctl->n_work) return;
+    const EWork W = work[blockIdx.x];
+    EView<X> v;
+    if (!eng_view(jobs, st, W.slot, v)) return;
+    TileRegs<VT, ITEMS> R;
+    tile_load<VT, kEngBS, ITEMS>(R, W.ci, (const VT*)W.val, W.b0, W.b1);
+    eng_tile<VT, ITEMS>(W, v, R, sm, red, [] {});
+}
+
+// Persistent variant: gridDim.x (a multiple of 8, at most the resident
+// capacity) workgroups; workgroup b takes entries b, b+G, b+2G, ... (all on
+// its own XCD).  The next entry is read while the current tile is staged,
+// and its col_ind/value loads are issued right after the staging barrier, so
+// they are in flight while the current tile is computed.
+template <typename VT, int ITEMS>
+__global__ __launch_bounds__(kEngBS) void k_eng_tiles_p(const EWork* __restrict__ work, const EJob* __restrict__ jobs,
+                                                        const EState* __restrict__ st, const ECtl* __restrict__ ctl) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    __shared__ MergeTileSmem<Acc, kEngBS, ITEMS> sm;
+    __shared__ double red[48];
+    const int nw = ctl->n_work;
+    const int Gs = gridDim.x;
+    int e = blockIdx.x;
+    if (e >= nw) return;
+    EWork W = work[e];
+    TileRegs<VT, ITEMS> R;
+    if (W.slot >= 0) tile_load<VT, kEngBS, ITEMS>(R, W.ci, (const VT*)W.val, W.b0, W.b1);
+    for (;;) {
+        const int en = e + Gs;
+        EWork Wn;
+        Wn.slot = -1;
+        if (en < nw) Wn = work[en];
+        EView<X> v;
+        if (eng_view(jobs, st, W.slot, v)) {
+            eng_tile<VT, ITEMS>(W, v, R, sm, red, [&] {
+                if (Wn.slot >= 0) tile_load<VT, kEngBS, ITEMS>(R, Wn.ci, (const VT*)Wn.val, Wn.b0, Wn.b1);
+            });
+        } else if (Wn.slot >= 0) {
+            tile_load<VT, kEngBS, ITEMS>(R, Wn.ci, (const VT*)Wn.val, Wn.b0, Wn.b1);
+        }
+        if (en >= nw) break;
+        e = en;
+        W = Wn;
     }
 }
 
@@ -182,28 +279,62 @@ __global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
 }
 
 template <typename VT, int ITEMS>
-static hipError_t step_t(const EArgs& a, int gw, int gs, hipStream_t s) {
-    hipLaunchKernelGGL((k_eng_tiles<VT, ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a);
+static hipError_t step_t(const EArgs& a, int gw, int gp, int gs, hipStream_t s) {
+    if (gp > 0)
+        hipLaunchKernelGGL((k_eng_tiles_p<VT, ITEMS>), dim3(gp), dim3(kEngBS), 0, s, a.work, a.jobs,
+                           (const EState*)a.st, a.ctl);
+    else
+        hipLaunchKernelGGL((k_eng_tiles<VT, ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a.work, a.jobs,
+                           (const EState*)a.st, a.ctl);
     hipLaunchKernelGGL((k_eng_finish<VT>), dim3(gs), dim3(kEngBS), 0, s, a);
     return hipGetLastError();
 }
 
+template <typename VT, int ITEMS>
+static int resident_t() {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eng_tiles_p<VT, ITEMS>, kEngBS, 0) != hipSuccess)
+        return 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return per_cu * cus / 8 * 8;
+}
+
 template <typename VT>
-static hipError_t step_v(const EArgs& a, int ipt, int gw, int gs, hipStream_t s) {
+static hipError_t step_v(const EArgs& a, int ipt, int gw, int gp, int gs, hipStream_t s) {
     switch (ipt) {
-        case 4: return step_t<VT, 4>(a, gw, gs, s);
-        case 8: return step_t<VT, 8>(a, gw, gs, s);
-        case 16: return step_t<VT, 16>(a, gw, gs, s);
+        case 4: return step_t<VT, 4>(a, gw, gp, gs, s);
+        case 8: return step_t<VT, 8>(a, gw, gp, gs, s);
+        case 16: return step_t<VT, 16>(a, gw, gp, gs, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t engine_step(const EArgs& a, int vtype, int ipt, int gw, int gs, hipStream_t s) {
+template <typename VT>
+static int resident_v(int ipt) {
+    switch (ipt) {
+        case 4: return resident_t<VT, 4>();
+        case 8: return resident_t<VT, 8>();
+        case 16: return resident_t<VT, 16>();
+        default: return 0;
+    }
+}
+
+hipError_t engine_step(const EArgs& a, int vtype, int ipt, int gw, int gp, int gs, hipStream_t s) {
     switch (vtype) {
-        case VT_F64: return step_v<double>(a, ipt, gw, gs, s);
-        case VT_F32: return step_v<float>(a, ipt, gw, gs, s);
-        case VT_BF16: return step_v<bf16_t>(a, ipt, gw, gs, s);
+        case VT_F64: return step_v<double>(a, ipt, gw, gp, gs, s);
+        case VT_F32: return step_v<float>(a, ipt, gw, gp, gs, s);
+        case VT_BF16: return step_v<bf16_t>(a, ipt, gw, gp, gs, s);
         default: return hipErrorInvalidValue;
+    }
+}
+
+int engine_resident_blocks(int vtype, int ipt) {
+    switch (vtype) {
+        case VT_F64: return resident_v<double>(ipt);
+        case VT_F32: return resident_v<float>(ipt);
+        case VT_BF16: return resident_v<bf16_t>(ipt);
+        default: return 0;
     }
 }
 
@@ -307,6 +438,7 @@ int Engine::init(int vtype, int max_slots, hipStream_t stream, std::string& err)
     memset(h_state_, 0, ns * sizeof(pwmk::EState));
     memset(h_ctl_, 0, sizeof(pwmk::ECtl));
     in_use_.assign(ns, 0);
+    h_tiles_.assign(ns, nullptr);
     running_.assign(ns, 0);
     fresh_.assign(ns, 0);
     return PWM_OK;
@@ -320,10 +452,11 @@ int Engine::set_ipt(int v) {
     return PWM_OK;
 }
 
-int Engine::add(const pwmk::EJob& j) {
+int Engine::add(const pwmk::EJob& j, const pwmk::ETile* host_tiles) {
     for (int s = 0; s < max_slots_; ++s) {
         if (in_use_[s]) continue;
         h_jobs_[s] = j;
+        h_tiles_[s] = host_tiles;
         pwmk::EState st;
         memset(&st, 0, sizeof(st));
         st.scale = 1.0;
@@ -353,14 +486,44 @@ void* Engine::last_buffer(int slot) const {
     return (h_state_[slot].it & 1) ? j.buf1 : j.buf0;
 }
 
-int Engine::build_lists(std::string& err) {
-    int64_t nw = 0;
-    int ns = 0;
-    for (int s = 0; s < max_slots_; ++s)
-        if (running_[s]) {
-            nw += h_jobs_[s].n_tiles;
-            ++ns;
+// Work list layout: entry b runs on XCD b % 8, so XCD j's entries are
+// j, j+8, j+16, ...  Every job with at least 8 tiles gives each XCD one
+// contiguous eighth of its tiles (neighbouring tiles share x in that XCD's
+// L2, and every XCD gets the same share of every kind of job, so no XCD
+// waits on the others); smaller jobs go whole to the least loaded XCD.
+// Shorter XCD sequences are padded with (-1, -1) entries.
+static void xcd_sequences(const pwmk::EJob* jobs, const std::vector<char>& running, int max_slots,
+                          std::vector<std::vector<int2>>& seq) {  // (slot, tile) per XCD
+    constexpr int kX = 8;
+    seq.assign(kX, {});
+    int64_t load[kX] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < max_slots; ++s) {
+        if (!running[s]) continue;
+        const int32_t T = jobs[s].n_tiles;
+        if (T >= kX) {
+            for (int j = 0; j < kX; ++j) {
+                const int32_t lo = (int32_t)((int64_t)T * j / kX), hi = (int32_t)((int64_t)T * (j + 1) / kX);
+                for (int32_t t = lo; t < hi; ++t) seq[j].push_back(make_int2(s, t));
+                load[j] += hi - lo;
+            }
+        } else {
+            int j = 0;
+            for (int k = 1; k < kX; ++k)
+                if (load[k] < load[j]) j = k;
+            for (int32_t t = 0; t < T; ++t) seq[j].push_back(make_int2(s, t));
+            load[j] += T;
         }
+    }
+}
+
+int Engine::build_lists(std::string& err) {
+    std::vector<std::vector<int2>> seq;
+    xcd_sequences(h_jobs_, running_, max_slots_, seq);
+    size_t L = 0;
+    for (const auto& q : seq) L = std::max(L, q.size());
+    const int64_t nw = (int64_t)L * (int64_t)seq.size();
+    int ns = 0;
+    for (int s = 0; s < max_slots_; ++s) ns += running_[s] ? 1 : 0;
     if (nw > INT32_MAX / 2) {
         err = "engine: too many tiles in flight";
         return PWM_ERR_INVALID;
@@ -374,8 +537,8 @@ int Engine::build_lists(std::string& err) {
         h_work_ = nullptr;
         work_cap_alloc_ = 0;
         hipError_t e;
-        if ((e = hipMalloc(&d_work_, cap * sizeof(int2))) != hipSuccess ||
-            (e = hipHostMalloc((void**)&h_work_, cap * sizeof(int2), hipHostMallocDefault)) != hipSuccess) {
+        if ((e = hipMalloc(&d_work_, cap * sizeof(pwmk::EWork))) != hipSuccess ||
+            (e = hipHostMalloc((void**)&h_work_, cap * sizeof(pwmk::EWork), hipHostMallocDefault)) != hipSuccess) {
             err = std::string("engine work list: ") + hipGetErrorString(e);
             return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
         }
@@ -385,13 +548,30 @@ int Engine::build_lists(std::string& err) {
             graph_ = nullptr;
         }
     }
-    int64_t k = 0;
+    const int nx = (int)seq.size();
+    for (size_t i = 0; i < L; ++i)
+        for (int j = 0; j < nx; ++j) {
+            pwmk::EWork& W = h_work_[i * nx + j];
+            memset(&W, 0, sizeof(W));
+            W.slot = -1;
+            if (i >= seq[j].size()) continue;
+            const int s = seq[j][i].x, t = seq[j][i].y;
+            const pwmk::EJob& J = h_jobs_[s];
+            const pwmk::ETile& T = h_tiles_[s][t];
+            W.rs = J.rs;
+            W.ci = J.ci;
+            W.val = J.val;
+            W.slot = s;
+            W.tile = t;
+            W.a0 = T.a0;
+            W.a1 = T.a1;
+            W.b0 = T.b0;
+            W.b1 = T.b1;
+            W.stream = T.stream;
+        }
     int j = 0;
     for (int s = 0; s < max_slots_; ++s)
-        if (running_[s]) {
-            for (int32_t t = 0; t < h_jobs_[s].n_tiles; ++t) h_work_[k++] = make_int2(s, t);
-            h_slots_[j++] = s;
-        }
+        if (running_[s]) h_slots_[j++] = s;
     n_work_ = (int)nw;
     n_slots_ = ns;
     n_stale_work_ = 0;
@@ -404,7 +584,8 @@ int Engine::build_lists(std::string& err) {
     const int gs = std::min(max_slots_, std::max(64, (n_slots_ * 5 / 4 + 63) / 64 * 64));
     if (n_slots_ > grid_slots_ || n_slots_ < grid_slots_ / 4) grid_slots_ = std::max(gs, n_slots_);
     hipError_t e = hipSuccess;
-    if (n_work_ > 0) e = hipMemcpyAsync(d_work_, h_work_, (size_t)n_work_ * sizeof(int2), hipMemcpyHostToDevice, stream_);
+    if (n_work_ > 0)
+        e = hipMemcpyAsync(d_work_, h_work_, (size_t)n_work_ * sizeof(pwmk::EWork), hipMemcpyHostToDevice, stream_);
     if (e == hipSuccess && n_slots_ > 0)
         e = hipMemcpyAsync(d_slots_, h_slots_, (size_t)n_slots_ * 4, hipMemcpyHostToDevice, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(d_ctl_, h_ctl_, sizeof(pwmk::ECtl), hipMemcpyHostToDevice, stream_);
@@ -442,11 +623,20 @@ int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
     pwmk::EArgs a;
     a.jobs = (const pwmk::EJob*)d_jobs_;
     a.st = (pwmk::EState*)d_state_;
-    a.work = (const int2*)d_work_;
+    a.work = (const pwmk::EWork*)d_work_;
     a.slots = (const int32_t*)d_slots_;
     a.ctl = (const pwmk::ECtl*)d_ctl_;
+    // persistent tile grid: the resident capacity, at most the list length
+    int gp = 0;
+    if (persistent) {
+        if (resident_ipt_ != ipt_) {
+            resident_ = pwmk::engine_resident_blocks(vtype_, ipt_);
+            resident_ipt_ = ipt_;
+        }
+        if (resident_ >= 8) gp = std::min(resident_, (n_work_ + 7) / 8 * 8);
+    }
     if (use_graph) {
-        if (graph_ && (graph_steps_ != steps || graph_gw_ != grid_work_ || graph_gs_ != grid_slots_)) {
+        if (graph_ && (graph_steps_ != steps || graph_gw_ != grid_work_ || graph_gs_ != grid_slots_ || graph_gp_ != gp)) {
             (void)hipGraphExecDestroy(graph_);
             graph_ = nullptr;
         }
@@ -455,7 +645,7 @@ int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
             e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
             hipError_t le = hipSuccess;
             for (int i = 0; i < steps && e == hipSuccess && le == hipSuccess; ++i)
-                le = pwmk::engine_step(a, vtype_, ipt_, grid_work_, grid_slots_, stream_);
+                le = pwmk::engine_step(a, vtype_, ipt_, grid_work_, gp, grid_slots_, stream_);
             if (e == hipSuccess) e = hipStreamEndCapture(stream_, &g);
             if (e == hipSuccess) e = le;
             if (e == hipSuccess) e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
@@ -468,12 +658,13 @@ int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
             graph_steps_ = steps;
             graph_gw_ = grid_work_;
             graph_gs_ = grid_slots_;
+            graph_gp_ = gp;
             ++graph_builds;
         }
         e = hipGraphLaunch(graph_, stream_);
     } else {
         for (int i = 0; i < steps && e == hipSuccess; ++i)
-            e = pwmk::engine_step(a, vtype_, ipt_, grid_work_, grid_slots_, stream_);
+            e = pwmk::engine_step(a, vtype_, ipt_, grid_work_, gp, grid_slots_, stream_);
     }
     launches += 2 * steps;
     if (e == hipSuccess)

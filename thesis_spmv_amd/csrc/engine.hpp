@@ -36,9 +36,16 @@
 
 namespace pwmk {
 
-struct ETile {  // one merge-path tile, matrix-relative offsets
+struct ETile {  // rows [a0,a1), nonzeros [b0,b1), matrix-relative
     int32_t a0, a1, b0, b1;
+    int32_t stream;  // 1: row-aligned, every row <= kStreamRow nonzeros
+    int32_t pad[3];
 };
+
+// Tiles hold at most 256*ipt merge items.  The planner cuts the merge path
+// every 3/4 of that and moves a cut back to the start of its row when the row
+// has at most 1/4 of it, so short rows are never split.
+constexpr int kStreamRow = 64;
 
 struct EJob {  // device view of one slot
     const int32_t* rs;
@@ -50,6 +57,7 @@ struct EJob {  // device view of one slot
     double* partials;  // 3 per tile
     void* carry;       // one accumulator per tile
     int64_t n;
+    int64_t nnz;
     int64_t max_it;
     double tol;
     int32_t n_tiles;
@@ -65,6 +73,18 @@ struct EState {  // per slot, written by k_eng_finish
     int32_t pad;
 };
 
+// One entry of the work list: everything a tile needs to start loading
+// (64 B, one vector load).  slot < 0: padding.
+struct EWork {
+    const int32_t* rs;
+    const int32_t* ci;
+    const void* val;
+    int32_t slot, tile;
+    int32_t a0, a1, b0, b1;
+    int32_t stream;
+    int32_t pad[3];
+};
+
 struct ECtl {  // list lengths, read by the kernels
     int32_t n_work;
     int32_t n_slots;
@@ -74,7 +94,7 @@ struct ECtl {  // list lengths, read by the kernels
 struct EArgs {
     const EJob* jobs;
     EState* st;
-    const int2* work;      // (slot, tile)
+    const EWork* work;
     const int32_t* slots;  // active slots
     const ECtl* ctl;
 };
@@ -85,7 +105,10 @@ int32_t engine_tiles_for(int64_t n, int64_t nnz, int ipt);
 hipError_t engine_plan_tiles(const int32_t* rs, int64_t n, int64_t nnz, int ipt, ETile* tiles, hipStream_t s);
 hipError_t engine_start_vector(void* x, int64_t n, int vtype, uint64_t seed, uint64_t stream_id, hipStream_t s);
 hipError_t engine_normalize(void* v, int64_t n, int vtype, double inv_norm, hipStream_t s);
-hipError_t engine_step(const EArgs& a, int vtype, int ipt, int grid_work, int grid_slots, hipStream_t s);
+// grid_persist > 0: persistent tile kernel with that many workgroups.
+hipError_t engine_step(const EArgs& a, int vtype, int ipt, int grid_work, int grid_persist, int grid_slots,
+                       hipStream_t s);
+int engine_resident_blocks(int vtype, int ipt);
 
 }  // namespace pwmk
 
@@ -122,9 +145,12 @@ class Engine {
     int set_ipt(int v);  // only while no job is active
     int chunk = 32;      // steps per replay
     bool use_graph = true;
+    bool persistent = false;  // persistent, prefetching tile kernel (slower here: fewer waves)
 
     // Claim a free slot for job `j` (state reset to step 0); -1 if none.
-    int add(const pwmk::EJob& j);
+    // `host_tiles` (j.n_tiles entries, the host copy of j.tiles) must stay
+    // valid until the job is released.
+    int add(const pwmk::EJob& j, const pwmk::ETile* host_tiles);
     int active() const { return n_active_; }
     // Run `steps` steps of every active job.  On return `finished` lists
     // the slots whose jobs stopped; their state is in state(slot) and the
@@ -144,7 +170,8 @@ class Engine {
     pwmk::EJob* h_jobs_ = nullptr;      // pinned mirrors
     pwmk::EState* h_state_ = nullptr;
     pwmk::ECtl* h_ctl_ = nullptr;
-    int2* h_work_ = nullptr;
+    pwmk::EWork* h_work_ = nullptr;
+    std::vector<const pwmk::ETile*> h_tiles_;
     int32_t* h_slots_ = nullptr;
     void* d_jobs_ = nullptr;
     void* d_state_ = nullptr;
@@ -158,7 +185,8 @@ class Engine {
     int n_work_ = 0, n_slots_ = 0, n_stale_work_ = 0;
     int grid_work_ = 0, grid_slots_ = 0;
     hipGraphExec_t graph_ = nullptr;
-    int graph_steps_ = 0, graph_gw_ = 0, graph_gs_ = 0;
+    int graph_steps_ = 0, graph_gw_ = 0, graph_gs_ = 0, graph_gp_ = 0;
+    int resident_ = 0, resident_ipt_ = 0;
 };
 
 }  // namespace pwm_impl

@@ -91,6 +91,44 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
     return t;
 }
 
+// Global-address-space view of a device pointer.  Pointers read from memory
+// (job descriptors) are otherwise generic, and generic (flat) accesses count
+// against the LDS/scalar wait counter as well, so every LDS wait would also
+// drain all outstanding global loads.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+    return (gptr<T>)p;
+}
+
+// Three block sums in one pass (same order as three block_sum calls).
+// `scratch` holds >= 48 doubles.
+template <int BS>
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* scratch) {
+    a = group_sum<64>(a);
+    b = group_sum<64>(b);
+    c = group_sum<64>(c);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        scratch[wave] = a;
+        scratch[16 + wave] = b;
+        scratch[32 + wave] = c;
+    }
+    __syncthreads();
+    double ta = 0.0, tb = 0.0, tc = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) {
+        ta += scratch[w];
+        tb += scratch[16 + w];
+        tc += scratch[32 + w];
+    }
+    __syncthreads();
+    a = ta;
+    b = tb;
+    c = tc;
+}
+
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs, each
 // with its own 4 MB L2.  Renumber blocks so that XCD j works on the j-th
 // contiguous eighth of the work items: neighbouring rows (which share x

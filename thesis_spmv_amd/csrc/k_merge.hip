@@ -8,15 +8,10 @@
 // (k_merge_splits, one thread per cut; the cuts are computed once at load
 // time and are bit-identical to the reference's).
 //
-// One 256-thread workgroup per tile:
-//   1. stage the tile's row ends and its products val[k]*x[col_ind[k]] in LDS
-//      (coalesced loads of val/col_ind, x gathered);
-//   2. every thread merges ITEMS consecutive items of the tile (its own
-//      merge-path search inside the tile), writing completed rows to LDS;
-//   3. a segmented inclusive scan over the threads' carries (key = row) adds
-//      the partial sums of rows that span threads;
-//   4. rows completed in the tile are written to y with coalesced stores, and
-//      the tile's trailing partial row is left as (carry_row, carry_val).
+// One 256-thread workgroup per tile runs merge_tile (merge_tile.hpp): LDS
+// staging of the tile's products and row ends, a per-thread merge of ITEMS
+// items, a segmented scan for rows that span threads; the tile's trailing
+// partial row is left as (carry_row, carry_val).
 // k_merge_fixup then adds the carries (the reference's serial fix-up,
 // :136-140): each run of equal carry rows is summed in tile order and added
 // once, so the result does not depend on scheduling.
@@ -54,7 +49,7 @@ __global__ __launch_bounds__(kMBS) void k_merge_spmv(CrsDev A, MergePlan mp, con
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
     __shared__ MergeTileSmem<Acc, kMBS, ITEMS> sm;
-    __shared__ double red[16];
+    __shared__ double red[48];
 
     const X* __restrict__ x = x_arg;
     X* __restrict__ y = y_arg;
@@ -76,13 +71,11 @@ __global__ __launch_bounds__(kMBS) void k_merge_spmv(CrsDev A, MergePlan mp, con
         ((Acc*)mp.carry_val)[p] = carry;
     }
     if constexpr (POWER) {
-        const double ra = block_sum<kMBS>(yy, red);
-        const double rb = block_sum<kMBS>(xy, red);
-        const double rc = block_sum<kMBS>(xx, red);
+        block_sum3<kMBS>(yy, xy, xx, red);
         if (threadIdx.x == 0) {
-            pw.partials[3 * p + 0] = ra;
-            pw.partials[3 * p + 1] = rb;
-            pw.partials[3 * p + 2] = rc;
+            pw.partials[3 * p + 0] = yy;
+            pw.partials[3 * p + 1] = xy;
+            pw.partials[3 * p + 2] = xx;
         }
     }
 }

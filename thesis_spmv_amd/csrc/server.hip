@@ -36,6 +36,7 @@ struct SrvMatrix {
     size_t off = 0, bytes = 0;
     size_t o_ci = 0, o_val = 0, o_tiles = 0;  // inside the block
     int32_t n_tiles = 0;
+    std::vector<pwmk::ETile> tiles;  // host copy of the plan while resident
     int refs = 0;
     uint64_t last_use = 0;
 };
@@ -90,9 +91,10 @@ static void matrix_layout(const pwm_server* s, SrvMatrix& M) {
     using pwm_impl::Arena;
     const size_t vs = pwm_impl::value_size(s->vtype);
     M.n_tiles = pwmk::engine_tiles_for(M.n, M.nnz, s->ipt);
+    const int64_t zp = M.nnz + pwm_impl::kPadEntries;  // tiles read aligned groups of 4
     M.o_ci = Arena::round((M.n + 1) * 4);
-    M.o_val = M.o_ci + Arena::round(M.nnz * 4);
-    M.o_tiles = M.o_val + Arena::round(M.nnz * vs);
+    M.o_val = M.o_ci + Arena::round(zp * 4);
+    M.o_tiles = M.o_val + Arena::round(zp * vs);
     M.bytes = M.o_tiles + Arena::round((size_t)M.n_tiles * sizeof(pwmk::ETile));
 }
 
@@ -158,6 +160,10 @@ extern "C" int pwm_server_set_option(pwm_server* s, const char* key, int64_t val
     if (k == "graph") {
         s->use_graph = value != 0;
         s->eng.use_graph = s->use_graph != 0;
+        return PWM_OK;
+    }
+    if (k == "persistent") {
+        s->eng.persistent = value != 0;
         return PWM_OK;
     }
     if (k == "items_per_thread") {
@@ -279,10 +285,12 @@ static int make_resident(pwm_server* s, int mi) {
     SrvMatrix& M = s->mats[mi];
     char* base = (char*)s->arena.at(M.off);
     int32_t* rs = (int32_t*)base;
-    hipError_t e = hipSuccess;
-    if (M.kind == 1) {
+    const size_t vs = pwm_impl::value_size(s->vtype);
+    hipError_t e = hipMemsetAsync(base + M.o_ci + M.nnz * 4, 0, pwm_impl::kPadEntries * 4, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(base + M.o_val + M.nnz * vs, 0, pwm_impl::kPadEntries * vs, s->stream);
+    if (e == hipSuccess && M.kind == 1) {
         e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, s->vtype, s->stream);
-    } else {
+    } else if (e == hipSuccess) {
         e = hipMemcpyAsync(rs, M.rs_p, (M.n + 1) * 4, hipMemcpyHostToDevice, s->stream);
         if (e == hipSuccess && M.nnz) e = hipMemcpyAsync(base + M.o_ci, M.ci_p, M.nnz * 4, hipMemcpyHostToDevice, s->stream);
         if (e == hipSuccess && M.nnz) {
@@ -319,6 +327,11 @@ static int make_resident(pwm_server* s, int mi) {
     }
     if (e == hipSuccess)
         e = pwmk::engine_plan_tiles(rs, M.n, M.nnz, s->ipt, (pwmk::ETile*)(base + M.o_tiles), s->stream);
+    M.tiles.resize(M.n_tiles);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(M.tiles.data(), base + M.o_tiles, (size_t)M.n_tiles * sizeof(pwmk::ETile),
+                           hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return shp(s, e, "matrix upload");
     M.resident = true;
     return PWM_OK;
@@ -377,10 +390,11 @@ static int admit(pwm_server* s) {
         j.partials = (double*)(base + o_part);
         j.carry = base + o_carry;
         j.n = M.n;
+        j.nnz = M.nnz;
         j.max_it = J.max_it;
         j.tol = J.tol;
         j.n_tiles = M.n_tiles;
-        const int slot = s->eng.add(j);
+        const int slot = s->eng.add(j, M.tiles.data());
         if (slot < 0) {  // cannot happen: a slot was free above
             s->arena.release(J.off, J.bytes);
             J.has_block = false;

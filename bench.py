@@ -1,7 +1,13 @@
 """Benchmark: SpMV GB/s on one MI355X (default workload: power-law CRS,
 N = 4 194 304, ~67M nnz, FP64 values, int32 indices).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload powerlaw|poisson|stencil9]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload powerlaw|poisson|stencil9|batch|serve]
+
+batch: 512 uneven SPD power methods stepped together (solves/s against the
+plain FP64 per-matrix GPU loop, eigenvalue error against FP64); serve:
+Poisson-arrival requests on the continuous-batching server (p50/p99 latency
+and solves/s against the FP64 per-request path).  Otherwise:
 
 A step is one SpMV y = A x over the whole matrix with A, x, y resident in HBM.
 W untimed steps, then K timed steps between two device synchronisations;
@@ -33,6 +39,23 @@ def algorithmic_bytes(n, z, value_bytes=8, vector_bytes=8):
     return value_bytes * z + 4 * z + 4 * (n + 1) + 2 * vector_bytes * n
 
 
+def poisson_crs(m, n):
+    """2D 5-point Poisson matrix in CRS (4 on the diagonal, -1 to the four
+    neighbours; row r = i*m + j, columns ascending), built with numpy."""
+    N = m * n
+    r = np.arange(N, dtype=np.int64)
+    j = r % m
+    offs = [(-m, r >= m), (-1, j != 0), (0, np.ones(N, bool)), (1, j != m - 1), (m, r < N - m)]
+    cols, vals, keep = [], [], []
+    for d, mask in offs:
+        cols.append(r + d)
+        vals.append(np.full(N, 4.0 if d == 0 else -1.0))
+        keep.append(mask)
+    C, V, K = np.stack(cols, 1), np.stack(vals, 1), np.stack(keep, 1)
+    rs = np.concatenate([[0], np.cumsum(K.sum(1))]).astype(np.int32)
+    return N, rs, C[K].astype(np.int32), V[K]
+
+
 def make_workload(name):
     import thesis_spmv_amd as pw
     t0 = time.time()
@@ -46,9 +69,7 @@ def make_workload(name):
         rs, ci, dv = pw.gen_stencil9(side, 2, 3)
         desc = "9-point stencil 2896x2896 + 2 random off-band nnz per row"
     elif name == "poisson":
-        import oracle  # only to hand the CPU baseline the same CRS arrays
-        A = oracle.poisson_crs(1024, 1024)
-        n, rs, ci, dv = A.nor, A.row_start, A.col_ind, A.data
+        n, rs, ci, dv = poisson_crs(1024, 1024)
         desc = "2D 5-point Poisson 1024x1024"
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -178,6 +199,138 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def _serve_requests(rng, n_req, rate, n_pool):
+    """Poisson arrivals at `rate` req/s; each request is an SpMV chain
+    (length U{1..64}, 75%) or a power solve (tol 1e-7, <= 1000 steps, 25%);
+    20% reuse a matrix that is resident, the rest bring a fresh copy of a
+    pool matrix (upload and tile plan included in their latency)."""
+    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        chain = rng.random() < 0.75
+        reqs.append({"t": float(t[i]), "mat": int(rng.integers(n_pool)), "repeat": bool(rng.random() < 0.2),
+                     "steps": int(rng.integers(1, 65)) if chain else 1000, "tol": -1.0 if chain else 1e-7,
+                     "seed": 1000 + i})
+    return reqs
+
+
+def _percentiles(lat):
+    lat = np.asarray(lat) * 1e3
+    return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+
+
+def bench_serve(args):
+    """Served stream: requests arrive as a Poisson process and are served by
+    the continuous-batching server (FP32 storage, FP64 reductions, 64 GB
+    admission budget) and, for comparison, by the plain FP64 per-request GPU
+    path (one CRS handle per matrix, requests one after another)."""
+    import thesis_spmv_amd as pw
+    rng = np.random.default_rng(5)
+    n_pool = args.serve_pool
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(args.serve_max_n), n_pool)).astype(np.int64)
+    kinds = ["poisson", "powerlaw", "stencil9"]
+    pool = []
+    t0 = time.time()
+    for i, n in enumerate(ns):
+        kind = kinds[i % 3]
+        if kind == "poisson":
+            side = max(2, int(np.sqrt(n)))
+            pool.append({"kind": kind, "side": side, "n": side * side})
+        elif kind == "powerlaw":
+            rs, ci, dv = pw.gen_powerlaw(int(n), alpha=1.8, xm=7.3333, cap=50000, struct_seed=100 + i,
+                                         value_seed=200 + i)
+            pool.append({"kind": kind, "n": int(n), "crs": (rs, ci, dv)})
+        else:
+            side = max(3, int(np.sqrt(n)))
+            rs, ci, dv = pw.gen_stencil9(side, 2, 300 + i)
+            pool.append({"kind": kind, "n": side * side, "crs": (rs, ci, dv)})
+    print(f"# serve pool: {n_pool} matrices, N {min(p['n'] for p in pool)}..{max(p['n'] for p in pool)}, "
+          f"built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def run_server(reqs):
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
+        resident_ids = {}  # pool index -> list of server matrix ids
+        lat, jobs, pending, k = [], {}, list(reqs), 0
+        start = time.perf_counter()
+        while k < len(pending) or jobs:
+            now = time.perf_counter() - start
+            while k < len(pending) and pending[k]["t"] <= now:
+                r = pending[k]
+                p = pool[r["mat"]]
+                ids = [m for m in resident_ids.get(r["mat"], []) if srv.resident(m)]
+                if r["repeat"] and ids:
+                    mid = ids[0]
+                else:
+                    mid = srv.add_poisson(p["side"], p["side"]) if p["kind"] == "poisson" else \
+                        srv.add_matrix(*p["crs"], borrow=True)
+                    resident_ids.setdefault(r["mat"], []).append(mid)
+                jobs[srv.submit(mid, r["steps"], r["tol"], seed=r["seed"])] = r
+                k += 1
+            if not jobs:
+                time.sleep(max(0.0, pending[k]["t"] - (time.perf_counter() - start)))
+                continue
+            srv.step()
+            done_at = time.perf_counter() - start
+            for j in srv.finished():
+                r = jobs.pop(j)
+                srv.result(j)
+                lat.append(done_at - r["t"])
+        makespan = time.perf_counter() - start
+        tm = srv.timing()
+        print(f"#   server phases: admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), run {tm['run_s']:.2f}s, "
+              f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s", file=sys.stderr, flush=True)
+        del srv
+        return lat, makespan
+
+    def run_fp64(reqs):
+        cache, lat = {}, []
+        start = time.perf_counter()
+        for r in reqs:
+            now = time.perf_counter() - start
+            if now < r["t"]:
+                time.sleep(r["t"] - now)
+            p = pool[r["mat"]]
+            m = cache.get(r["mat"]) if r["repeat"] else None
+            if m is None:
+                m = pw.CRSHip(-1, "f64")
+                if p["kind"] == "poisson":
+                    m.generatePoissonMatrix(p["side"], p["side"], 1)
+                else:
+                    rs, ci, dv = p["crs"]
+                    m.loadCRS(len(rs) - 1, len(rs) - 1, rs, ci, dv)
+                cache[r["mat"]] = m
+            n = p["n"]
+            x = pw.DeviceVector.from_array(pw.server_start_vector(n, r["seed"]))
+            y = pw.DeviceVector(n)
+            m.power_method_device(x, y, r["steps"], r["tol"])
+            lat.append(time.perf_counter() - start - r["t"])
+            del x, y
+        return lat, time.perf_counter() - start
+
+    rows = []
+    for rate in args.serve_rates:
+        reqs = _serve_requests(np.random.default_rng(int(rate)), args.serve_requests, rate, n_pool)
+        lat, span = run_server(reqs)
+        row = {"rate": rate, "requests": len(reqs), "server": {"solves_per_s": round(len(lat) / span, 2),
+                                                               **_percentiles(lat)}}
+        print(f"# rate {rate}/s: server {row['server']}", file=sys.stderr, flush=True)
+        if not args.no_baseline:
+            lat64, span64 = run_fp64(reqs)
+            row["fp64_per_request"] = {"solves_per_s": round(len(lat64) / span64, 2), **_percentiles(lat64)}
+            print(f"#   fp64 per-request: {row['fp64_per_request']}", file=sys.stderr, flush=True)
+        rows.append(row)
+    top = rows[-1]
+    out = {"metric": METRIC, "value": top["server"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+           "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"served stream: Poisson arrivals, 75% SpMV chains (1-64) / 25% power solves "
+                                  f"(tol 1e-7, <=1000 steps), 20% resident repeats, 64 GB budget, pool of {n_pool} "
+                                  f"matrices N log-U[1e4,{args.serve_max_n:.0e}]",
+                      "rates": args.serve_rates, "requests_per_rate": args.serve_requests},
+           "p50_ms": top["server"]["p50_ms"], "p99_ms": top["server"]["p99_ms"], "rates": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +349,11 @@ def main():
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 1000.0])
+    ap.add_argument("--serve-requests", type=int, default=120, help="requests per arrival rate")
+    ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
+    ap.add_argument("--serve-max-n", type=float, default=1e7)
+    ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per pwm_server_step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -205,6 +363,8 @@ def main():
     import thesis_spmv_amd as pw
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     if args.calibrate_copy > 0:
         nb = args.calibrate_copy // 16 * 16
         src, dst = pw.DeviceVector(nb // 8), pw.DeviceVector(nb // 8)

@@ -132,6 +132,50 @@ class BlockCOHHip : public SparseMatrixHipBase<T, I, PWM_FORMAT_BLOCKCOH> {
     using SparseMatrixHipBase<T, I, PWM_FORMAT_BLOCKCOH>::SparseMatrixHipBase;
 };
 
+// Fill a Triplet from a Matrix-Market or .bin file with the library's
+// readers (Triplet::loadFromMM / loadFromBin, Matrix/Triplet.hpp:75-216; the
+// same flags as the reference's loaders).  The Triplet's arrays are new[]
+// allocations, as the reference's loaders leave them; the caller owns them.
+namespace detail {
+template <typename T, typename I>
+void take_coo(int rc, int64_t nor, int64_t noc, int64_t nnz, int32_t* r, int32_t* c, double* v,
+              pwm::Triplet<T, I>& t) {
+    if (rc != PWM_OK) throw HipError(rc, pwm_last_error(nullptr));
+    t.row_coord = new I[nnz ? nnz : 1];
+    t.col_coord = new I[nnz ? nnz : 1];
+    t.data = new T[nnz ? nnz : 1];
+    for (int64_t k = 0; k < nnz; ++k) {
+        t.row_coord[k] = (I)r[k];
+        t.col_coord[k] = (I)c[k];
+        t.data[k] = (T)v[k];
+    }
+    t.row_size = (I)nor;
+    t.col_size = (I)noc;
+    t.nnz = (I)nnz;
+    pwm_free(r);
+    pwm_free(c);
+    pwm_free(v);
+}
+}  // namespace detail
+
+template <typename T, typename I>
+void readMatrixMarket(const char* path, bool has_data, bool symmetric, bool random_fill, pwm::Triplet<T, I>& t) {
+    int64_t nor = 0, noc = 0, nnz = 0;
+    int32_t *r = nullptr, *c = nullptr;
+    double* v = nullptr;
+    const int rc = pwm_read_mm(path, has_data, symmetric, random_fill, &nor, &noc, &nnz, &r, &c, &v);
+    detail::take_coo(rc, nor, noc, nnz, r, c, v, t);
+}
+
+template <typename T, typename I>
+void readBin(const char* path, int64_t mat_size, bool symmetric, bool random_vals, pwm::Triplet<T, I>& t) {
+    int64_t nor = 0, noc = 0, nnz = 0;
+    int32_t *r = nullptr, *c = nullptr;
+    double* v = nullptr;
+    const int rc = pwm_read_bin(path, mat_size, symmetric, random_vals, &nor, &noc, &nnz, &r, &c, &v);
+    detail::take_coo(rc, nor, noc, nnz, r, c, v, t);
+}
+
 template <typename T, typename I>
 pwm::SparseMatrix<T, I>* selectTypeHip(int method, int threads) {
     switch (method) {

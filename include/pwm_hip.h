@@ -199,6 +199,22 @@ int pwm_gen_spd_powerlaw(int64_t n, uint64_t seed, int32_t** row_start, int32_t*
 int pwm_gen_uniform(int64_t n, uint64_t seed, double* out);
 void pwm_free(void* p);
 
+/* Triplet::loadFromMM (Matrix/Triplet.hpp:75-154): '%' lines skipped, the
+ * first other line gives rows, cols, entries, entries are 1-based
+ * "row col [value]", an explicit 0 is dropped when has_data, pattern entries
+ * are 1 or (random_fill) uniform_real(-100,100) from mt19937(747846),
+ * symmetric mirrors off-diagonal entries, reading stops at the first empty
+ * line.  COO arrays are allocated by the library (free with pwm_free).
+ * PWM_ERR_IO for an unreadable or malformed file (more entries than
+ * declared, an index outside the declared size). */
+int pwm_read_mm(const char* path, int has_data, int symmetric, int random_fill, int64_t* nor, int64_t* noc,
+                int64_t* nnz, int32_t** rows, int32_t** cols, double** vals);
+/* Triplet::loadFromBin (Matrix/Triplet.hpp:164-216): little-endian uint32
+ * (row, col) pairs, 0-based, entries = file size / 8, mat_size x mat_size,
+ * values 1 or random as above, symmetric mirrors off-diagonal pairs. */
+int pwm_read_bin(const char* path, int64_t mat_size, int symmetric, int random_vals, int64_t* nor, int64_t* noc,
+                 int64_t* nnz, int32_t** rows, int32_t** cols, double** vals);
+
 /* ---- batched power method (addition) ----------------------------------
  * Many independent power methods (the reference's powerIteration,
  * Matrix/CRS.hpp:118-130, plus the Rayleigh-quotient stopping rule of
@@ -261,6 +277,13 @@ int pwm_server_remove_matrix(pwm_server* s, int32_t mat);
 int pwm_server_matrix_resident(const pwm_server* s, int32_t mat, int32_t* resident);
 int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, double tol, uint64_t seed, int64_t* job);
 int pwm_server_step(pwm_server* s, int32_t* running, int32_t* finished);
+/* Ids of the jobs the last pwm_server_step retired: up to `cap` into ids,
+ * their number into *count. */
+int pwm_server_finished(const pwm_server* s, int64_t* ids, int32_t cap, int32_t* count);
+/* Seconds spent so far in pwm_server_step's phases: admission (including
+ * matrix uploads and tile plans), running the engine, retiring jobs; and the
+ * number of matrix uploads. */
+int pwm_server_timing(const pwm_server* s, double* admit_s, double* run_s, double* retire_s, int64_t* uploads);
 /* Finished job: eigenvalue estimate, |y| of the last step, steps taken and
  * (vec non-NULL, n doubles) the normalised final iterate.  PWM_ERR_STATE
  * while the job is queued or running; afterwards the job is forgotten. */

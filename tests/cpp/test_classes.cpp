@@ -85,6 +85,39 @@ static void run_all(const char* itype) {
             expect_close(name, yr, refr, 1e-13);
         }
     }
+    // Matrix-Market file read by the library's reader, loaded through
+    // loadFromTriplets, against the oracle's loader + TripletToCRS + CRS::mv
+    {
+        const char* path = PWM_FIXTURE_DIR "/gre_1107.mtx";
+        pwm::Triplet<double, I> tm;
+        pwm::readMatrixMarket(path, true, false, false, tm);
+        int64_t nr = 0, nc = 0, nz = 0;
+        int32_t *orr = nullptr, *occ = nullptr;
+        double* ov = nullptr;
+        if (orc_load_mm(path, 1, 0, 0, &nr, &nc, &nz, &orr, &occ, &ov) != 0 || nz != (int64_t)tm.nnz) {
+            ++failures;
+            std::printf("[%s] reader: entry count FAIL\n", itype);
+        } else {
+            std::vector<int32_t> mrs(nr + 1), mci(nz);
+            std::vector<double> mdv(nz);
+            orc_triplet_to_crs(nr, nz, orr, occ, ov, mrs.data(), mci.data(), mdv.data());
+            std::unique_ptr<pwm::SparseMatrix<double, I>> mat(pwm::selectTypeHip<double, I>(108, -1));
+            mat->loadFromTriplets(&tm, 1);
+            std::vector<double> xm(nr), ym(nr, 0.0), refm(nr);
+            for (int64_t i = 0; i < nr; ++i) xm[i] = std::cos(0.11 * (double)i);
+            mat->mv(xm.data(), ym.data());
+            orc_crs_mv(nr, mrs.data(), mci.data(), mdv.data(), xm.data(), refm.data());
+            char name[128];
+            std::snprintf(name, sizeof name, "[%s] readMatrixMarket gre_1107 + loadFromTriplets mv", itype);
+            expect_close(name, ym, refm, 1e-13);
+        }
+        orc_free(orr);
+        orc_free(occ);
+        orc_free(ov);
+        delete[] tm.row_coord;
+        delete[] tm.col_coord;
+        delete[] tm.data;
+    }
     // unknown id -> nullptr, like selectType's NULL
     if (pwm::selectTypeHip<double, I>(7, 1) != nullptr) {
         ++failures;

@@ -70,6 +70,10 @@ _SIGS = {
     "pwm_gen_uniform": [I64, C.c_uint64, VP],
     "pwm_gen_spd_powerlaw": [I64, C.c_uint64, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(I64)],
     "pwm_free": [VP],
+    "pwm_read_mm": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
+                    C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)],
+    "pwm_read_bin": [C.c_char_p, I64, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
+                     C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)],
     "pwm_batch_create": [C.c_int, C.POINTER(VP)],
     "pwm_batch_destroy": [VP],
     "pwm_batch_last_error": [VP],
@@ -91,6 +95,8 @@ _SIGS = {
     "pwm_server_matrix_resident": [VP, I32, C.POINTER(I32)],
     "pwm_server_submit": [VP, I32, I64, F64, C.c_uint64, C.POINTER(I64)],
     "pwm_server_step": [VP, C.POINTER(I32), C.POINTER(I32)],
+    "pwm_server_finished": [VP, VP, I32, C.POINTER(I32)],
+    "pwm_server_timing": [VP, C.POINTER(F64), C.POINTER(F64), C.POINTER(F64), C.POINTER(I64)],
     "pwm_server_result": [VP, I64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64), VP],
     "pwm_server_stats": [VP, C.POINTER(I64), C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(I64)],
 }
@@ -521,6 +527,16 @@ class Server:
         self._chk(lib().pwm_server_step(self.h, C.byref(a), C.byref(f)))
         return a.value, f.value
 
+    def finished(self):
+        """Ids of the jobs the last step retired."""
+        cnt = I32()
+        buf = np.zeros(256, np.int64)
+        self._chk(lib().pwm_server_finished(self.h, _ptr(buf), len(buf), C.byref(cnt)))
+        if cnt.value > len(buf):
+            buf = np.zeros(cnt.value, np.int64)
+            self._chk(lib().pwm_server_finished(self.h, _ptr(buf), len(buf), C.byref(cnt)))
+        return buf[:cnt.value].tolist()
+
     def result(self, job, n=None):
         """(lambda, ynorm, iterations, vector or None); raises while unfinished."""
         lam, yn, its = F64(), F64(), I64()
@@ -528,6 +544,12 @@ class Server:
         self._chk(lib().pwm_server_result(self.h, int(job), C.byref(lam), C.byref(yn), C.byref(its),
                                           None if vec is None else _ptr(vec)))
         return lam.value, yn.value, its.value, vec
+
+    def timing(self):
+        """Seconds in step's phases so far: admit (incl. uploads), run, retire."""
+        a, r, t, u = F64(), F64(), F64(), I64()
+        self._chk(lib().pwm_server_timing(self.h, C.byref(a), C.byref(r), C.byref(t), C.byref(u)))
+        return {"admit_s": a.value, "run_s": r.value, "retire_s": t.value, "uploads": u.value}
 
     def stats(self):
         u, r, q, a, f = I64(), I32(), I32(), I32(), I64()
@@ -547,6 +569,47 @@ def server_start_vector(n, seed):
         z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)) & M
         z = z ^ (z >> np.uint64(31))
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+
+
+class Triplet:
+    """COO matrix with the reference Triplet's members (Matrix/Triplet.hpp:26-45):
+    row_coord, col_coord, data, row_size, col_size, nnz -- what
+    SparseMatrixHip.loadFromTriplets reads."""
+
+    def __init__(self, row_size, col_size, row_coord, col_coord, data):
+        self.row_size, self.col_size = int(row_size), int(col_size)
+        self.row_coord = np.ascontiguousarray(row_coord, np.int32)
+        self.col_coord = np.ascontiguousarray(col_coord, np.int32)
+        self.data = np.ascontiguousarray(data, np.float64)
+
+    @property
+    def nnz(self):
+        return len(self.row_coord)
+
+
+def _take_coo(fn, *args):
+    nr, nc, nz = I64(), I64(), I64()
+    r, c, v = VP(), VP(), VP()
+    _check(fn(*args, C.byref(nr), C.byref(nc), C.byref(nz), C.byref(r), C.byref(c), C.byref(v)))
+    try:
+        n = nz.value
+        rows = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_int32)), (max(n, 1),))[:n].copy()
+        cols = np.ctypeslib.as_array(C.cast(c, C.POINTER(C.c_int32)), (max(n, 1),))[:n].copy()
+        vals = np.ctypeslib.as_array(C.cast(v, C.POINTER(C.c_double)), (max(n, 1),))[:n].copy()
+    finally:
+        for p in (r, c, v):
+            lib().pwm_free(p)
+    return Triplet(nr.value, nc.value, rows, cols, vals)
+
+
+def read_mm(path, has_data, symmetric, random_fill=False):
+    """Triplet::loadFromMM (Matrix/Triplet.hpp:75-154) into a Triplet."""
+    return _take_coo(lib().pwm_read_mm, os.fsencode(path), int(has_data), int(symmetric), int(random_fill))
+
+
+def read_bin(path, mat_size, symmetric, random_vals=False):
+    """Triplet::loadFromBin (Matrix/Triplet.hpp:164-216) into a Triplet."""
+    return _take_coo(lib().pwm_read_bin, os.fsencode(path), int(mat_size), int(symmetric), int(random_vals))
 
 
 def gen_uniform(n, seed=0):

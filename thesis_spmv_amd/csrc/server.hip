@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <vector>
@@ -70,9 +71,16 @@ struct pwm_server {
     std::map<int64_t, SrvJob> jobs;
     std::deque<int64_t> queue;
     std::vector<int64_t> slot_job;
+    std::vector<int64_t> last_finished;  // jobs retired by the last step
     int64_t next_job = 0, finished_total = 0;
     uint64_t clock = 0;
+    double t_admit = 0.0, t_run = 0.0, t_retire = 0.0;  // seconds in each phase of pwm_server_step
+    int64_t uploads = 0;
 };
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 static int sfail(pwm_server* s, int code, const std::string& m) {
     if (s)
@@ -304,19 +312,26 @@ static int make_resident(pwm_server* s, int mi) {
                 std::vector<uint16_t> b16;
                 const void* src;
                 size_t nb;
+                const double* v = M.val_p;
                 if (s->vtype == PWM_VALUES_F32) {
                     f32.resize(M.nnz);
-                    for (int64_t k = 0; k < M.nnz; ++k) f32[k] = (float)M.val_p[k];
+                    float* o = f32.data();
+                    pwm_impl::parallel_for(M.nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
+                        for (int64_t k = b; k < e; ++k) o[k] = (float)v[k];
+                    });
                     src = f32.data();
                     nb = M.nnz * 4;
                 } else {
                     b16.resize(M.nnz);
-                    for (int64_t k = 0; k < M.nnz; ++k) {
-                        const float f = (float)M.val_p[k];
-                        uint32_t u;
-                        memcpy(&u, &f, 4);
-                        b16[k] = (uint16_t)(u >> 16);
-                    }
+                    uint16_t* o = b16.data();
+                    pwm_impl::parallel_for(M.nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
+                        for (int64_t k = b; k < e; ++k) {
+                            const float f = (float)v[k];
+                            uint32_t u;
+                            memcpy(&u, &f, 4);
+                            o[k] = (uint16_t)(u >> 16);
+                        }
+                    });
                     src = b16.data();
                     nb = M.nnz * 2;
                 }
@@ -364,6 +379,7 @@ static int admit(pwm_server* s) {
         if (!M.resident) {
             if (!alloc_evicting(s, M.bytes, &M.off, J.mat)) break;
             const int rc = make_resident(s, J.mat);
+            ++s->uploads;
             if (rc != PWM_OK) {
                 s->arena.release(M.off, M.bytes);
                 return rc;
@@ -412,7 +428,10 @@ static int admit(pwm_server* s) {
 
 extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
     if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
+    const double t0 = now_s();
     int rc = admit(s);
+    const double t1 = now_s();
+    s->t_admit += t1 - t0;
     if (rc != PWM_OK) return rc;
     if (s->eng.active() == 0 && !s->queue.empty())
         return sfail(s, PWM_ERR_BUDGET,
@@ -423,9 +442,13 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
         std::string err;
         if ((rc = s->eng.run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
     }
+    const double t2 = now_s();
+    s->t_run += t2 - t1;
     hipError_t e = hipSuccess;
+    s->last_finished.clear();
     for (int slot : fin) {
         const int64_t id = s->slot_job[slot];
+        s->last_finished.push_back(id);
         SrvJob& J = s->jobs[id];
         const pwmk::EState& S = s->eng.state(slot);
         J.lambda = S.lambda;
@@ -441,6 +464,7 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
     }
     if (e == hipSuccess && !fin.empty()) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return shp(s, e, "job retirement");
+    s->t_retire += now_s() - t2;
     if (active) *active = s->eng.active();
     if (finished) *finished = (int32_t)fin.size();
     return PWM_OK;
@@ -488,6 +512,27 @@ extern "C" int pwm_server_stats(const pwm_server* s, int64_t* used_bytes, int32_
     if (queued) *queued = (int32_t)s->queue.size();
     if (running) *running = s->eng.active();
     if (finished) *finished = s->finished_total;
+    return PWM_OK;
+}
+
+// Seconds spent in pwm_server_step's phases so far (admission including
+// matrix uploads, running the engine, retiring jobs) and the uploads made.
+extern "C" int pwm_server_timing(const pwm_server* s, double* admit_s, double* run_s, double* retire_s,
+                                 int64_t* uploads) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_timing: NULL");
+    if (admit_s) *admit_s = s->t_admit;
+    if (run_s) *run_s = s->t_run;
+    if (retire_s) *retire_s = s->t_retire;
+    if (uploads) *uploads = s->uploads;
+    return PWM_OK;
+}
+
+// Ids of the jobs the last pwm_server_step retired (up to `cap`).
+extern "C" int pwm_server_finished(const pwm_server* s, int64_t* ids, int32_t cap, int32_t* count) {
+    if (!s || !count || (cap > 0 && !ids)) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_finished: bad argument");
+    const int32_t n = (int32_t)std::min<size_t>(s->last_finished.size(), cap > 0 ? (size_t)cap : 0);
+    for (int32_t i = 0; i < n; ++i) ids[i] = s->last_finished[i];
+    *count = (int32_t)s->last_finished.size();
     return PWM_OK;
 }
 

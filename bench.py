@@ -15,10 +15,14 @@ W untimed steps, then K timed steps between two device synchronisations;
 bytes of an FP64/int32 CRS SpMV are 12 Z + 4 (N+1) + 16 N (values and
 column indices once, row offsets once, x and y once).
 
-Both GPU kernels are measured (CRS vector kernel and merge path); the faster
-one gives `value`, its dominant kernel's HIP-event time gives
-`roofline.achieved`.  The CPU baseline is the oracle's restatement of the
-reference's CRSOMP::mv (OpenMP dynamic,512) timed on the host's cores.
+All four GPU formats are measured (CRS vector kernel, merge path, CSB,
+BlockCOH); the fastest gives `value`, its dominant kernel's HIP-event time
+gives `roofline.achieved`, and `roofline.traffic` comes from the committed
+rocprof PMC summary for that workload and format.  Also reported: the
+device power method per iteration, and (--values f32, default for stencil9)
+FP32-value runs with their error against FP64.  The CPU baseline is the
+oracle's restatements of the reference's CRSOMP::mv and CRS_Merge::mv timed
+on the host's cores.
 """
 import argparse
 import json
@@ -76,23 +80,39 @@ def make_workload(name):
     return n, rs, ci, dv, desc, time.time() - t0
 
 
-def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=8.0):
-    """Oracle restatement of CRSOMP::mv (Env_Implementations/CRSOMP.hpp:110-123)
-    on `threads` host cores, bounded to about `budget_s` seconds."""
+def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0):
+    """The oracle's restatements of CRSOMP::mv (Env_Implementations/CRSOMP.hpp:110-123,
+    OpenMP dynamic,512) and CRS_Merge::mv (SOA_Implementations/CRS_Merge.hpp:95-141,
+    P = threads) on `threads` host cores, each bounded to about `budget_s`
+    seconds of full SpMVs of the same matrix, plus the power iteration
+    (CRSOMP.hpp:135-157) per iteration.  `value` is the faster SpMV."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     A = oracle.CRSMatrix(n, n, rs, ci, dv)
-    oracle.crsomp_mv(A, x, threads)  # warm-up
-    times = []
-    t_start = time.time()
-    while len(times) < 30 and time.time() - t_start < budget_s:
-        t0 = time.perf_counter()
-        oracle.crsomp_mv(A, x, threads)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": round(bytes_per_mv / med / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} full SpMVs of the same matrix (median {med * 1e3:.1f} ms), "
-                      f"oracle CRSOMP::mv restatement, g++ -O2 -fopenmp, dynamic,512"}
+
+    def timed(fn):
+        fn()  # warm-up
+        times, t_start = [], time.time()
+        while len(times) < 30 and time.time() - t_start < budget_s:
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), len(times)
+
+    t_omp, k_omp = timed(lambda: oracle.crsomp_mv(A, x, threads))
+    t_mrg, k_mrg = timed(lambda: oracle.merge_mv(A, x, threads, threads))
+    # power iteration: calls of `it` iterations (100 for small matrices)
+    it = 100 if n <= (1 << 21) else 10
+    px, py = np.array(x, np.float64), np.zeros(n)
+    L = oracle.lib()
+    t_pow, k_pow = timed(lambda: L.orc_crsomp_power_iteration(threads, n, A.row_start, A.col_ind, A.data, px, py, it))
+    best = min(t_omp, t_mrg)
+    return {"value": round(bytes_per_mv / best / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"median of {k_omp} CRSOMP::mv and {k_mrg} CRS_Merge::mv (P={threads}) full SpMVs of the same "
+                      f"matrix: {t_omp * 1e3:.2f} ms and {t_mrg * 1e3:.2f} ms (oracle restatements, g++ -O2 -fopenmp); "
+                      f"power iteration {t_pow / it * 1e3:.2f} ms/iteration ({k_pow} calls of {it})",
+            "crsomp_ms": round(t_omp * 1e3, 3), "crs_merge_ms": round(t_mrg * 1e3, 3),
+            "power_ms_per_iteration": round(t_pow / it * 1e3, 3)}
 
 
 def bench_batch(args):
@@ -340,6 +360,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
     ap.add_argument("--formats", default="crs_vector,merge_path,csb,blockcoh")
+    ap.add_argument("--values", default=None,
+                    help="f64 and/or f32 SpMV passes (default f64; f64,f32 for stencil9)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes: run the library's streaming copy kernel once first (profiler byte-counter reference)")
     ap.add_argument("--batch-instances", type=int, default=512)
@@ -372,6 +394,8 @@ def main():
         pw._check(pw.lib().pwm_device_copy(dst.ptr, src.ptr, nb))
         print(f"# calibration copy: {nb} bytes read and written once", file=sys.stderr, flush=True)
         del src, dst
+    if args.values is None:
+        args.values = "f64,f32" if args.workload == "stencil9" else "f64"
     n, rs, ci, dv, desc, gen_s = make_workload(args.workload)
     z = int(rs[-1])
     x_host = pw.gen_uniform(n, 0)
@@ -407,7 +431,7 @@ def main():
         results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
                          "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush,
                          "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf, "load_s": t_load,
-                         "y_checksum": float(np.sum(np.abs(yv)))}
+                         "y_checksum": float(np.sum(np.abs(yv))), "y": yv}
         print(f"# {name}: wall {wall_ms * 1e3:.1f} us/step, events {ms_ev * 1e3:.1f} us, dominant kernel "
               f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us",
               file=sys.stderr, flush=True)
@@ -415,6 +439,51 @@ def main():
 
     best = max(results, key=lambda k: results[k]["GBs"])
     r = results[best]
+
+    # FP32 values and vectors, FP32 row sums: rate and error against FP64
+    f32 = {}
+    if "f32" in args.values.split(","):
+        x32 = pw.DeviceVector.from_array(x_host, np.float32)
+        y32 = pw.DeviceVector(n, np.float32)
+        bytes32 = algorithmic_bytes(n, z, 4, 4)
+        for name, cls, opts in formats:
+            if name not in results:
+                continue
+            m = cls(-1, "f32", **opts)
+            m.loadCRS(n, n, rs, ci, dv)
+            for _ in range(args.warmup):
+                m.time_mv(x32, y32, 1)
+            pw.synchronize()
+            t0 = time.perf_counter()
+            ms_ev, kms = m.time_mv(x32, y32, args.steps)
+            pw.synchronize()
+            wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            y64 = results[name]["y"]
+            err = float(np.max(np.abs(y32.download().astype(np.float64) - y64)) / np.max(np.abs(y64)))
+            f32[name] = {"GBs_wall": round(bytes32 / (wall_ms * 1e-3) / 1e9, 1), "us_per_step": round(wall_ms * 1e3, 2),
+                         "dominant_kernel_us": round(kms * 1e3, 2),
+                         "kernel_GBs": round(bytes32 / (kms * 1e-3) / 1e9, 1), "y_rel_err_vs_f64": err}
+            print(f"# {name} f32: wall {wall_ms * 1e3:.1f} us/step, kernel {kms * 1e3:.1f} us "
+                  f"({bytes32 / (kms * 1e-3) / 1e9:.0f} GB/s of {bytes32 / 1e6:.1f} MB), y rel.err vs FP64 {err:.2e}",
+                  file=sys.stderr, flush=True)
+            del m
+        del x32, y32
+
+    # power method on the device with the best format: ms per iteration
+    cls, opts = [(c, o) for nm, c, o in formats if nm == best][0]
+    m = cls(-1, "f64", **opts)
+    m.loadCRS(n, n, rs, ci, dv)
+    xp = pw.DeviceVector.from_array(x_host)
+    m.power_method_device(xp, y, 3, -1.0)
+    xp.upload(x_host)
+    pw.synchronize()
+    t0 = time.perf_counter()
+    m.power_method_device(xp, y, 100, -1.0)
+    pw.synchronize()
+    power_ms = (time.perf_counter() - t0) * 1e3 / 100
+    print(f"# power method ({best}, FP64, device vectors): {power_ms * 1e3:.1f} us/iteration", file=sys.stderr,
+          flush=True)
+    del m, xp
     achieved = bytes_mv / (r["kernel_ms"] * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{best}.json")
@@ -447,7 +516,10 @@ def main():
                         "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
                         "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"]}
                     for k, v in results.items()},
+        "power_method_us_per_iteration": round(power_ms * 1e3, 2),
     }
+    if f32:
+        out["f32"] = f32
     if not args.no_cpu:
         try:
             cores = len(os.sched_getaffinity(0))

@@ -115,9 +115,9 @@ int pwm_info(const pwm_matrix* h, pwm_info_t* out);
  * 0 = automatic), "long_row" (CRS: rows longer than this use the long-row
  * path), "items_per_thread" (merge path), "block_bits" (CSB beta override),
  * "csb_groups" (CSB block-column groups pinned to XCDs, 0 = time 1/8/16 at
- * load and keep the fastest),
- * "autotune" (1 = time the candidate kernels on the next call and keep the
- * fastest).  Must be set before the matrix is loaded unless noted. */
+ * load and keep the fastest), "crs_kernel" (CRS: 0 = time the vector kernel
+ * and the row-aligned tiles kernel at load and keep the faster, 1 = vector,
+ * 2 = tiles).  Must be set before the matrix is loaded unless noted. */
 int pwm_set_option(pwm_matrix* h, const char* key, int64_t value);
 
 /* ---- device-resident additions ----------------------------------------

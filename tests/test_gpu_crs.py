@@ -292,3 +292,82 @@ def test_cpp_dropin_classes():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+# ---------------------------------------------------------------- CRS kernels
+# The CRS format runs either the vector kernel (lanes per row) or the
+# row-aligned tiles kernel (tiles of whole short rows, long rows split with
+# carries); by default it times both at load.  Each is checked on its own.
+
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("idx", range(24))
+def test_golden_each_crs_kernel(goldens, kernel, idx):
+    case = goldens[idx]
+    A, t = load_case_matrix(case)
+    m = pw.CRSHip(-1, "f64", crs_kernel=kernel)
+    if case["matrix"]["kind"] == "poisson":
+        m.generatePoissonMatrix(case["matrix"]["m"], case["matrix"]["n"], 1)
+    else:
+        m.loadCRS(A.nor, A.noc, A.row_start, A.col_ind, A.data)
+    x, y = np.ones(A.nor), np.zeros(A.nor)
+    if case["op"] == "mv":
+        m.mv(x, y)
+        got = y
+    else:
+        m.powerIteration(x, y, case["iterations"])
+        got = x if case["checks"] == "x" else y
+    ok = boost_close(got, case["expected"], case["tolerance"])
+    assert ok.all(), (case["name"], kernel, np.flatnonzero(~ok)[:5])
+
+
+def _long_row_matrix(rng, n=16000):
+    lens = rng.integers(0, 9, n)
+    lens[[3, 1000, 2500, n - 1]] = [9000, 30, 4100, 12000]  # rows longer than a tile, one at the end
+    lens[[10, 11, 12]] = 0
+    lens = np.minimum(lens, n)
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(L), replace=False)) for L in lens]).astype(np.int32)
+    return oracle.CRSMatrix(n, n, rs, ci, rng.standard_normal(int(rs[-1])))
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("values,tol", [("f64", 1e-13), ("f32", 5e-5), ("bf16", 5e-5)])
+def test_crs_kernels_long_rows(kernel, values, tol):
+    rng = np.random.default_rng(17)
+    A = _long_row_matrix(rng)
+    if values == "bf16":
+        A = oracle.CRSMatrix(A.nor, A.noc, A.row_start, A.col_ind, np.round(A.data * 8) / 8)
+    m = _load(pw.CRSHip, A, values=values, crs_kernel=kernel)
+    info = m.info()
+    assert (info["group_size"] == 0) == (kernel == 2)
+    x = rng.uniform(-1, 1, A.nor)
+    if values != "f64":
+        x = x.astype(np.float32).astype(np.float64)
+    y = np.zeros(A.nor)
+    m.mv(x, y)
+    assert _relerr(y, oracle.crs_mv(A, x)) <= tol
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_crs_kernels_power_method(kernel):
+    rng = np.random.default_rng(5)
+    A = _long_row_matrix(rng, 4000)
+    A = oracle.CRSMatrix(A.nor, A.noc, A.row_start, A.col_ind, np.abs(A.data) + 0.1)
+    m = _load(pw.CRSHip, A, crs_kernel=kernel)
+    x0 = rng.uniform(0, 1, A.nor)
+    xd, yd = pw.DeviceVector.from_array(x0), pw.DeviceVector(A.nor)
+    lam, yn, its = m.power_method_device(xd, yd, 40, -1.0)
+    xr, yr = oracle.crs_power_iteration(A, x0, 39)
+    v = xr if 39 % 2 == 0 else yr
+    ref = float(v @ oracle.crs_mv(A, v) / (v @ v))
+    assert its == 40 and abs(lam - ref) <= 1e-10 * abs(ref)
+    x, y = x0.copy(), np.zeros(A.nor)
+    m.powerIteration(x, y, 7)
+    xr, yr = oracle.crs_power_iteration(A, x0, 7)
+    assert _relerr(y, yr) <= 1e-12 and _relerr(x, xr) <= 1e-12
+
+
+def test_crs_default_times_both_kernels():
+    A = oracle.poisson_crs(300, 300)
+    m = _load(pw.CRSHip, A)
+    assert m.info()["group_size"] in (0, 2, 4)  # whichever kernel was faster

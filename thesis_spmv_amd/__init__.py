@@ -280,7 +280,9 @@ class SparseMatrixHip:
 
 
 class CRSHip(SparseMatrixHip):
-    """CRS vector kernel (beside CRS / CRSOMP, Matrix/CRS.hpp, Env_Implementations/CRSOMP.hpp)."""
+    """CRS (beside CRS / CRSOMP, Matrix/CRS.hpp, Env_Implementations/CRSOMP.hpp): the
+    vector kernel or the row-aligned tiles kernel, whichever timed faster at
+    load (option crs_kernel: 1 vector, 2 tiles)."""
     FORMAT = FORMAT_CRS
 
 

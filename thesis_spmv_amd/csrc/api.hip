@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "api_internal.hpp"
+#include "engine.hpp"
 
 using namespace pwm_impl;
 using pwmk::CrsDev;
@@ -74,6 +75,91 @@ static int pow2_ceil(int64_t v) {
     return g;
 }
 
+// CRS tiles kernel: the engine's row-aligned tile plan, kept as merge-path
+// splits (split_a/split_b = each tile's first row / nonzero) for the fix-up.
+static int plan_crs_tiles(pwm_matrix* h) {
+    const int ipt = pwmk::tiles_items_per_thread();
+    const int32_t nt = pwmk::engine_tiles_for(h->nor, h->nnz, ipt);
+    hipError_t e;
+    if ((e = h->tiles.alloc((size_t)nt * sizeof(pwmk::ETile))) != hipSuccess ||
+        (e = h->split_a.alloc((nt + 1) * 8)) != hipSuccess || (e = h->split_b.alloc((nt + 1) * 8)) != hipSuccess ||
+        (e = h->carry_row.alloc(nt * 4)) != hipSuccess || (e = h->carry_val.alloc(nt * 8)) != hipSuccess)
+        return hip_fail(h, e, "tiles plan allocation");
+    e = pwmk::engine_plan_tiles(h->rs.as<int32_t>(), h->nor, h->nnz, ipt, h->tiles.as<pwmk::ETile>(), h->stream);
+    std::vector<pwmk::ETile> t(nt);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t.data(), h->tiles.p, (size_t)nt * sizeof(pwmk::ETile), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "tiles plan");
+    std::vector<int64_t> sa(nt + 1), sb(nt + 1);
+    for (int32_t i = 0; i < nt; ++i) {
+        sa[i] = t[i].a0;
+        sb[i] = t[i].b0;
+    }
+    sa[nt] = h->nor;
+    sb[nt] = h->nnz;
+    if ((e = hipMemcpy(h->split_a.p, sa.data(), (nt + 1) * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(h->split_b.p, sb.data(), (nt + 1) * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "tiles plan upload");
+    h->mp = pwmk::MergePlan{};
+    h->mp.P = nt;
+    h->mp.ipt = ipt;
+    h->mp.items_per_tile = 0;
+    h->mp.split_a = h->split_a.as<int64_t>();
+    h->mp.split_b = h->split_b.as<int64_t>();
+    h->mp.carry_row = h->carry_row.as<int32_t>();
+    h->mp.carry_val = h->carry_val.p;
+    return PWM_OK;
+}
+
+static hipError_t crs_kernel_launch(pwm_matrix* h, int which, const void* x, void* y, const PowerArgs* pw) {
+    const CrsDev A = crs_of(h);
+    hipError_t e;
+    if (which == 2) {
+        e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, pw, h->stream);
+        if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, pw, h->stream);
+        return e;
+    }
+    e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, pw, h->stream);
+    if (e == hipSuccess && (pw || h->lr.n_rows > 0)) e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, pw, h->stream);
+    return e;
+}
+
+// Time both CRS kernels on x = 1 (2 warm-up + 5 timed SpMVs each) and keep
+// the faster.
+static int pick_crs_kernel(pwm_matrix* h) {
+    const size_t vs = vector_size(h->vtype);
+    DevBuf x, y;
+    hipError_t e;
+    if ((e = x.alloc(std::max<int64_t>(h->noc, 1) * vs)) != hipSuccess ||
+        (e = y.alloc(std::max<int64_t>(h->nor, 1) * vs)) != hipSuccess)
+        return hip_fail(h, e, "CRS kernel choice vectors");
+    e = pwmk::launch_fill(x.p, h->noc, h->vtype == PWM_VALUES_F64 ? 1 : 0, 1.0, h->stream);
+    float best = 0.0f;
+    int pick = 2;
+    for (int which = 2; which >= 1 && e == hipSuccess; --which) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x.p, y.p, nullptr);
+        (void)hipEventRecord(a, h->stream);
+        for (int i = 0; i < 5 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x.p, y.p, nullptr);
+        (void)hipEventRecord(b, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        if (which == 2 || ms < best) {
+            best = ms;
+            pick = which;
+        }
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "CRS kernel choice");
+    h->crs_used = pick;
+    return PWM_OK;
+}
+
 // CRS vector kernel: lanes per row, grid size and the long-row chunk plan.
 int plan_crs(pwm_matrix* h) {
     const double avg = h->nor > 0 ? (double)h->nnz / (double)h->nor : 0.0;
@@ -125,6 +211,12 @@ int plan_crs(pwm_matrix* h) {
         h->lr.chunk_begin = h->lr_begin.as<int32_t>();
         h->lr.chunk_partial = h->lr_partial.p;
     }
+    h->crs_used = 1;
+    if (h->crs_kernel == 1) return PWM_OK;
+    int rc = plan_crs_tiles(h);
+    if (rc != PWM_OK) return rc;
+    h->crs_used = 2;
+    if (h->crs_kernel == 0) return pick_crs_kernel(h);
     return PWM_OK;
 }
 
@@ -164,10 +256,8 @@ int spmv_launch(pwm_matrix* h, const void* x, void* y, const PowerArgs* pw, int*
     int np = 0;
     switch (h->format) {
         case PWM_FORMAT_CRS:
-            e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, pw, h->stream);
-            if (e == hipSuccess && (pw || h->lr.n_rows > 0))
-                e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, pw, h->stream);
-            np = h->vec_blocks;
+            e = crs_kernel_launch(h, h->crs_used, x, y, pw);
+            np = h->crs_used == 2 ? (int)h->mp.P : h->vec_blocks;
             break;
         case PWM_FORMAT_CRS_MERGE:
             e = pwmk::launch_merge_spmv(A, h->mp, x, y, pw, h->stream);
@@ -192,8 +282,9 @@ int spmv_launch(pwm_matrix* h, const void* x, void* y, const PowerArgs* pw, int*
 
 static int64_t partials_needed(const pwm_matrix* h) {
     switch (h->format) {
-        case PWM_FORMAT_CRS: return h->vec_blocks;
-        case PWM_FORMAT_CRS_MERGE: return h->mp.P;
+        case PWM_FORMAT_CRS:
+            return std::max<int64_t>(h->vec_blocks, h->tiles.p ? h->mp.P + pwmk::merge_fix_blocks(h->mp.P) : 0);
+        case PWM_FORMAT_CRS_MERGE: return h->mp.P + pwmk::merge_fix_blocks(h->mp.P);
         case PWM_FORMAT_CSB: return csb_partials(h);
         case PWM_FORMAT_BLOCKCOH: return coh_partials(h);
     }
@@ -265,12 +356,13 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_groups") {
         if (value < 0 || value > 64) return fail(h, PWM_ERR_INVALID, "csb_groups must be 0..64");
         h->csb_groups = (int32_t)value;
-    } else if (k == "autotune") {
-        h->autotune = value != 0;
+    } else if (k == "crs_kernel") {
+        if (value < 0 || value > 2) return fail(h, PWM_ERR_INVALID, "crs_kernel must be 0 (time both), 1 (vector) or 2 (tiles)");
+        h->crs_kernel = (int)value;
     } else {
         return fail(h, PWM_ERR_UNSUPPORTED, "unknown option " + k);
     }
-    if (h->loaded && (k == "group_size" || k == "vec_blocks" || k == "long_row" || k == "long_chunk") &&
+    if (h->loaded && (k == "group_size" || k == "vec_blocks" || k == "long_row" || k == "long_chunk" || k == "crs_kernel") &&
         h->format == PWM_FORMAT_CRS)
         return plan_crs(h);
     if (h->loaded && k == "items_per_thread" && h->format == PWM_FORMAT_CRS_MERGE) return plan_merge(h);
@@ -316,6 +408,8 @@ static void reset_matrix(pwm_matrix* h) {
     h->h_rs.clear();
     h->lr = pwmk::LongRows{};
     h->mp = pwmk::MergePlan{};
+    h->tiles.release();
+    h->crs_used = 1;
     h->csb.clear();
     h->coh.clear();
 }
@@ -461,16 +555,20 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
     out->nnz = h->nnz;
     out->format = h->format;
     out->value_type = h->vtype;
-    out->group_size = h->format == PWM_FORMAT_CSB ? h->csb.groups : h->group;
+    // CRS: the lanes per row of the vector kernel, 0 when the tiles kernel runs
+    out->group_size = h->format == PWM_FORMAT_CSB ? h->csb.groups
+                      : (h->format == PWM_FORMAT_CRS && h->crs_used == 2) ? 0
+                                                                           : h->group;
     out->long_rows = h->lr.n_rows;
-    out->threads = h->format == PWM_FORMAT_CRS_MERGE ? (int32_t)h->mp.P
+    out->threads = (h->format == PWM_FORMAT_CRS_MERGE || (h->format == PWM_FORMAT_CRS && h->crs_used == 2))
+                       ? (int32_t)h->mp.P
                    : h->format == PWM_FORMAT_BLOCKCOH ? (int32_t)h->coh.bands.size()
                    : h->format == PWM_FORMAT_CSB      ? (int32_t)h->csb.n_tasks
                                                       : h->vec_blocks;
     out->block_bits = h->format == PWM_FORMAT_CSB ? h->csb.bits : 0;
     out->balanced = h->format == PWM_FORMAT_CSB ? h->csb.balanced : 0;
     const pwm_impl::DevBuf* bufs[] = {&h->rs, &h->ci, &h->val, &h->lr_chunks, &h->lr_rows, &h->lr_begin,
-                                      &h->lr_partial, &h->split_a, &h->split_b, &h->carry_row, &h->carry_val,
+                                      &h->lr_partial, &h->split_a, &h->split_b, &h->carry_row, &h->carry_val, &h->tiles,
                                       &h->csb.ind, &h->csb.data, &h->csb.blk_ptr, &h->csb.work,
                                       &h->coh.row_jump, &h->coh.col_jump, &h->coh.data, &h->coh.row_jump_block,
                                       &h->coh.col_jump_block, &h->coh.block_nnz, &h->coh.block_coords,
@@ -738,6 +836,12 @@ extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int6
         // dominant kernel first, then the rest of the sequence
         switch (h->format) {
             case PWM_FORMAT_CRS:
+                if (h->crs_used == 2) {
+                    e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, nullptr, h->stream);
+                    (void)hipEventRecord(ev[3 * r + 1], h->stream);
+                    if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+                    break;
+                }
                 e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, nullptr, h->stream);
                 (void)hipEventRecord(ev[3 * r + 1], h->stream);
                 if (e == hipSuccess && h->lr.n_rows > 0)

@@ -137,6 +137,12 @@ struct pwm_matrix {
     int chunk = 4096;
     pwmk::LongRows lr;
     pwm_impl::DevBuf lr_chunks, lr_rows, lr_begin, lr_partial;
+    // CRS kernel: 0 = time the vector and tiles kernels at load and keep the
+    // faster, 1 = vector, 2 = tiles (row-aligned tiles, carries through the
+    // merge fix-up; the tiles plan lives in mp / split_* / carry_*)
+    int crs_kernel = 0;
+    int crs_used = 1;
+    pwm_impl::DevBuf tiles;
 
     // merge plan
     int ipt = 8;
@@ -157,7 +163,6 @@ struct pwm_matrix {
     // timing flush buffer
     pwm_impl::DevBuf flush;
 
-    bool autotune = false;
 };
 
 namespace pwm_impl {

@@ -149,7 +149,7 @@ __device__ __forceinline__ void eng_tile(const EWork& W, const EView<typename Ar
     double yy = 0.0, xy = 0.0, xx = 0.0;
     Acc c = 0;
     if (W.stream)
-        stream_compute<VT, kEngBS, ITEMS>(v.y, xr, W.a0, W.a1, W.b0, v.scale, sm, yy, xy, xx);
+        stream_compute<VT, kEngBS, ITEMS, true>(v.y, xr, W.a0, W.a1, W.b0, v.scale, sm, yy, xy, xx);
     else
         c = merge_compute<VT, kEngBS, ITEMS, true>(W.rs, v.x, v.y, W.a0, W.a1, W.b0, W.b1, v.scale, sm, yy, xy, xx);
     if (threadIdx.x == kEngBS - 1) G((Acc*)v.carry)[W.tile] = c;
@@ -601,6 +601,11 @@ int Engine::build_lists(std::string& err) {
 int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
     finished.clear();
     if (n_active_ == 0) return PWM_OK;
+    // no more steps than any running job can still take (max_it bounds it)
+    int64_t most = 1;
+    for (int s = 0; s < max_slots_; ++s)
+        if (running_[s]) most = std::max<int64_t>(most, h_jobs_[s].max_it - h_state_[s].it);
+    if (steps > most) steps = (int)most;
     if (steps < 1) steps = 1;
     hipError_t e = hipSuccess;
     // new jobs: descriptor and step-0 state

@@ -96,15 +96,19 @@ __global__ void k_merge_fixup(int64_t nrows, MergePlan mp, typename Arith<VT>::X
     y[r] = (X)((Acc)y[r] + s);
 }
 
-constexpr int kFixBS = 1024;
+constexpr int kFixBS = 256;
 
-// Power mode: carry fix-up plus the FP64 reduction of the tiles' partials,
-// in one workgroup (fixed order), then step k's record and k+1.
+// Power mode, pass 1: workgroup b owns tiles [b*per, (b+1)*per): it adds the
+// carries of the runs whose head tile it owns (a run of tiles p..q-1 carrying
+// into row r is summed in tile order and added to y[r] once, by the thread of
+// the run's head, which follows the run past its own range) and sums, in a
+// fixed order, its tiles' FP64 partials plus the completed rows'
+// contributions, into the second-level slot after the tile partials.
 template <typename VT>
-__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(CrsDev A, MergePlan mp, PowerArgs pw) {
+__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power1(CrsDev A, MergePlan mp, PowerArgs pw, int64_t per) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
-    __shared__ double red[16];
+    __shared__ double red[48];
     if (pw.ctrl[1]) return;
     const int k = pw.ctrl[0];
     const X* __restrict__ x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
@@ -112,8 +116,10 @@ __global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(CrsDev A, MergePla
     const double scale = step_scale(pw, k);
     const Acc* cv = (const Acc*)mp.carry_val;
     const int64_t nrows = A.nrows;
+    const int64_t p0 = (int64_t)blockIdx.x * per;
+    const int64_t p1 = p0 + per < mp.P ? p0 + per : mp.P;
     double yy = 0.0, xy = 0.0, xx = 0.0;
-    for (int64_t p = threadIdx.x; p < mp.P; p += kFixBS) {
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += kFixBS) {
         yy += pw.partials[3 * p + 0];
         xy += pw.partials[3 * p + 1];
         xx += pw.partials[3 * p + 2];
@@ -133,9 +139,29 @@ __global__ __launch_bounds__(kFixBS) void k_merge_fixup_power(CrsDev A, MergePla
         xy += xd * yd;
         xx += xd * xd;
     }
-    const double a = block_sum<kFixBS>(yy, red);
-    const double b = block_sum<kFixBS>(xy, red);
-    const double c = block_sum<kFixBS>(xx, red);
+    block_sum3<kFixBS>(yy, xy, xx, red);
+    if (threadIdx.x == 0) {
+        double* q2 = pw.partials + 3 * mp.P + 3 * (int64_t)blockIdx.x;
+        q2[0] = yy;
+        q2[1] = xy;
+        q2[2] = xx;
+    }
+}
+
+// Power mode, pass 2 (one workgroup): the second-level sums in workgroup
+// order, then step k's record and k+1.
+__global__ __launch_bounds__(kFixBS) void k_merge_fixup_power2(MergePlan mp, PowerArgs pw, int nblk) {
+    __shared__ double red[48];
+    if (pw.ctrl[1]) return;
+    const int k = pw.ctrl[0];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += kFixBS) {
+        const double* q2 = pw.partials + 3 * mp.P + 3 * (int64_t)i;
+        a += q2[0];
+        b += q2[1];
+        c += q2[2];
+    }
+    block_sum3<kFixBS>(a, b, c, red);
     if (threadIdx.x == 0) {
         const double lam = b / c;
         pw.rec[4 * k + 0] = a;
@@ -198,9 +224,12 @@ hipError_t launch_merge_spmv(const CrsDev& A, const MergePlan& mp, const void* x
 template <typename VT>
 static hipError_t launch_fix_t(const CrsDev& A, const MergePlan& mp, void* y, const PowerArgs* pw, hipStream_t s) {
     using X = typename Arith<VT>::X;
-    if (pw)
-        hipLaunchKernelGGL((k_merge_fixup_power<VT>), dim3(1), dim3(kFixBS), 0, s, A, mp, *pw);
-    else
+    if (pw) {
+        const int nblk = merge_fix_blocks(mp.P);
+        const int64_t per = (mp.P + nblk - 1) / nblk;
+        hipLaunchKernelGGL((k_merge_fixup_power1<VT>), dim3(nblk), dim3(kFixBS), 0, s, A, mp, *pw, per);
+        hipLaunchKernelGGL(k_merge_fixup_power2, dim3(1), dim3(kFixBS), 0, s, mp, *pw, nblk);
+    } else
         hipLaunchKernelGGL((k_merge_fixup<VT>), dim3((unsigned)((mp.P + 255) / 256)), dim3(256), 0, s, A.nrows, mp,
                            (X*)y);
     return hipGetLastError();

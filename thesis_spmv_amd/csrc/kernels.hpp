@@ -60,6 +60,20 @@ hipError_t launch_merge_splits(const CrsDev& A, const MergePlan& mp, hipStream_t
 hipError_t launch_merge_spmv(const CrsDev& A, const MergePlan& mp, const void* x, void* y, const PowerArgs* pw,
                              hipStream_t s);
 hipError_t launch_merge_fixup(const CrsDev& A, const MergePlan& mp, void* y, const PowerArgs* pw, hipStream_t s);
+// CRS over row-aligned tiles (k_tiles.hip): the tiles are the iteration
+// engine's plan (struct ETile, engine.hpp) with tiles_items_per_thread()
+// items per thread; carries go through launch_merge_fixup with mp.P = tiles.
+struct ETile;
+int tiles_items_per_thread();
+hipError_t launch_tiles_spmv(const CrsDev& A, const ETile* tiles, const MergePlan& mp, const void* x, void* y,
+                             const PowerArgs* pw, hipStream_t s);
+
+// Power-mode carry fix-up runs in this many workgroups (each sums its share
+// of the tile partials into a second-level slot after the P tile slots).
+inline int merge_fix_blocks(int64_t P) {
+    const int64_t b = (P + 255) / 256;
+    return (int)(b < 1 ? 1 : (b > 256 ? 256 : b));
+}
 
 // ---- utilities (k_util.hip) -------------------------------------------------
 // Util/Poisson.hpp:32-83 on the device, one thread per row.

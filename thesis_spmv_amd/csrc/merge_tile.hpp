@@ -188,9 +188,10 @@ __device__ __forceinline__ void tile_stage(const TileRegs<VT, ITEMS>& R, const i
     }
 }
 
-// Stream compute (after the barrier): rows [a0,a1) complete, b0 = rs[a0];
-// xr from stream_xrows.  Reads only LDS (no global load waits here).
-template <typename VT, int BS, int ITEMS>
+// Stream compute (after the barrier): rows [a0,a1) complete, b0 = rs[a0].
+// POWER: outputs scaled and norms accumulated, with xr from stream_xrows.
+// Reads only LDS (no global load waits here).
+template <typename VT, int BS, int ITEMS, bool POWER>
 __device__ __forceinline__ void stream_compute(typename Arith<VT>::X* __restrict__ y, const typename Arith<VT>::X* xr,
                                                int64_t a0, int64_t a1, int64_t b0, double scale,
                                                const MergeTileSmem<typename Arith<VT>::A, BS, ITEMS>& sm, double& yy,
@@ -204,17 +205,21 @@ __device__ __forceinline__ void stream_compute(typename Arith<VT>::X* __restrict
     for (int u = 0; u < ITEMS; ++u) {
         const int r = tid + u * BS;
         if (u < nu && r < na) {
-            const double xd = (double)xr[u] * scale;
             const int beg = r == 0 ? 0 : (int)(sm.rowend[r - 1] - b0);
             const int end = (int)(sm.rowend[r] - b0);
             Acc s = 0;
             for (int k = beg; k < end; ++k) s += sm.prod[k];
-            const X out = (X)((double)s * scale);
-            G(y)[a0 + r] = out;
-            const double yd = (double)out;
-            yy += yd * yd;
-            xy += xd * yd;
-            xx += xd * xd;
+            if constexpr (POWER) {
+                const double xd = (double)xr[u] * scale;
+                const X out = (X)((double)s * scale);
+                G(y)[a0 + r] = out;
+                const double yd = (double)out;
+                yy += yd * yd;
+                xy += xd * yd;
+                xx += xd * xd;
+            } else {
+                G(y)[a0 + r] = (X)s;
+            }
         }
     }
 }

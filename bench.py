@@ -359,7 +359,7 @@ def main():
     ap.add_argument("--workload", default="powerlaw")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
-    ap.add_argument("--formats", default="crs_vector,merge_path,csb,blockcoh")
+    ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh")
     ap.add_argument("--values", default=None,
                     help="f64 and/or f32 SpMV passes (default f64; f64,f32 for stencil9)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -379,7 +379,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU row sharding is a design in DESIGN.md, not built", file=sys.stderr)
+              "sharding the rows over several GPUs is designed, not built", file=sys.stderr)
         sys.exit(2)
 
     import thesis_spmv_amd as pw
@@ -407,7 +407,7 @@ def main():
     y = pw.DeviceVector(n)
     results = {}
     wanted = set(args.formats.split(","))
-    formats = (("crs_vector", pw.CRSHip, {}), ("merge_path", pw.CRSMergeHip, {}),
+    formats = (("crs", pw.CRSHip, {}), ("merge_path", pw.CRSMergeHip, {}),
                ("csb", pw.CSBHip, {"block_bits": 14} if args.workload == "stencil9" else {}),
                ("blockcoh", pw.BlockCOHHip, {}))
     for name, cls, opts in formats:
@@ -513,6 +513,7 @@ def main():
                         "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
                                                        else round(v["kernel_ms_flushed"] * 1e3, 2)),
                         "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
+                        **({"crs_kernel": "tiles" if v["info"]["group_size"] == 0 else "vector"} if k == "crs" else {}),
                         "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
                         "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"]}
                     for k, v in results.items()},

@@ -106,7 +106,7 @@ class SparseMatrixHipBase : public pwm::SparseMatrix<T, int_type> {
         }
     }
 
-    // names used by the north-star description of the same operations
+    // the same operations under the names SpMV users often call them
     void mult(const T* x, T* y) { mv(x, y); }
     void powerMethod(T* x, T* y, const int_type it) { powerIteration(x, y, it); }
 };

@@ -125,50 +125,32 @@ static hipError_t crs_kernel_launch(pwm_matrix* h, int which, const void* x, voi
     return e;
 }
 
-// Time both CRS kernels on x = 1 (2 warm-up + 5 timed SpMVs each) and keep
-// the faster.
-static int pick_crs_kernel(pwm_matrix* h) {
-    const size_t vs = vector_size(h->vtype);
-    DevBuf x, y;
-    hipError_t e;
-    if ((e = x.alloc(std::max<int64_t>(h->noc, 1) * vs)) != hipSuccess ||
-        (e = y.alloc(std::max<int64_t>(h->nor, 1) * vs)) != hipSuccess)
-        return hip_fail(h, e, "CRS kernel choice vectors");
-    e = pwmk::launch_fill(x.p, h->noc, h->vtype == PWM_VALUES_F64 ? 1 : 0, 1.0, h->stream);
-    float best = 0.0f;
-    int pick = 2;
-    for (int which = 2; which >= 1 && e == hipSuccess; --which) {
-        hipEvent_t a, b;
-        (void)hipEventCreate(&a);
-        (void)hipEventCreate(&b);
-        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x.p, y.p, nullptr);
-        (void)hipEventRecord(a, h->stream);
-        for (int i = 0; i < 5 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x.p, y.p, nullptr);
-        (void)hipEventRecord(b, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        if (which == 2 || ms < best) {
-            best = ms;
-            pick = which;
-        }
-    }
-    if (e != hipSuccess) return hip_fail(h, e, "CRS kernel choice");
-    h->crs_used = pick;
-    return PWM_OK;
+// Average time of 5 SpMVs (after 2 warm-up ones) of CRS kernel `which` on
+// the device vectors x, y.
+static hipError_t time_crs_kernel(pwm_matrix* h, int which, const void* x, void* y, float* ms) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < 2 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x, y, nullptr);
+    (void)hipEventRecord(a, h->stream);
+    for (int i = 0; i < 5 && e == hipSuccess; ++i) e = crs_kernel_launch(h, which, x, y, nullptr);
+    (void)hipEventRecord(b, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    *ms = 0.0f;
+    (void)hipEventElapsedTime(ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return e;
 }
 
-// CRS vector kernel: lanes per row, grid size and the long-row chunk plan.
-int plan_crs(pwm_matrix* h) {
-    const double avg = h->nor > 0 ? (double)h->nnz / (double)h->nor : 0.0;
-    int G = h->group;
-    if (G <= 0) G = std::min(32, std::max(2, pow2_ceil((int64_t)ceil(avg / 2.0))));
+// CRS vector kernel with G lanes per row: grid size and the long-row chunk
+// plan (rows longer than max(1024, 128 G) are cut into chunks).
+static int plan_crs_vector(pwm_matrix* h, int G) {
     h->group = G;
     const int64_t rpw = 64 / G;
     const int64_t units = (h->nor + rpw - 1) / rpw;
-    int64_t blocks = h->vec_blocks > 0 ? h->vec_blocks : std::min<int64_t>((units + 3) / 4, 2048);
+    int64_t blocks = h->vec_blocks_opt > 0 ? h->vec_blocks_opt : std::min<int64_t>((units + 3) / 4, 2048);
     h->vec_blocks = (int)std::max<int64_t>(blocks, 1);
 
     int64_t thr = h->long_row > 0 ? h->long_row : std::max<int64_t>(1024, 128 * G);
@@ -211,13 +193,61 @@ int plan_crs(pwm_matrix* h) {
         h->lr.chunk_begin = h->lr_begin.as<int32_t>();
         h->lr.chunk_partial = h->lr_partial.p;
     }
-    h->crs_used = 1;
-    if (h->crs_kernel == 1) return PWM_OK;
-    int rc = plan_crs_tiles(h);
-    if (rc != PWM_OK) return rc;
-    h->crs_used = 2;
-    if (h->crs_kernel == 0) return pick_crs_kernel(h);
     return PWM_OK;
+}
+
+// CRS planning and the autotuner.  Candidates: the vector kernel with the
+// rows-per-wave heuristic G0 = pow2(avg row length / 2) in [2, 32] and with
+// G0/2 and 2 G0 (only G0 when the caller fixed group_size), and the tiles
+// kernel (unless crs_kernel excludes either).  With more than one candidate,
+// each is timed on x = 1 (2 warm-up + 5 timed SpMVs) and the fastest is kept.
+int plan_crs(pwm_matrix* h) {
+    const double avg = h->nor > 0 ? (double)h->nnz / (double)h->nor : 0.0;
+    const int G0 = h->group_opt > 0 ? h->group_opt : std::min(32, std::max(2, pow2_ceil((int64_t)ceil(avg / 2.0))));
+    std::vector<int> gs;
+    if (h->crs_kernel != 2) {
+        if (h->group_opt > 0) {
+            gs.push_back(G0);
+        } else {
+            for (int g : {G0 / 2, G0, 2 * G0})
+                if (g >= 1 && g <= 64) gs.push_back(g);
+        }
+    }
+    const bool tiles = h->crs_kernel != 1;
+    int rc;
+    if (tiles && (rc = plan_crs_tiles(h)) != PWM_OK) return rc;
+    if ((int)gs.size() + (tiles ? 1 : 0) == 1) {
+        h->crs_used = tiles ? 2 : 1;
+        return plan_crs_vector(h, tiles ? G0 : gs[0]);
+    }
+    const size_t vs = vector_size(h->vtype);
+    DevBuf x, y;
+    hipError_t e;
+    if ((e = x.alloc(std::max<int64_t>(h->noc, 1) * vs)) != hipSuccess ||
+        (e = y.alloc(std::max<int64_t>(h->nor, 1) * vs)) != hipSuccess)
+        return hip_fail(h, e, "CRS autotune vectors");
+    e = pwmk::launch_fill(x.p, h->noc, h->vtype == PWM_VALUES_F64 ? 1 : 0, 1.0, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "CRS autotune vectors");
+    float best = -1.0f;
+    int best_kernel = 1, best_g = G0;
+    if (tiles) {
+        float ms = 0.0f;
+        if ((e = time_crs_kernel(h, 2, x.p, y.p, &ms)) != hipSuccess) return hip_fail(h, e, "CRS autotune");
+        best = ms;
+        best_kernel = 2;
+    }
+    for (int g : gs) {
+        if ((rc = plan_crs_vector(h, g)) != PWM_OK) return rc;
+        float ms = 0.0f;
+        if ((e = time_crs_kernel(h, 1, x.p, y.p, &ms)) != hipSuccess) return hip_fail(h, e, "CRS autotune");
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            best_kernel = 1;
+            best_g = g;
+        }
+    }
+    h->crs_used = best_kernel;
+    return plan_crs_vector(h, best_kernel == 1 ? best_g : G0);
 }
 
 // Merge path: P tiles of at most 256*ipt merge items (the threads count
@@ -338,9 +368,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     if (k == "group_size") {
         if (value != 0 && (value < 1 || value > 64 || (value & (value - 1))))
             return fail(h, PWM_ERR_INVALID, "group_size must be 0 or a power of two in 1..64");
-        h->group = (int)value;
+        h->group_opt = (int)value;
     } else if (k == "vec_blocks") {
-        h->vec_blocks = (int)std::max<int64_t>(0, value);
+        h->vec_blocks_opt = (int)std::max<int64_t>(0, value);
     } else if (k == "long_row") {
         h->long_row = std::max<int64_t>(0, value);
     } else if (k == "long_chunk") {

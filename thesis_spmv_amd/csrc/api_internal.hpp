@@ -131,8 +131,10 @@ struct pwm_matrix {
     std::vector<int32_t> h_rs;
 
     // CRS vector kernel plan
-    int group = 0;          // 0 = auto
-    int vec_blocks = 0;     // 0 = auto
+    int group_opt = 0;       // option: lanes per row, 0 = autotune
+    int vec_blocks_opt = 0;  // option: grid size, 0 = automatic
+    int group = 0;           // the plan in use
+    int vec_blocks = 0;
     int64_t long_row = 0;   // 0 = auto
     int chunk = 4096;
     pwmk::LongRows lr;

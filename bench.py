@@ -360,6 +360,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
     ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh")
+    ap.add_argument("--crs-kernel", type=int, default=0,
+                    help="CRS kernel: 0 = time the candidates at load, 1 = vector, 2 = row-aligned tiles")
+    ap.add_argument("--csb-groups", type=int, default=0, help="CSB block-column groups (0 = time 1, 8, 16 at load)")
+    ap.add_argument("--csb-unroll", type=int, default=0, help="CSB stream kernel unroll (0 = time 2 and 4 at load)")
     ap.add_argument("--values", default=None,
                     help="f64 and/or f32 SpMV passes (default f64; f64,f32 for stencil9)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -407,9 +411,10 @@ def main():
     y = pw.DeviceVector(n)
     results = {}
     wanted = set(args.formats.split(","))
-    formats = (("crs", pw.CRSHip, {}), ("merge_path", pw.CRSMergeHip, {}),
-               ("csb", pw.CSBHip, {"block_bits": 14} if args.workload == "stencil9" else {}),
-               ("blockcoh", pw.BlockCOHHip, {}))
+    csb_opts = {"block_bits": 14} if args.workload == "stencil9" else {}
+    csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
+    formats = (("crs", pw.CRSHip, {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}),
+               ("merge_path", pw.CRSMergeHip, {}), ("csb", pw.CSBHip, csb_opts), ("blockcoh", pw.BlockCOHHip, {}))
     for name, cls, opts in formats:
         if name not in wanted:
             continue
@@ -469,6 +474,22 @@ def main():
             del m
         del x32, y32
 
+    # gather-rate probe on this workload's x: a 12-byte stream per entry plus
+    # one random x gather each, over all of x, over an eighth of x per XCD (a
+    # CSB column group's window) and over 128 KiB per XCD (L2-resident)
+    gather = {}
+    if n >= 8 and z >= 8192:
+        m_eff = z // 8192 * 8192
+        for label, window, per_xcd in (("all_x", n, False), ("eighth_per_xcd", n // 8, True),
+                                       ("128KiB_per_xcd", min(16384, n // 8), True)):
+            ms = pw.gather_rate(x, window, per_xcd, z)
+            rate = m_eff / (ms * 1e-3)
+            gather[label] = {"window_doubles": int(window), "Ggathers_per_s": round(rate / 1e9, 1),
+                             "us_for_nnz": round(z / rate * 1e6, 1)}
+            print(f"# gather probe {label} ({window} doubles{' per XCD' if per_xcd else ''}): "
+                  f"{rate / 1e9:.1f} G gathers/s with the 12 B/entry stream, {z / rate * 1e6:.1f} us for nnz entries",
+                  file=sys.stderr, flush=True)
+
     # power method on the device with the best format: ms per iteration
     cls, opts = [(c, o) for nm, c, o in formats if nm == best][0]
     m = cls(-1, "f64", **opts)
@@ -484,7 +505,10 @@ def main():
     print(f"# power method ({best}, FP64, device vectors): {power_ms * 1e3:.1f} us/iteration", file=sys.stderr,
           flush=True)
     del m, xp
-    achieved = bytes_mv / (r["kernel_ms"] * 1e-3) / 1e9
+    # the SpMV's device time: its kernels back to back between two HIP events
+    # (CSB with column groups: slab kernel + group reduction; CRS/merge: the
+    # main kernel + its small fix-up)
+    achieved = bytes_mv / (r["event_ms"] * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{best}.json")
     if os.path.exists(tf):
@@ -507,8 +531,10 @@ def main():
         "config": {"workload": desc, "format": best, "nnz": z, "rows": n, "index": "int32",
                    "algorithmic_bytes_per_spmv": bytes_mv, "pct_of_hbm_peak": round(100 * r["GBs"] / HBM_PEAK_GBS, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "timed": f"all kernels of one {best} SpMV, {round(r['event_ms'] * 1e3, 2)} us (HIP events)"},
         "formats": {k: {"GBs_wall": round(v["GBs"], 1), "us_per_step": round(v["ms_per_step"] * 1e3, 2),
+                        "device_us": round(v["event_ms"] * 1e3, 2),
                         "dominant_kernel_us": round(v["kernel_ms"] * 1e3, 2),
                         "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
                                                        else round(v["kernel_ms_flushed"] * 1e3, 2)),
@@ -519,6 +545,8 @@ def main():
                     for k, v in results.items()},
         "power_method_us_per_iteration": round(power_ms * 1e3, 2),
     }
+    if gather:
+        out["gather_probe"] = gather
     if f32:
         out["f32"] = f32
     if not args.no_cpu:

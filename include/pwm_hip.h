@@ -115,9 +115,15 @@ int pwm_info(const pwm_matrix* h, pwm_info_t* out);
  * 0 = automatic), "long_row" (CRS: rows longer than this use the long-row
  * path), "items_per_thread" (merge path), "block_bits" (CSB beta override),
  * "csb_groups" (CSB block-column groups pinned to XCDs, 0 = time 1/8/16 at
- * load and keep the fastest), "crs_kernel" (CRS: 0 = time the vector kernel
- * and the row-aligned tiles kernel at load and keep the faster, 1 = vector,
- * 2 = tiles).  Must be set before the matrix is loaded unless noted. */
+ * load and keep the fastest), "csb_layout" (CSB device layout: 1 = slab
+ * pieces over the stored block order, 2 = one contiguous entry stream per row
+ * slab with (row in slab, column) packed in 32 bits, 0 = 2 when slabs of at
+ * least 256 rows pack), "csb_unroll" (slab-stream kernel groups of 4 entries
+ * per thread per step, 2 or 4; 0 = time both with each group count),
+ * "crs_kernel" (CRS: 0 = time the vector kernel at
+ * three lane counts and the row-aligned tiles kernel at load and keep the
+ * fastest, 1 = vector, 2 = tiles).  Must be set before the matrix is loaded
+ * unless noted. */
 int pwm_set_option(pwm_matrix* h, const char* key, int64_t value);
 
 /* ---- device-resident additions ----------------------------------------
@@ -145,7 +151,8 @@ int pwm_crs_export(pwm_matrix* h, int32_t* row_start, int32_t* col_ind, double* 
 
 /* Device layout export for bit-exact parity checks.
  * CSB (SOA_Implementations/CSB.hpp:464-567): ind[nnz], data[nnz] (double),
- * blk_ptr[H*V+1].  NULL arrays: sizes only. */
+ * blk_ptr[H*V+1] in the stored block order (a slab-stream handle rebuilds it
+ * from its streams).  NULL arrays: sizes only. */
 int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nblocks, uint32_t* ind, double* data,
                    int32_t* blk_ptr);
 /* BlockCOH band p (SOA_Implementations/BlockCOH.hpp:319-455).  counts[0..5] =
@@ -177,6 +184,15 @@ int pwm_device_copy(void* dst, const void* src, int64_t bytes);
  * not NULL, of the dominant kernel alone. */
 int pwm_time_mv(pwm_matrix* h, const void* x_dev, void* y_dev, int reps, int64_t flush_bytes,
                 double* ms_per_mv, double* kernel_ms);
+
+/* Gather-rate probe (addition, used by bench.py): mean milliseconds of one
+ * pass over n entries (rounded down to a multiple of 8192) of a stream of
+ * (u32 index, f64 value) pairs, 12 bytes each like a CRS nonzero, where every
+ * index gathers one double of x_dev (x_len doubles): from [0, window), or with
+ * per_xcd from the window [j*window, (j+1)*window) of the XCD j that runs the
+ * workgroup.  Indices are hashed, uniform in the window.  The bound a sparse
+ * SpMV's gathers meet: n nonzeros cannot take less than n / this rate. */
+int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n, double* ms);
 
 /* ---- generators for the benchmark workloads (host side, deterministic) -
  * Each writes CRS arrays into malloc'd buffers released with pwm_free.

@@ -14,6 +14,9 @@ pw = pytest.importorskip("thesis_spmv_amd")
 pytestmark = pytest.mark.gpu
 
 FORMATS = [pw.CSBHip, pw.BlockCOHHip]
+# CSB device layouts: 1 = slab pieces over the stored block order, 2 = one
+# contiguous stream per slab (the automatic choice when its index packs)
+CSB_LAYOUTS = [1, 2]
 
 
 def _relerr(a, b):
@@ -48,8 +51,10 @@ def test_golden(goldens, cls, idx):
     case = goldens[idx]
     A, t = load_case_matrix(case)
     print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
-    for threads in ((1, 2, 3, 4, 8, -1) if cls is pw.BlockCOHHip else (-1,)):
-        m = cls(threads)
+    runs = ([(t, {}) for t in (1, 2, 3, 4, 8, -1)] if cls is pw.BlockCOHHip
+            else [(-1, {"csb_layout": lay}) for lay in CSB_LAYOUTS])
+    for threads, opts in runs:
+        m = cls(threads, **opts)
         if case["matrix"]["kind"] == "poisson":
             m.generatePoissonMatrix(case["matrix"]["m"], case["matrix"]["n"], 1)
         else:
@@ -68,10 +73,11 @@ def test_golden(goldens, cls, idx):
 
 # ---------------------------------------------------------------- layouts
 
-@pytest.mark.parametrize("bits", [0, 2, 3, 5])
-def test_csb_layout_bit_exact(bits):
+@pytest.mark.parametrize("layout", CSB_LAYOUTS)
+@pytest.mark.parametrize("bits", [0, 2, 3, 5, 9])
+def test_csb_layout_bit_exact(bits, layout):
     for A in (oracle.poisson_crs(9, 7), oracle.poisson_crs(40, 31), _powerlaw(3000, cap=400)):
-        m = _load(pw.CSBHip, A, block_bits=bits)
+        m = _load(pw.CSBHip, A, block_bits=bits, csb_layout=layout)
         gb, gind, gdata, gblk = m.csb_layout()
         rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), bits)
         assert gb == rb
@@ -79,7 +85,8 @@ def test_csb_layout_bit_exact(bits):
         assert m.info()["balanced"] == rbal
 
 
-def test_csb_layout_duplicates_and_unbalanced():
+@pytest.mark.parametrize("layout", CSB_LAYOUTS)
+def test_csb_layout_duplicates_and_unbalanced(layout):
     rng = np.random.default_rng(4)
     N = 700
     r = np.concatenate([rng.integers(0, 40, 3000), rng.integers(0, N, 500)]).astype(np.int32)
@@ -88,7 +95,7 @@ def test_csb_layout_duplicates_and_unbalanced():
     v = rng.integers(1, 5, 3500).astype(np.float64)
     t = oracle.Triplets(N, N, r, c, v)
     A = oracle.triplet_to_crs(t)
-    m = pw.CSBHip()
+    m = pw.CSBHip(csb_layout=layout)
     m.loadFromTriplets(t)
     gb, gind, gdata, gblk = m.csb_layout()
     rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), 0)
@@ -127,7 +134,10 @@ def test_spmv_random_and_stencil(cls):
         x = pw.gen_uniform(A.noc, 0)
         ref = oracle.crs_mv(A, x)
         csb_opts = ({}, {"block_bits": 7}, {"csb_groups": 1}, {"csb_groups": 3}, {"csb_groups": 8},
-                    {"csb_groups": 16}, {"block_bits": 7, "csb_groups": 16})
+                    {"csb_groups": 16}, {"block_bits": 7, "csb_groups": 16}, {"csb_layout": 1},
+                    {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
+                    {"csb_layout": 2, "block_bits": 3, "csb_groups": 5}, {"csb_unroll": 2, "csb_groups": 1},
+                    {"csb_unroll": 4, "csb_groups": 8})
         for opt in csb_opts if cls is pw.CSBHip else ({}, ):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)
@@ -158,10 +168,12 @@ def test_edge_shapes(cls):
     for A in cases:
         x = rng.standard_normal(A.noc)
         ref = oracle.crs_mv(A, x)
-        m = _load(cls, A)
-        y = np.full(A.nor, np.nan)
-        m.mv(x, y)
-        assert _relerr(y, ref) <= 1e-13
+        for opt in ({"csb_layout": 1}, {"csb_layout": 2}, {"csb_layout": 2, "csb_groups": 8}) \
+                if cls is pw.CSBHip else ({},):
+            m = _load(cls, A, **opt)
+            y = np.full(A.nor, np.nan)
+            m.mv(x, y)
+            assert _relerr(y, ref) <= 1e-13, (A.nor, A.noc, opt)
 
 
 def test_config2_full_size():
@@ -199,7 +211,9 @@ def test_power_iteration_state(cls, it):
     A.data[:] = np.abs(A.data)
     x0 = pw.gen_uniform(A.nor, 3)
     xr, yr = oracle.crs_power_iteration(A, x0, it)
-    for opt in (({"csb_groups": 1}, {"csb_groups": 8, "block_bits": 9}, {"csb_groups": 16, "block_bits": 8})
+    for opt in (({"csb_groups": 1}, {"csb_groups": 8, "block_bits": 9}, {"csb_groups": 16, "block_bits": 8},
+                 {"csb_unroll": 4, "csb_groups": 1}, {"csb_unroll": 2, "csb_groups": 8},
+                 {"csb_layout": 1, "csb_groups": 1}, {"csb_layout": 1, "csb_groups": 8, "block_bits": 9})
                 if cls is pw.CSBHip else ({}, {"long_chunk": 256})[:1]):
         m = _load(cls, A, **opt)
         x, y = x0.copy(), np.zeros(A.nor)

@@ -371,3 +371,14 @@ def test_crs_default_times_both_kernels():
     A = oracle.poisson_crs(300, 300)
     m = _load(pw.CRSHip, A)
     assert m.info()["group_size"] in (0, 2, 4)  # whichever kernel was faster
+
+
+def test_gather_rate_probe():
+    """pwm_gather_rate runs on any FP64 vector and rejects windows that do not fit."""
+    x = pw.DeviceVector.from_array(np.ones(1 << 16))
+    assert pw.gather_rate(x, 1 << 13, True, 1 << 16) > 0
+    assert pw.gather_rate(x, 1 << 16, False, 3 * 8192 + 5) > 0
+    with pytest.raises(pw.PwmError):
+        pw.gather_rate(x, 1 << 14, True, 1 << 16)  # eight windows exceed x
+    with pytest.raises(pw.PwmError):
+        pw.gather_rate(x, 1 << 13, True, 100)  # fewer entries than one workgroup

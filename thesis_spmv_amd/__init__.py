@@ -64,6 +64,7 @@ _SIGS = {
     "pwm_device_fill": [VP, I64, C.c_int, F64],
     "pwm_device_copy": [VP, VP, I64],
     "pwm_time_mv": [VP, VP, VP, C.c_int, I64, C.POINTER(F64), C.POINTER(F64)],
+    "pwm_gather_rate": [VP, I64, I64, C.c_int, I64, C.POINTER(F64)],
     "pwm_gen_powerlaw": [I64, F64, F64, I64, C.c_uint64, C.c_uint64, C.POINTER(VP), C.POINTER(VP),
                          C.POINTER(VP), C.POINTER(I64)],
     "pwm_gen_stencil9": [I64, I64, C.c_uint64, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(I64)],
@@ -168,6 +169,17 @@ class DeviceVector:
             self.free()
         except Exception:
             pass
+
+
+def gather_rate(x, window, per_xcd, n):
+    """Milliseconds of one pass of the gather-rate probe (pwm_gather_rate):
+    n 12-byte (index, value) entries, each gathering one double of the FP64
+    DeviceVector x from a window of `window` doubles (per XCD with per_xcd)."""
+    if x.dtype != np.float64:
+        raise ValueError("gather_rate needs an FP64 DeviceVector")
+    ms = F64()
+    _check(lib().pwm_gather_rate(x.ptr, int(x.n), int(window), int(bool(per_xcd)), int(n), C.byref(ms)))
+    return ms.value
 
 
 def synchronize():

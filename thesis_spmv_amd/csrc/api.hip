@@ -386,6 +386,13 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_groups") {
         if (value < 0 || value > 64) return fail(h, PWM_ERR_INVALID, "csb_groups must be 0..64");
         h->csb_groups = (int32_t)value;
+    } else if (k == "csb_layout") {
+        if (value < 0 || value > 2)
+            return fail(h, PWM_ERR_INVALID, "csb_layout must be 0 (automatic), 1 (slab pieces) or 2 (slab streams)");
+        h->csb_layout = (int32_t)value;
+    } else if (k == "csb_unroll") {
+        if (value != 0 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_unroll must be 0, 2 or 4");
+        h->csb_unroll = (int32_t)value;
     } else if (k == "crs_kernel") {
         if (value < 0 || value > 2) return fail(h, PWM_ERR_INVALID, "crs_kernel must be 0 (time both), 1 (vector) or 2 (tiles)");
         h->crs_kernel = (int)value;
@@ -848,6 +855,38 @@ extern "C" int pwm_device_copy(void* dst, const void* src, int64_t bytes) {
 
 // ---------------------------------------------------------------- timing
 
+extern "C" int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n,
+                               double* ms) {
+    if (!x_dev || !ms || window <= 0 || window > UINT32_MAX || (per_xcd ? 8 * window : window) > x_len)
+        return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: window must fit x (8 windows when per_xcd)");
+    const int64_t m = pwmk::gather_probe_entries(n);
+    if (m <= 0) return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: n must be at least 8192");
+    DevBuf idx, val, out;
+    hipError_t e;
+    if ((e = idx.alloc(m * 4)) != hipSuccess || (e = val.alloc(m * 8)) != hipSuccess ||
+        (e = out.alloc(m / 8192 * 256 * 8)) != hipSuccess)
+        return hip_fail(nullptr, e, "pwm_gather_rate buffers");
+    e = pwmk::launch_gather_fill(idx.as<uint32_t>(), val.as<double>(), m, (uint32_t)window, nullptr);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    const double* x = (const double*)x_dev;
+    for (int r = 0; r < 2 && e == hipSuccess; ++r)
+        e = pwmk::launch_gather_probe(idx.as<uint32_t>(), val.as<double>(), x, out.as<double>(), m, window, per_xcd, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(a, nullptr);
+    for (int r = 0; r < 10 && e == hipSuccess; ++r)
+        e = pwmk::launch_gather_probe(idx.as<uint32_t>(), val.as<double>(), x, out.as<double>(), m, window, per_xcd, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(b, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "pwm_gather_rate");
+    *ms = t / 10.0;
+    return PWM_OK;
+}
+
 extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int64_t flush_bytes, double* ms_per_mv,
                            double* kernel_ms) {
     if (!h || !x || !y || reps <= 0) return fail(h, PWM_ERR_INVALID, "pwm_time_mv: bad argument");
@@ -881,6 +920,11 @@ extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int6
                 e = pwmk::launch_merge_spmv(A, h->mp, x, y, nullptr, h->stream);
                 (void)hipEventRecord(ev[3 * r + 1], h->stream);
                 if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+                break;
+            case PWM_FORMAT_CSB:
+                e = csb_launch(h, x, y, nullptr, 1);
+                (void)hipEventRecord(ev[3 * r + 1], h->stream);
+                if (e == hipSuccess) e = csb_launch(h, x, y, nullptr, 2);
                 break;
             default:
                 rc = spmv_launch(h, x, y, nullptr, nullptr);

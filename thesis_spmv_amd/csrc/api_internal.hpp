@@ -75,6 +75,13 @@ struct CsbDev {
     int64_t slab_rows = 0;
     int32_t groups = 1;          // block-column groups (one per XCD)
     DevBuf ypart;                // groups x nor partial sums
+    // Slab-stream layout (stream = 1): ind/data hold every slab's entries
+    // contiguously, block columns ascending, as (row in slab << col_bits) |
+    // column; soff[t*H + bc] is the first entry of slab t in block column bc.
+    int32_t stream = 0;
+    int32_t col_bits = 0;
+    int32_t unroll = 2;          // stream kernel groups of 4 per thread per step
+    DevBuf soff;
     void clear() {
         bits = 0;
         H = V = 0;
@@ -85,6 +92,10 @@ struct CsbDev {
         blk_ptr.release();
         work.release();
         ypart.release();
+        soff.release();
+        stream = 0;
+        col_bits = 0;
+        unroll = 2;
         n_tasks = 0;
         slab_bits = 0;
     }
@@ -154,6 +165,8 @@ struct pwm_matrix {
     // blocked formats
     int32_t csb_bits_override = 0;
     int32_t csb_groups = 0;  // 0 = autotune
+    int32_t csb_layout = 0;  // 0 = streams when the packed index fits, 1 = pieces, 2 = streams
+    int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     pwm_impl::CsbDev csb;
     pwm_impl::CohDev coh;
 
@@ -181,7 +194,7 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
 int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals);
 // one SpMV of the loaded format on device vectors; pw != nullptr: power step
 int spmv_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw, int* n_partials_out);
-hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw);
+hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw, int part = 0);
 hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw);
 int64_t csb_partials(const pwm_matrix* h);
 int64_t coh_partials(const pwm_matrix* h);

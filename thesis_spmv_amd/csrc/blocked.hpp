@@ -39,6 +39,14 @@ struct CsbArgs {
     int32_t groups = 1;
     void* ypart = nullptr;          // accumulator type, groups * nor
     int vtype = 0;
+    // Slab-stream layout: entries of slab task t in block columns [bc0, bc1)
+    // are ind/val[soff[t*H + bc0] .. soff[t*H + bc1]), each index packed as
+    // (row in slab << col_bits) | column; ind/val extend 4 zero entries past
+    // the last one.  slabs/slab_rows describe these slabs; pieces is unused.
+    int32_t stream = 0;
+    int32_t col_bits = 0;
+    const int32_t* soff = nullptr;
+    int32_t unroll = 2;  // groups of 4 entries per thread per step: 2 or 4
 };
 
 // One BlockCOH block (offsets are relative to its band's arrays).
@@ -84,7 +92,10 @@ struct CohArgs {
     int vtype = 0;
 };
 
-hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s);
+// part: 0 = the whole SpMV, 1 = the slab kernel only, 2 = the column-group
+// reduction only (nothing when groups = 1)
+hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const pwmk::PowerArgs* pw, hipStream_t s,
+                      int part = 0);
 // workgroups of the CSB column-group reduction (= its power-method partials)
 int csb_reduce_blocks(int64_t nor);
 hipError_t launch_coh(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,

@@ -157,50 +157,121 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
             break;
         }
 
-    // Slab pieces: a workgroup owns `slab` rows of a block row (its y-slice
-    // lives in LDS).  With slab = beta >> k, the entries of one block whose
-    // top k row bits equal s form 2^(k-1) contiguous Z-order runs (one per
-    // value of the top k-1 column bits), or the whole block when k = 0.
-    const int64_t lds_rows = (int64_t)blocked::lds_rows(h->vtype);
-    int k = 0;
-    while ((beta >> k) > lds_rows) ++k;
-    const int64_t slab = beta >> k;
-    const int64_t np = k == 0 ? 1 : ((int64_t)1 << (k - 1));
-    const int64_t S = (int64_t)1 << k;  // slabs per block row
-    c.slab_bits = bits - k;
-    // task t = br*S + s; pieces[t][bc][j] = (start, end)
-    std::vector<blocked::Piece> pieces(V * S * H * np);
-    const int shift = 2 * bits - (k == 0 ? 0 : 2 * k - 1);  // key bits below the piece prefix
-    parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
-        std::vector<int64_t> cntp;
-        for (int64_t br = b0; br < b1; ++br)
-            for (int64_t bc = 0; bc < H; ++bc) {
-                const int64_t s0 = blk_ptr[br * H + bc], s1 = blk_ptr[br * H + bc + 1];
-                if (k == 0) {
-                    pieces[((br * S) * H + bc) * np] = {(int32_t)s0, (int32_t)s1};
-                    continue;
-                }
-                // prefixes in key order: 2^(2k-1) runs; prefix bits are
-                // r_{b-1} c_{b-1} ... r_{b-k}: slab = its k row bits, piece = its k-1 column bits
-                const int64_t nprefix = (int64_t)1 << (2 * k - 1);
-                cntp.assign(nprefix + 1, 0);
-                for (int64_t e = s0; e < s1; ++e) {
-                    const uint32_t key = zkey(ind[e] >> 16, ind[e] & 0xffffu);
-                    cntp[(key >> shift) + 1]++;
-                }
-                for (int64_t q = 0; q < nprefix; ++q) cntp[q + 1] += cntp[q];
-                for (int64_t q = 0; q < nprefix; ++q) {
-                    // decode prefix q: bit 2k-2 is r_{b-1}, then c_{b-1}, r_{b-2}, ...
-                    int64_t sr = 0, pc = 0;
-                    for (int lvl = 0; lvl < k; ++lvl) {
-                        const int rb = (int)((q >> (2 * k - 2 - 2 * lvl)) & 1);
-                        sr = (sr << 1) | rb;
-                        if (lvl < k - 1) pc = (pc << 1) | ((q >> (2 * k - 3 - 2 * lvl)) & 1);
+    // Slab-stream layout: slabs of 2^rb rows, rb <= the LDS rows, beta and
+    // 32 - (bits of a column index), so that (row in slab, column) packs into
+    // 32 bits.  Used unless the caller asks for pieces, or the slabs would be
+    // shorter than 256 rows (automatic choice only).
+    const int lds_bits = blocked::lds_rows(h->vtype) == 8192 ? 13 : 12;
+    int cb = 0;
+    while (((int64_t)1 << cb) < noc) ++cb;
+    // no longer than needed for about 1024 slabs (a few workgroups per CU),
+    // but at least 256 rows
+    int fill_bits = 8;
+    while (fill_bits < lds_bits && (nor >> (fill_bits + 1)) >= 1024) ++fill_bits;
+    const int rb = std::min(std::min(std::min(lds_bits, fill_bits), bits), 32 - cb);
+    const bool use_stream = h->csb_layout == 2 || (h->csb_layout == 0 && rb >= std::min(8, bits));
+    if (use_stream) {
+        const int64_t S = beta >> rb, hrows = (int64_t)1 << rb;
+        const int64_t cells = V * S * H;
+        std::vector<int64_t> off(cells + 1, 0);
+        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+            for (int64_t br = b0; br < b1; ++br)
+                for (int64_t bc = 0; bc < H; ++bc)
+                    for (int64_t e = blk_ptr[br * H + bc]; e < blk_ptr[br * H + bc + 1]; ++e)
+                        off[(br * S + ((ind[e] >> 16) >> rb)) * H + bc + 1]++;
+        });
+        for (int64_t i = 0; i < cells; ++i) off[i + 1] += off[i];
+        const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
+        std::vector<uint32_t> sind(zpad, 0u);
+        std::vector<double> sdat(zpad, 0.0);
+        // within a slab: block columns ascending, each block's entries in
+        // their stored (Z-Morton) order
+        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+            std::vector<int64_t> pos(S);
+            for (int64_t br = b0; br < b1; ++br)
+                for (int64_t bc = 0; bc < H; ++bc) {
+                    for (int64_t sl = 0; sl < S; ++sl) pos[sl] = off[(br * S + sl) * H + bc];
+                    for (int64_t e = blk_ptr[br * H + bc]; e < blk_ptr[br * H + bc + 1]; ++e) {
+                        const uint32_t lrow = ind[e] >> 16;
+                        const int64_t p = pos[lrow >> rb]++;
+                        sind[p] = ((lrow & (uint32_t)(hrows - 1)) << cb) | (uint32_t)(bc * beta + (ind[e] & 0xffffu));
+                        sdat[p] = data[e];
                     }
-                    pieces[((br * S + sr) * H + bc) * np + pc] = {(int32_t)(s0 + cntp[q]), (int32_t)(s0 + cntp[q + 1])};
                 }
-            }
-    });
+        });
+        std::vector<int32_t> off32(off.begin(), off.end());
+        hipError_t e;
+        if ((e = upload(c.ind, sind.data(), zpad * 4)) != hipSuccess ||
+            (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
+            (e = upload(c.soff, off32.data(), off32.size() * 4)) != hipSuccess)
+            return hip_fail(h, e, "CSB upload");
+        rc = upload_blocked_values(h, c.data, sdat);
+        if (rc != PWM_OK) return rc;
+        c.stream = 1;
+        c.col_bits = cb;
+        c.slab_bits = rb;
+        c.slabs = (int32_t)S;
+        c.slab_rows = hrows;
+        c.pieces_per_block = 1;
+    } else {
+        // Slab pieces: a workgroup owns `slab` rows of a block row (its y-slice
+        // lives in LDS).  With slab = beta >> k, the entries of one block whose
+        // top k row bits equal s form 2^(k-1) contiguous Z-order runs (one per
+        // value of the top k-1 column bits), or the whole block when k = 0.
+        const int64_t lds_rows = (int64_t)blocked::lds_rows(h->vtype);
+        int k = 0;
+        while ((beta >> k) > lds_rows) ++k;
+        const int64_t slab = beta >> k;
+        const int64_t np = k == 0 ? 1 : ((int64_t)1 << (k - 1));
+        const int64_t S = (int64_t)1 << k;  // slabs per block row
+        c.slab_bits = bits - k;
+        // task t = br*S + s; pieces[t][bc][j] = (start, end)
+        std::vector<blocked::Piece> pieces(V * S * H * np);
+        const int shift = 2 * bits - (k == 0 ? 0 : 2 * k - 1);  // key bits below the piece prefix
+        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+            std::vector<int64_t> cntp;
+            for (int64_t br = b0; br < b1; ++br)
+                for (int64_t bc = 0; bc < H; ++bc) {
+                    const int64_t s0 = blk_ptr[br * H + bc], s1 = blk_ptr[br * H + bc + 1];
+                    if (k == 0) {
+                        pieces[((br * S) * H + bc) * np] = {(int32_t)s0, (int32_t)s1};
+                        continue;
+                    }
+                    // prefixes in key order: 2^(2k-1) runs; prefix bits are
+                    // r_{b-1} c_{b-1} ... r_{b-k}: slab = its k row bits, piece = its k-1 column bits
+                    const int64_t nprefix = (int64_t)1 << (2 * k - 1);
+                    cntp.assign(nprefix + 1, 0);
+                    for (int64_t e = s0; e < s1; ++e) {
+                        const uint32_t key = zkey(ind[e] >> 16, ind[e] & 0xffffu);
+                        cntp[(key >> shift) + 1]++;
+                    }
+                    for (int64_t q = 0; q < nprefix; ++q) cntp[q + 1] += cntp[q];
+                    for (int64_t q = 0; q < nprefix; ++q) {
+                        // decode prefix q: bit 2k-2 is r_{b-1}, then c_{b-1}, r_{b-2}, ...
+                        int64_t sr = 0, pc = 0;
+                        for (int lvl = 0; lvl < k; ++lvl) {
+                            const int rbit = (int)((q >> (2 * k - 2 - 2 * lvl)) & 1);
+                            sr = (sr << 1) | rbit;
+                            if (lvl < k - 1) pc = (pc << 1) | ((q >> (2 * k - 3 - 2 * lvl)) & 1);
+                        }
+                        pieces[((br * S + sr) * H + bc) * np + pc] = {(int32_t)(s0 + cntp[q]), (int32_t)(s0 + cntp[q + 1])};
+                    }
+                }
+        });
+        hipError_t e;
+        if ((e = upload(c.ind, ind.data(), nnz * 4)) != hipSuccess ||
+            (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
+            (e = upload(c.work, pieces.data(), pieces.size() * sizeof(blocked::Piece))) != hipSuccess)
+            return hip_fail(h, e, "CSB upload");
+        data.resize(nnz);
+        rc = upload_blocked_values(h, c.data, data);
+        if (rc != PWM_OK) return rc;
+        c.pieces_per_block = (int32_t)np;
+        c.slabs = (int32_t)S;
+        c.slab_rows = slab;
+    }
+    const int64_t S = c.slabs;
+    const int64_t units = V * S;  // workgroups per column group
     // block-column groups: the caller's count, else the fastest of 1, 8 and
     // 16 (timed below, once the arrays are on the device)
     std::vector<int32_t> cands;
@@ -211,21 +282,22 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
             if (std::find(cands.begin(), cands.end(), (int32_t)std::min(gcount, H)) == cands.end())
                 cands.push_back((int32_t)std::min(gcount, H));
     const int32_t gmax = *std::max_element(cands.begin(), cands.end());
+    // stream kernel groups of 4 entries per thread per step: the caller's
+    // choice, else 2 and 4 are timed with each group count
+    std::vector<int32_t> unrolls;
+    if (!c.stream)
+        unrolls.push_back(2);
+    else if (h->csb_unroll > 0)
+        unrolls.push_back(h->csb_unroll);
+    else
+        unrolls = {2, 4};
     c.groups = cands[0];
-    c.n_tasks = V * S * c.groups;
+    c.unroll = unrolls[0];
+    c.n_tasks = units * c.groups;
     hipError_t e;
     if (gmax > 1 && (e = c.ypart.alloc((size_t)gmax * nor * vector_size(h->vtype))) != hipSuccess)
         return hip_fail(h, e, "CSB partial sums");
-    if ((e = upload(c.ind, ind.data(), nnz * 4)) != hipSuccess || (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
-        (e = upload(c.work, pieces.data(), pieces.size() * sizeof(blocked::Piece))) != hipSuccess)
-        return hip_fail(h, e, "CSB upload");
-    data.resize(nnz);
-    rc = upload_blocked_values(h, c.data, data);
-    if (rc != PWM_OK) return rc;
-    c.pieces_per_block = (int32_t)np;
-    c.slabs = (int32_t)S;
-    c.slab_rows = slab;
-    if (cands.size() > 1) {
+    if (cands.size() * unrolls.size() > 1) {
         // autotune: a few timed SpMVs per candidate on x = 1
         DevBuf tx, ty;
         const size_t vs = vector_size(h->vtype);
@@ -236,26 +308,30 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
         float best = 1e30f;
-        int32_t best_g = cands[0];
-        for (int32_t gc : cands) {
-            c.groups = gc;
-            c.n_tasks = V * S * gc;
-            for (int w = 0; w < 2; ++w) (void)csb_launch(h, tx.p, ty.p, nullptr);
-            (void)hipEventRecord(e0, h->stream);
-            for (int r = 0; r < 5; ++r) (void)csb_launch(h, tx.p, ty.p, nullptr);
-            (void)hipEventRecord(e1, h->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) {
-                best = ms;
-                best_g = gc;
+        int32_t best_g = cands[0], best_u = unrolls[0];
+        for (int32_t gc : cands)
+            for (int32_t uc : unrolls) {
+                c.groups = gc;
+                c.unroll = uc;
+                c.n_tasks = units * gc;
+                for (int w = 0; w < 2; ++w) (void)csb_launch(h, tx.p, ty.p, nullptr);
+                (void)hipEventRecord(e0, h->stream);
+                for (int r = 0; r < 5; ++r) (void)csb_launch(h, tx.p, ty.p, nullptr);
+                (void)hipEventRecord(e1, h->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    best_g = gc;
+                    best_u = uc;
+                }
             }
-        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c.groups = best_g;
-        c.n_tasks = V * S * best_g;
+        c.unroll = best_u;
+        c.n_tasks = units * best_g;
         if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(h, e, "CSB autotune");
     }
     // the CRS arrays are not needed by the CSB kernels
@@ -540,7 +616,7 @@ int64_t csb_partials(const pwm_matrix* h) {
 }
 int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks + h->coh.n_atomic; }
 
-hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw) {
+hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerArgs* pw, int part) {
     blocked::CsbArgs a;
     a.nor = h->nor;
     a.bits = h->csb.bits;
@@ -557,8 +633,13 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.groups = h->csb.groups;
     a.ypart = h->csb.ypart.p;
     a.vtype = h->vtype;
-    hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream);
-    if (e == hipSuccess && pw) e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);
+    a.stream = h->csb.stream;
+    a.col_bits = h->csb.col_bits;
+    a.soff = h->csb.soff.as<int32_t>();
+    a.unroll = h->csb.unroll;
+    hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream, part);
+    if (e == hipSuccess && pw && part != 1)
+        e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);
     return e;
 }
 
@@ -586,6 +667,59 @@ hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
 
 using namespace pwm_impl;
 
+// The stored CSB arrays of a slab-stream handle: every block's entries are
+// collected from its slabs' runs and put back in Z-Morton order (stable: a
+// duplicate (row, column) pair lies in one slab, in its stored order), which
+// is the order the build gave them before they were split into slabs.
+static int csb_stream_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t* blk_ptr) {
+    const CsbDev& c = h->csb;
+    const int64_t H = c.H, V = c.V, S = c.slabs, nnz = h->nnz;
+    const int rb = c.slab_bits, cb = c.col_bits;
+    const int64_t beta = (int64_t)1 << c.bits;
+    const uint32_t cmask = (uint32_t)(((uint64_t)1 << cb) - 1);
+    std::vector<int32_t> off(V * S * H + 1);
+    std::vector<uint32_t> sind(std::max<int64_t>(nnz, 1));
+    std::vector<double> sdat(std::max<int64_t>(nnz, 1));
+    hipError_t e = hipMemcpy(blk_ptr, c.blk_ptr.p, (H * V + 1) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(off.data(), c.soff.p, off.size() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nnz) e = hipMemcpy(sind.data(), c.ind.p, nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nnz) {
+        DevBuf tmp;
+        e = tmp.alloc(nnz * 8);
+        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(c.data.p, h->vtype, tmp.as<double>(), nnz, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(sdat.data(), tmp.p, nnz * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_csb_layout");
+    parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+        struct Ent {
+            uint32_t key, ind;
+            double v;
+        };
+        std::vector<Ent> tmp;
+        for (int64_t br = b0; br < b1; ++br)
+            for (int64_t bc = 0; bc < H; ++bc) {
+                tmp.clear();
+                for (int64_t sl = 0; sl < S; ++sl) {
+                    const int64_t cell = (br * S + sl) * H + bc;
+                    for (int64_t p = off[cell]; p < off[cell + 1]; ++p) {
+                        const uint32_t w = sind[p];
+                        const uint32_t lrow = (uint32_t)(sl << rb) + (cb < 32 ? (w >> cb) : 0u);
+                        const uint32_t col = (uint32_t)((w & cmask) - bc * beta);
+                        tmp.push_back({zkey(lrow, col), (lrow << 16) | col, sdat[p]});
+                    }
+                }
+                std::stable_sort(tmp.begin(), tmp.end(), [](const Ent& a, const Ent& b) { return a.key < b.key; });
+                const int64_t base = blk_ptr[br * H + bc];
+                for (size_t i = 0; i < tmp.size(); ++i) {
+                    ind[base + i] = tmp[i].ind;
+                    data[base + i] = tmp[i].v;
+                }
+            }
+    });
+    return PWM_OK;
+}
+
 extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nblocks, uint32_t* ind, double* data,
                               int32_t* blk_ptr) {
     if (!h || !block_bits || !nblocks) return fail(h, PWM_ERR_INVALID, "pwm_csb_layout: NULL argument");
@@ -593,6 +727,7 @@ extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nbloc
     *block_bits = h->csb.bits;
     *nblocks = h->csb.H * h->csb.V;
     if (!ind || !data || !blk_ptr) return PWM_OK;
+    if (h->csb.stream) return csb_stream_layout(h, ind, data, blk_ptr);
     hipError_t e = hipMemcpy(blk_ptr, h->csb.blk_ptr.p, (*nblocks + 1) * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && h->nnz) e = hipMemcpy(ind, h->csb.ind.p, h->nnz * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && h->nnz) {

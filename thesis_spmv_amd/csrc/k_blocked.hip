@@ -15,6 +15,7 @@
 // random columns.
 #include "blocked.hpp"
 #include "kernels.hpp"
+#include "merge_tile.hpp"
 
 namespace blocked {
 
@@ -100,6 +101,62 @@ __device__ __forceinline__ int block_seg_excl(int v, int f, int carry, int* tv, 
 
 // ---------------------------------------------------------------- CSB
 
+// workgroup -> (slab task t, column group g).  With a multiple of 8 groups the
+// launch runs in rounds of 8 groups so that XCD (b % 8) keeps one group per
+// round: b = round * 8 * slabs_total + slab * 8 + xcd.
+__device__ __forceinline__ void csb_task(const CsbArgs& a, int& g, int64_t& t) {
+    if (a.groups % 8 == 0) {
+        const int64_t per_round = (a.n_tasks / a.groups) * 8;
+        const int64_t round = blockIdx.x / per_round, rem = blockIdx.x % per_round;
+        t = rem / 8;
+        g = (int)(round * 8 + rem % 8);
+    } else {
+        g = (int)(blockIdx.x % a.groups);
+        t = blockIdx.x / a.groups;
+    }
+}
+
+// Writes a slab's LDS sums: to ypart[g] with several column groups, else to
+// y (POWER: scaled, with the slab's |y|^2, x.y, x.x in partial slot t).
+template <typename X, typename Acc, int BS, bool POWER>
+__device__ __forceinline__ void csb_write_slab(const CsbArgs& a, const Acc* ylds, int h, int64_t row0, int g, int64_t t,
+                                               const X* __restrict__ x, X* __restrict__ y, double scale,
+                                               const PowerArgs& pw, double* red) {
+    const int tid = threadIdx.x;
+    if (a.groups > 1) {  // partial slab sums; k_csb_reduce finishes y
+        Acc* yp = (Acc*)a.ypart + (int64_t)g * a.nor;
+        for (int i = tid; i < h; i += BS) {
+            const int64_t row = row0 + i;
+            if (row >= a.nor) break;
+            yp[row] = ylds[i];
+        }
+        return;
+    }
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = tid; i < h; i += BS) {
+        const int64_t row = row0 + i;
+        if (row >= a.nor) break;
+        if constexpr (POWER) {
+            const X out = (X)((double)ylds[i] * scale);
+            y[row] = out;
+            const double yd = (double)out, xd = (double)x[row] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        } else {
+            y[row] = (X)ylds[i];
+        }
+    }
+    if constexpr (POWER) {
+        pwmk::block_sum3<BS>(yy, xy, xx, red);
+        if (tid == 0) {
+            pw.partials[3 * t + 0] = yy;
+            pw.partials[3 * t + 1] = xy;
+            pw.partials[3 * t + 2] = xx;
+        }
+    }
+}
+
 template <typename VT, bool POWER>
 __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                              typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
@@ -110,7 +167,7 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     __shared__ int32_t p_start[kWin];
     __shared__ int32_t p_pre[kWin + 1];
     __shared__ int scan_tmp[kWaves];
-    __shared__ double red[16];
+    __shared__ double red[48];
 
     const X* __restrict__ x = x_arg;
     X* __restrict__ y = y_arg;
@@ -125,20 +182,9 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
     const VT* __restrict__ val = (const VT*)a.val;
     const uint32_t* __restrict__ ind = a.ind;
     const int tid = threadIdx.x;
-    // workgroup -> (slab task, column group).  With a multiple of 8 groups the
-    // launch runs in rounds of 8 groups so that XCD (b % 8) keeps one group
-    // per round: b = round * 8 * slabs_total + slab * 8 + xcd.
     int g;
     int64_t t;
-    if (a.groups % 8 == 0) {
-        const int64_t per_round = (a.n_tasks / a.groups) * 8;
-        const int64_t round = blockIdx.x / per_round, rem = blockIdx.x % per_round;
-        t = rem / 8;
-        g = (int)(round * 8 + rem % 8);
-    } else {
-        g = (int)(blockIdx.x % a.groups);
-        t = blockIdx.x / a.groups;
-    }
+    csb_task(a, g, t);
     const int64_t br = t / a.slabs;
     const int s = (int)(t % a.slabs);
     const int64_t beta = (int64_t)1 << a.bits;
@@ -207,42 +253,79 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
         }
     }
     __syncthreads();
+    csb_write_slab<X, Acc, kBS, POWER>(a, ylds, h, br * beta + slab_base, g, t, x, y, scale, pw, red);
+}
 
-    const int64_t row0 = br * beta + slab_base;
-    if (a.groups > 1) {  // partial slab sums; k_csb_reduce finishes y
-        Acc* yp = (Acc*)a.ypart + (int64_t)g * a.nor;
-        for (int i = tid; i < h; i += kBS) {
-            const int64_t row = row0 + i;
-            if (row >= a.nor) break;
-            yp[row] = ylds[i];
-        }
-        return;
-    }
-    double yy = 0.0, xy = 0.0, xx = 0.0;
-    for (int i = tid; i < h; i += kBS) {
-        const int64_t row = row0 + i;
-        if (row >= a.nor) break;
-        if constexpr (POWER) {
-            const X out = (X)((double)ylds[i] * scale);
-            y[row] = out;
-            const double yd = (double)out, xd = (double)x[row] * scale;
-            yy += yd * yd;
-            xy += xd * yd;
-            xx += xd * xd;
-        } else {
-            y[row] = (X)ylds[i];
-        }
-    }
+// Slab-stream CSB: the workgroup's entries are one contiguous run
+// [soff[t*H + bc_lo], soff[t*H + bc_hi]) read as aligned groups of 4 (128-bit
+// index loads, 128-bit value loads); every entry adds val * x[col] into the
+// slab's LDS rows.  Lanes past the run reload its last group and add nothing,
+// so the loads carry no branch.
+constexpr int kSBS = 256;
+
+// kSU: groups of 4 entries per thread per step (2 or 4)
+template <typename VT, bool POWER, int kSU>
+__global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
+                                                     typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    extern __shared__ __align__(16) unsigned char csb_smem[];
+    Acc* ylds = (Acc*)csb_smem;
+    __shared__ double red[48];
+
+    const X* __restrict__ x = x_arg;
+    X* __restrict__ y = y_arg;
+    double scale = 1.0;
     if constexpr (POWER) {
-        const double ra = pwmk::block_sum<kBS>(yy, red);
-        const double rb = pwmk::block_sum<kBS>(xy, red);
-        const double rc = pwmk::block_sum<kBS>(xx, red);
-        if (tid == 0) {
-            pw.partials[3 * t + 0] = ra;
-            pw.partials[3 * t + 1] = rb;
-            pw.partials[3 * t + 2] = rc;
-        }
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = pwmk::step_scale(pw, k);
     }
+    const VT* __restrict__ val = (const VT*)a.val;
+    const uint32_t* __restrict__ ind = a.ind;
+    const int tid = threadIdx.x;
+    int g;
+    int64_t t;
+    csb_task(a, g, t);
+    const int h = (int)a.slab_rows;
+    for (int i = tid; i < h; i += kSBS) ylds[i] = 0;
+    const int64_t bc_lo = g * a.H / a.groups, bc_hi = (g + 1) * a.H / a.groups;
+    // entry offsets fit 31 bits (int32 CRS row starts)
+    const int32_t e0 = a.soff[t * a.H + bc_lo], e1 = a.soff[t * a.H + bc_hi];
+    const int cb = a.col_bits;
+    const uint32_t cmask = (uint32_t)((1ull << cb) - 1);
+    __syncthreads();
+    const int32_t q0 = e0 >> 2, q1 = (e1 + 3) >> 2;
+    for (int32_t qb = q0; qb < q1; qb += kSBS * kSU) {
+        pwmk::vi4 w[kSU];
+        pwmk::Quad<VT> v[kSU];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+            const int32_t q = qb + tid + u * kSBS;
+            const int64_t qc = q < q1 ? q : q1 - 1;
+            w[u] = *pwmk::G((const pwmk::vi4*)(ind + 4 * qc));
+            v[u].load(val + 4 * qc);
+        }
+        X xv[kSU][4];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[u][j] = x[(uint32_t)w[u][j] & cmask];
+#pragma unroll
+        for (int u = 0; u < kSU; ++u)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t k = 4 * (qb + tid + u * kSBS) + j;
+                if (k >= e0 && k < e1)
+                    atomicAdd(&ylds[(uint32_t)w[u][j] >> cb], (Acc)v[u].get(j) * (Acc)xv[u][j]);
+            }
+    }
+    __syncthreads();
+    const int64_t beta = (int64_t)1 << a.bits;
+    const int64_t row0 = (t / a.slabs) * beta + (t % a.slabs) * (int64_t)h;
+    csb_write_slab<X, Acc, kSBS, POWER>(a, ylds, h, row0, g, t, x, y, scale, pw, red);
 }
 
 // y[r] = sum over the column groups of ypart[g][r], groups in order (and the
@@ -535,16 +618,35 @@ __global__ __launch_bounds__(kBS) void k_coh_stats(CohArgs a, const int32_t* __r
 // ---------------------------------------------------------------- launchers
 
 template <typename VT>
-static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s, int part) {
     using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
     if (a.n_tasks <= 0) return hipSuccess;
-    if (pw)
+    if (part == 2) {
+        // the column-group reduction only
+    } else if (a.stream) {
+        const size_t lds = (size_t)a.slab_rows * sizeof(Acc);
+        const dim3 grid((unsigned)a.n_tasks);
+        if (pw && a.unroll == 4)
+            hipLaunchKernelGGL((k_csb_stream<VT, true, 4>), grid, dim3(kSBS), lds, s, a, (const X*)nullptr,
+                               (X*)nullptr, *pw);
+        else if (pw)
+            hipLaunchKernelGGL((k_csb_stream<VT, true, 2>), grid, dim3(kSBS), lds, s, a, (const X*)nullptr,
+                               (X*)nullptr, *pw);
+        else if (a.unroll == 4)
+            hipLaunchKernelGGL((k_csb_stream<VT, false, 4>), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
+                               PowerArgs{});
+        else
+            hipLaunchKernelGGL((k_csb_stream<VT, false, 2>), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
+                               PowerArgs{});
+    } else if (pw) {
         hipLaunchKernelGGL((k_csb<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
                            (X*)nullptr, *pw);
-    else
+    } else {
         hipLaunchKernelGGL((k_csb<VT, false>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
                            PowerArgs{});
-    if (a.groups > 1) {
+    }
+    if (a.groups > 1 && part != 1) {
         const unsigned rb = (unsigned)csb_reduce_blocks(a.nor);
         if (pw)
             hipLaunchKernelGGL((k_csb_reduce<VT, true>), dim3(rb), dim3(kBS), 0, s, a, (X*)nullptr, *pw);
@@ -554,11 +656,11 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     return hipGetLastError();
 }
 
-hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s) {
+hipError_t launch_csb(const CsbArgs& a, const void* x, void* y, const PowerArgs* pw, hipStream_t s, int part) {
     switch (a.vtype) {
-        case 0: return csb_t<double>(a, x, y, pw, s);
-        case 1: return csb_t<float>(a, x, y, pw, s);
-        case 2: return csb_t<bf16_t>(a, x, y, pw, s);
+        case 0: return csb_t<double>(a, x, y, pw, s, part);
+        case 1: return csb_t<float>(a, x, y, pw, s, part);
+        case 2: return csb_t<bf16_t>(a, x, y, pw, s, part);
     }
     return hipErrorInvalidValue;
 }

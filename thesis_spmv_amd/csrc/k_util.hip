@@ -172,4 +172,69 @@ hipError_t launch_flush(void* p, int64_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Gather-rate probe: what one SpMV's x[col] gathers can reach at best.  A
+// stream of (u32 index, f64 value) entries, 12 bytes each like a CRS
+// nonzero, is read with coalesced loads and every index gathers one double
+// of x: from all of x, or (per_xcd) from the window [xcd*window,
+// (xcd+1)*window) of the XCD the workgroup runs on (b % 8), the way a CSB
+// column group keeps its eighth of x.  Indices are a hash of the entry
+// number modulo the window, so the gathers have no locality beyond it.
+__global__ void k_gather_fill(uint32_t* __restrict__ idx, double* __restrict__ val, int64_t n, uint32_t window) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t v = (uint32_t)k * 0x9e3779b9u + (uint32_t)(k >> 32);
+        v ^= v >> 16;
+        v *= 0x7feb352du;
+        v ^= v >> 15;
+        v *= 0x846ca68bu;
+        v ^= v >> 16;
+        idx[k] = v % window;
+        val[k] = 1.0;
+    }
+}
+
+constexpr int kProbeU = 8;
+constexpr int kProbePerBlock = 8192;  // entries per workgroup, a multiple of 256 * kProbeU
+
+template <bool PER_XCD>
+__global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict__ idx, const double* __restrict__ val,
+                                                      const double* __restrict__ x, double* __restrict__ out,
+                                                      int64_t window) {
+    const double* xw = PER_XCD ? x + (int64_t)(blockIdx.x % 8) * window : x;
+    const int64_t base = (int64_t)blockIdx.x * kProbePerBlock;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < kProbePerBlock; i += kProbeU * 256) {
+        uint32_t j[kProbeU];
+        double v[kProbeU];
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) {
+            j[u] = idx[base + i + u * 256];
+            v[u] = val[base + i + u * 256];
+        }
+        double xv[kProbeU];
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) xv[u] = xw[j[u]];
+#pragma unroll
+        for (int u = 0; u < kProbeU; ++u) acc += v[u] * xv[u];
+    }
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+int64_t gather_probe_entries(int64_t n) { return n / kProbePerBlock * kProbePerBlock; }
+
+hipError_t launch_gather_fill(uint32_t* idx, double* val, int64_t n, uint32_t window, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_fill, dim3(4096), dim3(256), 0, s, idx, val, n, window);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_probe(const uint32_t* idx, const double* val, const double* x, double* out, int64_t n,
+                               int64_t window, int per_xcd, hipStream_t s) {
+    const unsigned blocks = (unsigned)(n / kProbePerBlock);
+    if (blocks == 0) return hipSuccess;
+    if (per_xcd)
+        hipLaunchKernelGGL(k_gather_probe<true>, dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+    else
+        hipLaunchKernelGGL(k_gather_probe<false>, dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+    return hipGetLastError();
+}
+
 }  // namespace pwmk

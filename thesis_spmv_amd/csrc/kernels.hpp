@@ -93,5 +93,11 @@ hipError_t launch_power_reset(const PowerArgs& pw, hipStream_t s);
 hipError_t launch_flush(void* p, int64_t bytes, hipStream_t s);
 // Plain streaming copy (profiler counter calibration).
 hipError_t launch_stream_copy(void* dst, const void* src, int64_t bytes, hipStream_t s);
+// gather-rate probe: entries rounded down to whole workgroups; out holds
+// (entries / 8192) * 256 doubles
+int64_t gather_probe_entries(int64_t n);
+hipError_t launch_gather_fill(uint32_t* idx, double* val, int64_t n, uint32_t window, hipStream_t s);
+hipError_t launch_gather_probe(const uint32_t* idx, const double* val, const double* x, double* out, int64_t n,
+                               int64_t window, int per_xcd, hipStream_t s);
 
 }  // namespace pwmk

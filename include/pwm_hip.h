@@ -191,7 +191,8 @@ int pwm_time_mv(pwm_matrix* h, const void* x_dev, void* y_dev, int reps, int64_t
  * index gathers one double of x_dev (x_len doubles): from [0, window), or with
  * per_xcd from the window [j*window, (j+1)*window) of the XCD j that runs the
  * workgroup.  Indices are hashed, uniform in the window.  The bound a sparse
- * SpMV's gathers meet: n nonzeros cannot take less than n / this rate. */
+ * SpMV's gathers meet: n nonzeros cannot take less than n / this rate.
+ * window = 0: the stream alone (x_dev may be NULL), its streaming ceiling. */
 int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n, double* ms);
 
 /* ---- generators for the benchmark workloads (host side, deterministic) -

@@ -378,6 +378,7 @@ def test_gather_rate_probe():
     x = pw.DeviceVector.from_array(np.ones(1 << 16))
     assert pw.gather_rate(x, 1 << 13, True, 1 << 16) > 0
     assert pw.gather_rate(x, 1 << 16, False, 3 * 8192 + 5) > 0
+    assert pw.gather_rate(None, 0, False, 1 << 16) > 0  # the stream alone
     with pytest.raises(pw.PwmError):
         pw.gather_rate(x, 1 << 14, True, 1 << 16)  # eight windows exceed x
     with pytest.raises(pw.PwmError):

@@ -174,11 +174,13 @@ class DeviceVector:
 def gather_rate(x, window, per_xcd, n):
     """Milliseconds of one pass of the gather-rate probe (pwm_gather_rate):
     n 12-byte (index, value) entries, each gathering one double of the FP64
-    DeviceVector x from a window of `window` doubles (per XCD with per_xcd)."""
-    if x.dtype != np.float64:
+    DeviceVector x from a window of `window` doubles (per XCD with per_xcd);
+    window 0 (x may be None): the entry stream alone."""
+    if x is not None and x.dtype != np.float64:
         raise ValueError("gather_rate needs an FP64 DeviceVector")
     ms = F64()
-    _check(lib().pwm_gather_rate(x.ptr, int(x.n), int(window), int(bool(per_xcd)), int(n), C.byref(ms)))
+    _check(lib().pwm_gather_rate(x.ptr if x is not None else None, int(x.n) if x is not None else 0, int(window),
+                                 int(bool(per_xcd)), int(n), C.byref(ms)))
     return ms.value
 
 

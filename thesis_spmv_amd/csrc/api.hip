@@ -857,7 +857,7 @@ extern "C" int pwm_device_copy(void* dst, const void* src, int64_t bytes) {
 
 extern "C" int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n,
                                double* ms) {
-    if (!x_dev || !ms || window <= 0 || window > UINT32_MAX || (per_xcd ? 8 * window : window) > x_len)
+    if (!ms || window < 0 || window > UINT32_MAX || (window > 0 && (!x_dev || (per_xcd ? 8 * window : window) > x_len)))
         return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: window must fit x (8 windows when per_xcd)");
     const int64_t m = pwmk::gather_probe_entries(n);
     if (m <= 0) return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: n must be at least 8192");
@@ -866,7 +866,7 @@ extern "C" int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window,
     if ((e = idx.alloc(m * 4)) != hipSuccess || (e = val.alloc(m * 8)) != hipSuccess ||
         (e = out.alloc(m / 8192 * 256 * 8)) != hipSuccess)
         return hip_fail(nullptr, e, "pwm_gather_rate buffers");
-    e = pwmk::launch_gather_fill(idx.as<uint32_t>(), val.as<double>(), m, (uint32_t)window, nullptr);
+    e = pwmk::launch_gather_fill(idx.as<uint32_t>(), val.as<double>(), m, (uint32_t)std::max<int64_t>(window, 1), nullptr);
     hipEvent_t a = nullptr, b = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);

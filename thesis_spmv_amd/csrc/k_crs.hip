@@ -3,7 +3,10 @@
 // Rows of length <= lr.threshold: a group of G lanes (G = 1..64, a power of
 // two) per row; each lane strides over the row by G, the partial sums are
 // combined with DPP/ds_swizzle (group_sum), lane 0 writes y.  The loads of a
-// row's val/col_ind are coalesced across the group, x is gathered.
+// row's val/col_ind are coalesced across the group, x is gathered.  A lane's
+// first four entries are loaded and gathered unconditionally (clamped to the
+// row), so a row of at most 4G entries costs one round of independent loads
+// instead of a loop of dependent ones; longer rows continue in the loop.
 // Rows longer than the threshold are cut into chunks; each chunk is summed by
 // one workgroup and crs_finalize adds a row's chunk sums in chunk order, so a
 // 50 000-entry row of a power-law matrix does not hold one wave for its whole
@@ -92,7 +95,33 @@ __global__ __launch_bounds__(kBS) void k_crs_vector(CrsDev A, int n_vec_blocks, 
             }
         }
         Acc sum = 0;
-        int32_t k = s + lane;
+        // a lane's first 4 entries (all of a row of <= 4G) as independent
+        // loads; masked entries are pointed at the row's first entry and not
+        // added
+        if (e > s) {
+            int32_t kk[4];
+            bool ok[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t k = s + lane + j * G;
+                ok[j] = k < e;
+                kk[j] = ok[j] ? k : s;
+            }
+            int32_t jj[4];
+            Acc vv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                jj[j] = ci[kk[j]];
+                vv[j] = widen(val[kk[j]]);
+            }
+            Acc xv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[j] = (Acc)x[jj[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (ok[j]) sum += vv[j] * xv[j];
+        }
+        int32_t k = s + lane + 4 * G;
         for (; k + 3 * G < e; k += 4 * G) {
             const int32_t j0 = ci[k], j1 = ci[k + G], j2 = ci[k + 2 * G], j3 = ci[k + 3 * G];
             const Acc v0 = widen(val[k]), v1 = widen(val[k + G]), v2 = widen(val[k + 2 * G]),

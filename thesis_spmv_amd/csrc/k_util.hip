@@ -179,6 +179,7 @@ hipError_t launch_flush(void* p, int64_t bytes, hipStream_t s) {
 // (xcd+1)*window) of the XCD the workgroup runs on (b % 8), the way a CSB
 // column group keeps its eighth of x.  Indices are a hash of the entry
 // number modulo the window, so the gathers have no locality beyond it.
+// window = 0: the stream alone, no gathers (the streaming ceiling).
 __global__ void k_gather_fill(uint32_t* __restrict__ idx, double* __restrict__ val, int64_t n, uint32_t window) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         uint32_t v = (uint32_t)k * 0x9e3779b9u + (uint32_t)(k >> 32);
@@ -195,7 +196,7 @@ __global__ void k_gather_fill(uint32_t* __restrict__ idx, double* __restrict__ v
 constexpr int kProbeU = 8;
 constexpr int kProbePerBlock = 8192;  // entries per workgroup, a multiple of 256 * kProbeU
 
-template <bool PER_XCD>
+template <bool PER_XCD, bool GATHER>
 __global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict__ idx, const double* __restrict__ val,
                                                       const double* __restrict__ x, double* __restrict__ out,
                                                       int64_t window) {
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict
         }
         double xv[kProbeU];
 #pragma unroll
-        for (int u = 0; u < kProbeU; ++u) xv[u] = xw[j[u]];
+        for (int u = 0; u < kProbeU; ++u) xv[u] = GATHER ? xw[j[u]] : (double)j[u];
 #pragma unroll
         for (int u = 0; u < kProbeU; ++u) acc += v[u] * xv[u];
     }
@@ -230,10 +231,12 @@ hipError_t launch_gather_probe(const uint32_t* idx, const double* val, const dou
                                int64_t window, int per_xcd, hipStream_t s) {
     const unsigned blocks = (unsigned)(n / kProbePerBlock);
     if (blocks == 0) return hipSuccess;
-    if (per_xcd)
-        hipLaunchKernelGGL(k_gather_probe<true>, dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+    if (window == 0)
+        hipLaunchKernelGGL((k_gather_probe<false, false>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+    else if (per_xcd)
+        hipLaunchKernelGGL((k_gather_probe<true, true>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
     else
-        hipLaunchKernelGGL(k_gather_probe<false>, dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+        hipLaunchKernelGGL((k_gather_probe<false, true>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
     return hipGetLastError();
 }
 

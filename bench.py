@@ -239,6 +239,59 @@ def _percentiles(lat):
     return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
+def bench_coo(args):
+    """COO -> CRS (TripletToCRS, Util/TripletToCRS.hpp:135-172) of the
+    power-law matrix (N=4194304, ~67M entries) given as shuffled triplets:
+    loadFromTriplets of a CRS_Merge handle (host validation, H2D upload, the
+    device radix sort, gathers, row starts and the merge plan), against the
+    oracle's restatement of TripletToCRS on one host core.  The device arrays
+    are checked bit-exact against the oracle's."""
+    import thesis_spmv_amd as pw
+    n, rs, ci, dv, desc, gen_s = make_workload("powerlaw")
+    z = int(rs[-1])
+    t0 = time.time()
+    rows = np.repeat(np.arange(n, dtype=np.int32), np.diff(rs))
+    perm = np.random.default_rng(5).permutation(z)
+    t = pw.Triplet(n, n, rows[perm], ci[perm], dv[perm])
+    del rows, perm
+    print(f"# coo: {desc}; {z} shuffled triplets built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    small = pw.Triplet(1000, 1000, np.arange(1000), np.arange(1000), np.ones(1000))
+    pw.CRSMergeHip().loadFromTriplets(small)  # warm-up: loads the kernels
+    times = []
+    for _ in range(max(1, args.steps)):
+        m = pw.CRSMergeHip()
+        pw.synchronize()
+        t1 = time.perf_counter()
+        m.loadFromTriplets(t)
+        pw.synchronize()
+        times.append(time.perf_counter() - t1)
+        last = m
+    gpu_s = float(np.median(times))
+    grs, gci, gdv = last.crs_export()
+    del last, m
+    # CPU baseline leg (the oracle's restatement) and the parity check
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    ot = oracle.Triplets(n, n, t.row_coord, t.col_coord, t.data)
+    t1 = time.perf_counter()
+    ref = oracle.triplet_to_crs(ot)
+    cpu_s = time.perf_counter() - t1
+    exact = bool(np.array_equal(grs, ref.row_start) and np.array_equal(gci, ref.col_ind) and np.array_equal(gdv, ref.data))
+    print(f"# coo: loadFromTriplets {gpu_s * 1e3:.1f} ms (median of {len(times)}), {z / gpu_s / 1e6:.0f} M entries/s; "
+          f"oracle TripletToCRS {cpu_s * 1e3:.0f} ms; arrays bit-exact: {exact}", file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "value": round(z / gpu_s / 1e6, 1), "unit": "M entries/s", "n_gpus": 1,
+           "steps": len(times), "warmup": 1, "ms_per_step": round(gpu_s * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32 keys, f64 values", "data": "synthetic",
+           "config": {"workload": "COO -> CRS of the power-law matrix (N=4194304), shuffled triplets, loadFromTriplets",
+                      "entries": z, "bit_exact_vs_oracle": exact},
+           "cpu_baseline": {"value": round(z / cpu_s / 1e6, 1), "unit": "M entries/s", "cores": 1, "kind": "port",
+                            "sample": f"the oracle's TripletToCRS restatement on all {z} triplets, one call, "
+                                      f"{cpu_s * 1e3:.0f} ms"}}
+    print(json.dumps(out), flush=True)
+    if not exact:
+        sys.exit(1)
+
+
 def bench_serve(args):
     """Served stream: requests arrive as a Poisson process and are served by
     the continuous-batching server (FP32 storage, FP64 reductions, 64 GB
@@ -391,6 +444,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "coo":
+        return bench_coo(args)
     if args.calibrate_copy > 0:
         nb = args.calibrate_copy // 16 * 16
         src, dst = pw.DeviceVector(nb // 8), pw.DeviceVector(nb // 8)

@@ -95,6 +95,68 @@ def _powerlaw(n, seed=1, cap=50000):
     return oracle.CRSMatrix(n, n, rs, ci, dv)
 
 
+@pytest.mark.parametrize("values", ["f64", "f32"])
+def test_triplets_device_sort_large_shuffled(values):
+    """A 1.8M-entry power-law matrix as shuffled COO with 5000 duplicated
+    entries (new values, so their order shows): the device sort's CRS arrays
+    equal the oracle's TripletToCRS, for every format that keeps CRS."""
+    rng = np.random.default_rng(7)
+    A = _powerlaw(120000, cap=20000)
+    rows = np.repeat(np.arange(A.nor, dtype=np.int32), np.diff(A.row_start))
+    dup = rng.integers(0, A.nnz, 5000)
+    r = np.concatenate([rows, rows[dup]])
+    c = np.concatenate([A.col_ind, A.col_ind[dup]])
+    v = np.concatenate([A.data, rng.standard_normal(5000)])
+    perm = rng.permutation(len(r))
+    t = oracle.Triplets(A.nor, A.noc, r[perm], c[perm], v[perm])
+    ref = oracle.triplet_to_crs(t)
+    want = ref.data if values == "f64" else ref.data.astype(np.float32).astype(np.float64)
+    for cls in (pw.CRSHip, pw.CRSMergeHip):
+        h = cls(-1, values)
+        h.loadFromTriplets(t)
+        rs, ci, dv = h.crs_export()
+        assert np.array_equal(rs, ref.row_start) and np.array_equal(ci, ref.col_ind), cls.__name__
+        assert np.array_equal(dv, want), cls.__name__
+        x = pw.gen_uniform(A.noc, 2).astype(np.float64)
+        y = np.zeros(A.nor)
+        h.mv(x, y)
+        tol = 1e-13 if values == "f64" else 1e-5
+        assert _relerr(y, oracle.crs_mv(ref, x)) <= tol
+
+
+def test_triplets_device_sort_edge_shapes():
+    """Empty matrices, 1 x 1, a single column (0-bit column keys), all
+    entries in the last row, a wide matrix; rows given out of order."""
+    cases = [(5, 7, [], [], []),
+             (1, 1, [0], [0], [3.0]),
+             (6, 1, [5, 0, 5, 2], [0, 0, 0, 0], [1.0, 2.0, 3.0, 4.0]),
+             (9, 4, [8, 8, 8, 8, 8], [3, 1, 2, 1, 0], [1.0, 2.0, 3.0, 4.0, 5.0]),
+             (3, 70000, [2, 0, 2, 1, 0], [69999, 5, 0, 40000, 5], [1.0, -2.0, 3.0, 4.0, 0.5])]
+    for nor, noc, r, c, v in cases:
+        t = oracle.Triplets(nor, noc, r, c, v)
+        ref = oracle.triplet_to_crs(t)
+        h = pw.CRSHip()
+        h.loadFromTriplets(t)
+        rs, ci, dv = h.crs_export()
+        assert np.array_equal(rs, ref.row_start) and np.array_equal(ci[:ref.nnz], ref.col_ind), (nor, noc)
+        assert np.array_equal(dv[:ref.nnz], ref.data), (nor, noc)
+
+
+def test_triplets_errors_name_the_first_bad_entry():
+    h = pw.CRSHip()
+    bad = oracle.Triplets(4, 4, [0, 1, 2, 4, 9], [0, 0, 0, 0, 0], [1.0] * 5)
+    with pytest.raises(pw.PwmError, match="entry 3"):
+        h.loadFromTriplets(bad)
+    neg = oracle.Triplets(4, 4, [0, 1, 2, 3], [0, -1, 0, 7], [1.0] * 4)
+    with pytest.raises(pw.PwmError, match="entry 1"):
+        h.loadFromTriplets(neg)
+    hb = pw.CRSHip(-1, "bf16")
+    with pytest.raises(pw.PwmError, match="exact in BF16"):
+        hb.loadFromTriplets(oracle.Triplets(3, 3, [0, 1, 2], [0, 1, 2], [4.0, -1.0, 0.1]))
+    hb.loadFromTriplets(oracle.Triplets(3, 3, [2, 1, 0], [0, 1, 2], [4.0, -1.0, 0.5]))
+    assert hb.info()["nnz"] == 3
+
+
 @pytest.mark.parametrize("threads", [1, 2, 7, 64, 1000, -1])
 def test_merge_splits_bit_exact(threads):
     A = _powerlaw(20000)

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -500,8 +501,10 @@ extern "C" int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32
 }
 
 // COO -> CRS (Util/TripletToCRS.hpp:135-172): rows ascending, columns
-// ascending inside a row, duplicates kept in input order.  Counting sort by
-// row, then a stable sort of each row by column.
+// ascending inside a row, duplicates kept in input order.  The indices are
+// validated on the host (in parallel); the sort runs on the device
+// (coo_to_crs_device).  The blocked formats, whose builders run on the host,
+// then take the sorted arrays back.
 extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows,
                                  const int32_t* cols, const double* vals, int partitions) {
     (void)partitions;
@@ -509,43 +512,69 @@ extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_
     if (nor <= 0 || noc <= 0 || nnz < 0 || nor > INT32_MAX - 1 || noc > INT32_MAX || nnz > INT32_MAX)
         return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: need 0 < nor, noc < 2^31 and 0 <= nnz < 2^31");
     if (nnz > 0 && (!rows || !cols || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: NULL arrays");
-    std::vector<int32_t> rs(nor + 1, 0);
-    for (int64_t i = 0; i < nnz; ++i) {
-        if (rows[i] < 0 || rows[i] >= nor || cols[i] < 0 || cols[i] >= noc)
-            return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: index out of range at entry " + std::to_string(i));
-        rs[rows[i] + 1]++;
-    }
-    for (int64_t r = 0; r < nor; ++r) rs[r + 1] += rs[r];
-    std::vector<int32_t> ci(std::max<int64_t>(nnz, 1));
-    std::vector<double> dv(std::max<int64_t>(nnz, 1));
-    {
-        std::vector<int32_t> pos(rs.begin(), rs.end() - 1);
-        for (int64_t i = 0; i < nnz; ++i) {
-            const int32_t p = pos[rows[i]]++;
-            ci[p] = cols[i];
-            dv[p] = vals[i];
+    // first out-of-range entry and first value not exact in BF16 (when BF16 storage)
+    std::atomic<int64_t> bad_idx{INT64_MAX}, bad_val{INT64_MAX};
+    const bool bf16 = h->vtype == PWM_VALUES_BF16;
+    parallel_for(nnz, 1 << 16, [&](int64_t i0, int64_t i1, int) {
+        for (int64_t i = i0; i < i1; ++i) {
+            if (rows[i] < 0 || rows[i] >= nor || cols[i] < 0 || cols[i] >= noc) {
+                int64_t cur = bad_idx.load();
+                while (i < cur && !bad_idx.compare_exchange_weak(cur, i)) {
+                }
+                break;
+            }
         }
-    }
-    std::vector<std::pair<int32_t, double>> tmp;
-    for (int64_t r = 0; r < nor; ++r) {
-        const int32_t s = rs[r], e = rs[r + 1];
-        if (e - s < 2) continue;
-        bool sorted = true;
-        for (int32_t k = s + 1; k < e && sorted; ++k) sorted = ci[k - 1] <= ci[k];
-        if (sorted) continue;
-        tmp.resize(e - s);
-        for (int32_t k = s; k < e; ++k) tmp[k - s] = {ci[k], dv[k]};
-        std::stable_sort(tmp.begin(), tmp.end(),
-                         [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
-                             return a.first < b.first;
-                         });
-        for (int32_t k = s; k < e; ++k) {
-            ci[k] = tmp[k - s].first;
-            dv[k] = tmp[k - s].second;
+        if (!bf16) return;
+        for (int64_t i = i0; i < i1; ++i) {
+            const float f = (float)vals[i];
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            if ((double)f != vals[i] || (b & 0xffffu)) {
+                int64_t cur = bad_val.load();
+                while (i < cur && !bad_val.compare_exchange_weak(cur, i)) {
+                }
+                break;
+            }
         }
-    }
+    });
+    if (bad_idx.load() != INT64_MAX)
+        return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: index out of range at entry " + std::to_string(bad_idx.load()));
+    if (bad_val.load() != INT64_MAX)
+        return fail(h, PWM_ERR_INVALID,
+                    "BF16 storage needs values exact in BF16 (entry " + std::to_string(bad_val.load()) + ")");
     reset_matrix(h);
-    return load_crs_host(h, nor, noc, rs.data(), ci.data(), dv.data());
+    h->nor = nor;
+    h->noc = noc;
+    h->nnz = nnz;
+    const bool f64 = h->vtype == PWM_VALUES_F64;
+    DevBuf dtmp;
+    int rc = coo_to_crs_device(h, nor, noc, nnz, rows, cols, vals, h->rs, h->ci, f64 ? h->val : dtmp);
+    if (rc != PWM_OK) return rc;
+    hipError_t e;
+    if (!f64) {
+        if ((e = alloc_padded(h->val, nnz, value_size(h->vtype))) != hipSuccess)
+            return hip_fail(h, e, "value allocation");
+        if (nnz > 0) {
+            e = pwmk::launch_convert_values(dtmp.as<double>(), h->val.p, nnz, h->vtype, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) return hip_fail(h, e, "value conversion");
+        }
+    }
+    std::vector<int32_t> hrs(nor + 1);
+    if ((e = hipMemcpy(hrs.data(), h->rs.p, (nor + 1) * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(h, e, "row_start download");
+    h->h_rs = hrs;
+    if (h->format == PWM_FORMAT_CSB || h->format == PWM_FORMAT_BLOCKCOH) {
+        std::vector<int32_t> hci(std::max<int64_t>(nnz, 1));
+        std::vector<double> hdv(std::max<int64_t>(nnz, 1));
+        const DevBuf& dsrc = f64 ? h->val : dtmp;
+        if (nnz && (e = hipMemcpy(hci.data(), h->ci.p, nnz * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(h, e, "CRS download");
+        if (nnz && (e = hipMemcpy(hdv.data(), dsrc.p, nnz * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(h, e, "CRS download");
+        return finish_load(h, hrs.data(), hci.data(), hdv.data());
+    }
+    return finish_load(h, nullptr, nullptr, nullptr);
 }
 
 // SparseMatrix::generatePoissonMatrix: CRS filled on the device

@@ -183,6 +183,10 @@ struct pwm_matrix {
 namespace pwm_impl {
 
 int fail(pwm_matrix* h, int code, const std::string& msg);
+// COO -> CRS on the device (coo.hip): row_start, padded col_ind and padded
+// FP64 values of the sorted matrix.  Indices must already be validated.
+int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                      const double* vals, DevBuf& rs, DevBuf& ci, DevBuf& dv);
 int hip_fail(pwm_matrix* h, hipError_t e, const char* what);
 size_t value_size(int vtype);
 size_t vector_size(int vtype);

@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
         for (int u = 0; u < kSU; ++u)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int32_t k = 4 * (qb + tid + u * kSBS) + j;
+                const int64_t k = 4 * (int64_t)(qb + tid + u * kSBS) + j;
                 if (k >= e0 && k < e1)
                     atomicAdd(&ylds[(uint32_t)w[u][j] >> cb], (Acc)v[u].get(j) * (Acc)xv[u][j]);
             }

@@ -163,6 +163,18 @@ def bench_batch(args):
     out["achieved_GBps"] = round(traffic / wall / 1e9, 1)
     print(f"# algorithmic traffic {traffic / 1e12:.2f} TB over all steps -> {traffic / wall / 1e9:.0f} GB/s",
           file=sys.stderr, flush=True)
+    # one step with every instance active: 128 steps without the stop test
+    # (after a first such solve that captures their graphs)
+    full = sum((vb + 4) * z + 4 * (n + 1) + 2 * xb * n for n, z in sizes)
+    b.solve(128, -1.0, seed=0)
+    pw.synchronize()
+    t2 = time.perf_counter()
+    b.solve(128, -1.0, seed=0)
+    pw.synchronize()
+    step_ms = (time.perf_counter() - t2) * 1e3 / 128
+    out["full_step"] = {"ms": round(step_ms, 3), "GB": round(full / 1e9, 2), "GBps": round(full / step_ms / 1e6, 1)}
+    print(f"# one step with all {n_inst} instances active: {step_ms:.3f} ms for {full / 1e9:.2f} GB "
+          f"({full / step_ms / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
     if not args.no_baseline:
         # plain FP64 per-matrix GPU loop, same start vectors
         lam64 = np.zeros(n_inst)
@@ -423,7 +435,7 @@ def main():
                     help="bytes: run the library's streaming copy kernel once first (profiler byte-counter reference)")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-values", default="f32")
-    ap.add_argument("--batch-ipt", type=int, default=8, help="merge items per thread of a batch tile (4, 8, 16)")
+    ap.add_argument("--batch-ipt", type=int, default=16, help="merge items per thread of a batch tile (4, 8, 16)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="batch steps per graph replay")
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)

@@ -257,7 +257,7 @@ int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_t seed, dou
 int pwm_batch_start_vector(pwm_batch* b, int32_t id, uint64_t seed, double* out);
 int pwm_batch_vector(pwm_batch* b, int32_t id, double* out, double* ynorm);
 int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_t* nnz);
-/* "items_per_thread" (4, 8, 16: tile = 256 x items merge items),
+/* "items_per_thread" (4, 8, 16, default 16: tile = 256 x items merge items),
  * "chunk" (steps per graph replay, 1..4096), "graph" (0: plain launches),
  * "persistent" (0: one workgroup per tile instead of resident workgroups
  * that prefetch their next tile). */
@@ -283,7 +283,7 @@ int pwm_server_destroy(pwm_server* s);
 const char* pwm_server_last_error(const pwm_server* s);
 /* "chunk" (steps per pwm_server_step, default 8), "graph" (0: plain
  * launches), "persistent" (as for pwm_batch_set_option), "items_per_thread"
- * (4, 8, 16; only with nothing resident). */
+ * (4, 8, 16, default 16; only with nothing resident). */
 int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
 #define PWM_SERVER_BORROW 1 /* keep the caller's arrays (alive until removed) instead of a copy */
 int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* row_start, const int32_t* col_ind,

@@ -47,7 +47,7 @@ struct pwm_batch {
     int64_t rows = 0, nnz = 0;
     bool uploaded = false;
     int planned_ipt = 0;
-    int ipt = 8;
+    int ipt = 16;
     int chunk = 32;
     int use_graph = 1;
     int persistent = 0;

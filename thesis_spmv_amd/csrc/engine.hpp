@@ -165,7 +165,7 @@ class Engine {
    private:
     int build_lists(std::string& err);
     int ensure_capacity(std::string& err);
-    int vtype_ = 0, ipt_ = 8, max_slots_ = 0;
+    int vtype_ = 0, ipt_ = 16, max_slots_ = 0;
     hipStream_t stream_ = nullptr;
     pwmk::EJob* h_jobs_ = nullptr;      // pinned mirrors
     pwmk::EState* h_state_ = nullptr;

@@ -122,14 +122,18 @@ __device__ __forceinline__ void tile_load(TileRegs<VT, ITEMS>& R, const int32_t*
 template <typename VT, int BS, int ITEMS>
 __device__ __forceinline__ void stream_xrows(const typename Arith<VT>::X* __restrict__ x, int64_t a0, int na,
                                              typename Arith<VT>::X* xr) {
-    // every u is loaded (clamped to the last row) so that the loads need no
-    // per-lane branch; an empty tile (a0 may be n) loads nothing
+    // the first ceil(na/BS) slices only (a workgroup-uniform bound), each
+    // index clamped to the last row so that the loads need no per-lane
+    // branch; an empty tile (a0 may be n) loads nothing
     if (na == 0) return;
     const int last = na - 1;
+    const int nu = (na + BS - 1) / BS;
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
-        const int r = (int)threadIdx.x + u * BS;
-        xr[u] = G(x)[a0 + (r < last ? r : last)];
+        if (u < nu) {
+            const int r = (int)threadIdx.x + u * BS;
+            xr[u] = G(x)[a0 + (r < last ? r : last)];
+        }
     }
 }
 
@@ -147,13 +151,14 @@ __device__ __forceinline__ void tile_stage(const TileRegs<VT, ITEMS>& R, const i
     constexpr int VU = TileRegs<VT, ITEMS>::VU;
     const int tid = threadIdx.x;
     const int nu = (na + BS - 1) / BS;
-    // clamped to the last row end so that the loads need no per-lane branch;
-    // an empty tile (a0 may be n) loads no row end
+    // the first nu slices only (a workgroup-uniform bound), clamped to the
+    // last row end so that the loads need no per-lane branch; an empty tile
+    // (a0 may be n) loads no row end
     const int last = na > 0 ? na - 1 : 0;
     int32_t re[ITEMS];
-    if (na > 0) {
 #pragma unroll
-        for (int u = 0; u < ITEMS; ++u) {
+    for (int u = 0; u < ITEMS; ++u) {
+        if (u < nu) {
             const int i = tid + u * BS;
             re[u] = G(rs)[a0 + 1 + (i < last ? i : last)];
         }
@@ -207,8 +212,17 @@ __device__ __forceinline__ void stream_compute(typename Arith<VT>::X* __restrict
         if (u < nu && r < na) {
             const int beg = r == 0 ? 0 : (int)(sm.rowend[r - 1] - b0);
             const int end = (int)(sm.rowend[r] - b0);
+            // left to right, four LDS reads in flight at a time
             Acc s = 0;
-            for (int k = beg; k < end; ++k) s += sm.prod[k];
+            int k = beg;
+            for (; k + 4 <= end; k += 4) {
+                const Acc p0 = sm.prod[k], p1 = sm.prod[k + 1], p2 = sm.prod[k + 2], p3 = sm.prod[k + 3];
+                s += p0;
+                s += p1;
+                s += p2;
+                s += p3;
+            }
+            for (; k < end; ++k) s += sm.prod[k];
             if constexpr (POWER) {
                 const double xd = (double)xr[u] * scale;
                 const X out = (X)((double)s * scale);

@@ -60,7 +60,7 @@ struct SrvJob {
 
 struct pwm_server {
     int vtype = PWM_VALUES_F32;
-    int ipt = 8;
+    int ipt = 16;
     int chunk = 8;
     int use_graph = 1;
     hipStream_t stream = nullptr;

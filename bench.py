@@ -498,6 +498,7 @@ def main():
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
         inf = m.info()
+        mstats = m.stats()
         # parity spot check of this run's result against the first format's
         yv = y.download()
         results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
@@ -622,6 +623,7 @@ def main():
     }
     if gather:
         out["gather_probe"] = gather
+    out["matrix_stats"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in mstats.items()}
     if f32:
         out["f32"] = f32
     if not args.no_cpu:

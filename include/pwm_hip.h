@@ -111,6 +111,27 @@ int pwm_power_iteration(pwm_matrix* h, double* x, double* y, int64_t it);
 
 int pwm_info(const pwm_matrix* h, pwm_info_t* out);
 
+/* Matrix statistics (Util_Scripts/calc_stats.cpp:44-105), computed when the
+ * matrix is loaded (block counts on the device):
+ *   rows_avg = nnz/N, rows_var = sum_i (len_i - rows_avg)^2 / N;
+ *   block_bits = min(16, 3 + ceil(log2 sqrt(N))); nb_blocks whole blocks per
+ *   side (ceil(N / 2^bits), one fewer when 2^bits does not divide N);
+ *   block_avg, block_var over those nb_blocks^2 blocks; balanced = no whole
+ *   block row holds more than twice the mean block-row count.
+ * Entries whose column block lies beyond the square's blocks are not
+ * counted.  row_max (the longest row) is an addition.  The block statistics
+ * are left 0 (nb_blocks = -1) above 8192 blocks per side.  The autotuners
+ * read row_max.  Needs a loaded matrix. */
+typedef struct {
+    int64_t nor, nnz;
+    double rows_avg, rows_var;
+    int64_t row_max;
+    int32_t block_bits, balanced;
+    int64_t nb_blocks;
+    double block_avg, block_var;
+} pwm_stats_t;
+int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
+
 /* Options (addition).  Keys: "group_size" (CRS vector kernel lanes per row,
  * 0 = automatic), "long_row" (CRS: rows longer than this use the long-row
  * path), "items_per_thread" (merge path), "block_bits" (CSB beta override),

@@ -75,6 +75,9 @@ def lib():
             "orc_blockcoh_hilbert_size": (I64, [I64, I64]),
             "orc_blockcoh_build_band": (None, [I64, i32p, i32p, f64p, I64, I64, C.c_void_p]),
             "orc_blockcoh_mv_band": (None, [C.c_void_p, I64, I64, f64p, f64p]),
+            "orc_matrix_stats": (None, [I64, i32p, i32p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                        C.POINTER(I64), C.POINTER(C.c_int), C.POINTER(I64),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -323,3 +326,15 @@ def blockcoh_mv(bands, nor, x):
 def norm2(x):
     """Util/VectorUtill.hpp:23-26."""
     return lib().orc_norm2(np.ascontiguousarray(x, np.float64), len(x))
+
+
+# ---------------------------------------------------------------- statistics
+def matrix_stats(A):
+    """Util_Scripts/calc_stats.cpp:44-105 on CRS arrays (see pwm_oracle.c)."""
+    ra, rv, bav, bvr = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    rmax, nb = I64(), I64()
+    bits, bal = C.c_int(), C.c_int()
+    lib().orc_matrix_stats(A.nor, A.row_start, A.col_ind, C.byref(ra), C.byref(rv), C.byref(rmax), C.byref(bits),
+                           C.byref(nb), C.byref(bav), C.byref(bvr), C.byref(bal))
+    return {"rows_avg": ra.value, "rows_var": rv.value, "row_max": rmax.value, "block_bits": bits.value,
+            "nb_blocks": nb.value, "block_avg": bav.value, "block_var": bvr.value, "balanced": bal.value}

@@ -26,6 +26,9 @@
  *   hilbert                        Wikipedia xy2d known answers for n = 2 and 4.
  *   blockcoh_*                     brute-force band scan, mv goldens, ICRS decode
  *                                  round trip.
+ *   matrix_stats                   numpy brute force of every statistic, the
+ *                                  Poisson row-length closed form, hand-made
+ *                                  balanced / unbalanced block rows.
  */
 #include "pwm_oracle.h"
 
@@ -676,4 +679,68 @@ void orc_blockcoh_mv_band(const orc_coh_band* b, int64_t band_start, int64_t ban
             if (block_col >= H) { block_col -= H; block_row += b->row_jump_block[rbi++]; }
         }
     }
+}
+
+/* ======================= matrix statistics ============================= */
+
+/* Util_Scripts/calc_stats.cpp:44-105 on a CRS matrix (the same multiset of
+ * (row, col) pairs as the Triplet it reads):
+ *   rows_avg = nnz / N; rows_var = sum_i (len_i - rows_avg)^2 / N, summed
+ *   over the rows in order;
+ *   block_bits = min(16, 3 + ceil(log2(sqrt(N)))), nb = ceil(N / 2^bits)
+ *   block rows and columns (square, as the tool assumes), one fewer when
+ *   2^bits does not divide N (whole blocks only);
+ *   block_avg = sum_ij nnz_ij / nb^2 and block_var = sum_ij (nnz_ij -
+ *   block_avg)^2 / nb^2 over the whole blocks, i-major;
+ *   balanced = no whole block row holds more than twice the mean block-row
+ *   count (sum_i row_i / nb).
+ * Entries whose column block lies beyond the square's blocks are not counted
+ * (the tool would index past its block table there).  row_max (the longest
+ * row) is an addition. */
+void orc_matrix_stats(int64_t nor, const int32_t* row_start, const int32_t* col_ind, double* rows_avg,
+                      double* rows_var, int64_t* row_max, int* block_bits, int64_t* nb_blocks, double* block_avg,
+                      double* block_var, int* balanced) {
+    const int64_t nnz = row_start[nor];
+    const double avg = (double)nnz / (double)nor;
+    int lg = (int)ceil(log2(sqrt((double)nor)));
+    int bits = 16 < 3 + lg ? 16 : 3 + lg;
+    int64_t bs = (int64_t)1 << bits;
+    int64_t nb = orc_integer_ceil(nor, bs);
+    int64_t* blk = (int64_t*)calloc((size_t)(nb * nb > 0 ? nb * nb : 1), sizeof(int64_t));
+    double var = 0.0;
+    int64_t mx = 0;
+    for (int64_t i = 0; i < nor; ++i) {
+        const int64_t len = row_start[i + 1] - row_start[i];
+        var += pow((double)len - avg, 2) / (double)nor;
+        if (len > mx) mx = len;
+        for (int64_t k = row_start[i]; k < row_start[i + 1]; ++k) {
+            const int64_t bc = col_ind[k] / bs;
+            if (bc < nb) blk[(i / bs) * nb + bc]++;
+        }
+    }
+    const int64_t nbw = nor % bs != 0 ? nb - 1 : nb;
+    double bavg = 0.0, brow_avg = 0.0, bvar = 0.0;
+    int bal = 1;
+    int64_t* brow = (int64_t*)calloc((size_t)(nbw > 0 ? nbw : 1), sizeof(int64_t));
+    for (int64_t i = 0; i < nbw; ++i) {
+        for (int64_t j = 0; j < nbw; ++j) {
+            brow[i] += blk[i * nb + j];
+            bavg += (double)blk[i * nb + j] / pow((double)nbw, 2);
+        }
+        brow_avg += (double)brow[i] / (double)nbw;
+    }
+    for (int64_t i = 0; i < nbw; ++i) {
+        if ((double)brow[i] > 2 * brow_avg) bal = 0;
+        for (int64_t j = 0; j < nbw; ++j) bvar += pow((double)blk[i * nb + j] - bavg, 2) / pow((double)nbw, 2);
+    }
+    free(brow);
+    free(blk);
+    *rows_avg = avg;
+    *rows_var = var;
+    *row_max = mx;
+    *block_bits = bits;
+    *nb_blocks = nbw;
+    *block_avg = bavg;
+    *block_var = bvar;
+    *balanced = bal;
 }

@@ -142,6 +142,11 @@ void orc_blockcoh_build_band(int64_t noc, const int32_t* row_start, const int32_
 void orc_blockcoh_mv_band(const orc_coh_band* b, int64_t band_start, int64_t band_rows,
                           const double* x, double* y);                 /* :235-256, 479-510 */
 
+/* ---- matrix statistics: Util_Scripts/calc_stats.cpp:44-105 ------------ */
+void orc_matrix_stats(int64_t nor, const int32_t* row_start, const int32_t* col_ind, double* rows_avg,
+                      double* rows_var, int64_t* row_max, int* block_bits, int64_t* nb_blocks, double* block_avg,
+                      double* block_var, int* balanced);
+
 #ifdef __cplusplus
 }
 #endif

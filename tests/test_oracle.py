@@ -474,3 +474,95 @@ def test_blockcoh_beta_integer_form():
             k += 1
         assert L.orc_blockcoh_beta_bits(M, 1) == min(15, 3 + k)
         assert L.orc_blockcoh_hilbert_size(M, 1) == 1 << (M - 1).bit_length()
+
+
+# ---------------------------------------------------------------- matrix statistics
+def _stats_brute(A):
+    """Util_Scripts/calc_stats.cpp:44-105 written out again with numpy block
+    counts and Python loops in the tool's order."""
+    import math
+    N = A.nor
+    lens = np.diff(A.row_start).astype(np.int64)
+    avg = A.nnz / N
+    var = 0.0
+    for ln in lens:
+        var += (float(ln) - avg) ** 2 / N
+    bits = min(16, 3 + int(math.ceil(math.log2(math.sqrt(N)))))
+    bs = 1 << bits
+    nb = -(-N // bs)
+    rows = np.repeat(np.arange(N), lens)
+    br, bc = rows // bs, A.col_ind.astype(np.int64) // bs
+    keep = bc < nb
+    hist = np.zeros((nb, nb), np.int64)
+    np.add.at(hist, (br[keep], bc[keep]), 1)
+    nbw = nb - 1 if N % bs else nb
+    bavg = 0.0
+    brow_avg = 0.0
+    brow = hist[:nbw, :nbw].sum(axis=1)
+    for i in range(nbw):
+        for j in range(nbw):
+            bavg += float(hist[i, j]) / float(nbw) ** 2
+        brow_avg += float(brow[i]) / nbw
+    bvar = 0.0
+    for i in range(nbw):
+        for j in range(nbw):
+            bvar += (float(hist[i, j]) - bavg) ** 2 / float(nbw) ** 2
+    bal = int(all(brow[i] <= 2 * brow_avg for i in range(nbw)))
+    return {"rows_avg": avg, "rows_var": var, "row_max": int(lens.max()) if N else 0, "block_bits": bits,
+            "nb_blocks": nbw, "block_avg": bavg, "block_var": bvar, "balanced": bal}
+
+
+@pytest.mark.parametrize("case", ["poisson", "powerlaw", "rect", "tiny", "empty_rows"])
+def test_matrix_stats_brute_force(case):
+    rng = np.random.default_rng(11)
+    if case == "poisson":
+        A = oracle.poisson_crs(70, 53)
+    elif case == "powerlaw":
+        n = 5000
+        lens = np.minimum((3.0 * rng.random(n) ** (-1 / 1.8)).astype(np.int64), 900)
+        rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = np.concatenate([np.sort(rng.choice(n, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+        A = oracle.CRSMatrix(n, n, rs, ci, np.ones(int(rs[-1])))
+    elif case == "rect":  # columns beyond the square's blocks are not counted
+        n, nc = 3000, 9000
+        lens = rng.integers(0, 12, n)
+        rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = np.concatenate([np.sort(rng.choice(nc, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+        A = oracle.CRSMatrix(n, nc, rs, ci, np.ones(int(rs[-1])))
+    elif case == "tiny":
+        A = oracle.CRSMatrix(3, 3, [0, 1, 3, 4], [0, 0, 2, 1], [1.0] * 4)
+    else:
+        rs = np.array([0, 0, 0, 5, 5, 9] + [9] * 2000, np.int32)
+        A = oracle.CRSMatrix(len(rs) - 1, len(rs) - 1, rs, [1, 2, 3, 4, 5, 0, 7, 8, 9], np.ones(9))
+    got, want = oracle.matrix_stats(A), _stats_brute(A)
+    assert got == want, (got, want)
+
+
+def test_matrix_stats_poisson_closed_form():
+    """Row lengths of the m x n Poisson matrix: 4 corners of 3, 2(m-2)+2(n-2)
+    edge rows of 4, (m-2)(n-2) interior rows of 5."""
+    for m, n in ((5, 5), (40, 17), (1024, 1024)):
+        A = oracle.poisson_crs(m, n)
+        N = m * n
+        counts = {3: 4, 4: 2 * (m - 2) + 2 * (n - 2), 5: (m - 2) * (n - 2)}
+        avg = sum(k * c for k, c in counts.items()) / N
+        var = sum(c * (k - avg) ** 2 for k, c in counts.items()) / N
+        s = oracle.matrix_stats(A)
+        assert s["rows_avg"] == A.nnz / N and s["row_max"] == 5
+        assert abs(s["rows_var"] - var) <= 1e-12 * var
+        assert s["balanced"] == 1
+
+
+def test_matrix_stats_balance_flag():
+    """A matrix whose first block row holds most entries is unbalanced; the
+    same entries spread over every block row are balanced."""
+    N = 4096  # bits = min(16, 3 + 6) = 9: 8 whole block rows of 512
+    rows = np.concatenate([np.repeat(np.arange(512), 40), np.arange(512, N)])
+    cols = np.arange(len(rows)) % N
+    order = np.lexsort((cols, rows))
+    rs = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=N))]).astype(np.int32)
+    A = oracle.CRSMatrix(N, N, rs, cols[order], np.ones(len(rows)))
+    s = oracle.matrix_stats(A)
+    assert s["block_bits"] == 9 and s["nb_blocks"] == 8 and s["balanced"] == 0
+    B = oracle.poisson_crs(64, 64)
+    assert oracle.matrix_stats(B)["balanced"] == 1

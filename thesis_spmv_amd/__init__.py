@@ -33,6 +33,11 @@ class PwmError(RuntimeError):
         self.code = code
 
 
+class Stats(C.Structure):
+    _fields_ = [("nor", I64), ("nnz", I64), ("rows_avg", F64), ("rows_var", F64), ("row_max", I64),
+                ("block_bits", I32), ("balanced", I32), ("nb_blocks", I64), ("block_avg", F64), ("block_var", F64)]
+
+
 class Info(C.Structure):
     _fields_ = [("nor", I64), ("noc", I64), ("nnz", I64), ("format", I32), ("value_type", I32),
                 ("threads", I32), ("group_size", I32), ("device_bytes", I64), ("long_rows", I64),
@@ -64,6 +69,7 @@ _SIGS = {
     "pwm_device_fill": [VP, I64, C.c_int, F64],
     "pwm_device_copy": [VP, VP, I64],
     "pwm_time_mv": [VP, VP, VP, C.c_int, I64, C.POINTER(F64), C.POINTER(F64)],
+    "pwm_matrix_stats": [VP, VP],
     "pwm_gather_rate": [VP, I64, I64, C.c_int, I64, C.POINTER(F64)],
     "pwm_gen_powerlaw": [I64, F64, F64, I64, C.c_uint64, C.c_uint64, C.POINTER(VP), C.POINTER(VP),
                          C.POINTER(VP), C.POINTER(I64)],
@@ -282,6 +288,12 @@ class SparseMatrixHip:
         dv = np.zeros(max(inf["nnz"], 1), np.float64)
         _check(lib().pwm_crs_export(self.h, _ptr(rs), _ptr(ci), _ptr(dv)), self.h)
         return rs, ci[:inf["nnz"]], dv[:inf["nnz"]]
+
+    def stats(self):
+        """Matrix statistics of the loaded matrix (Util_Scripts/calc_stats.cpp)."""
+        st = Stats()
+        _check(lib().pwm_matrix_stats(self.h, C.byref(st)), self.h)
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
 
     def info(self):
         i = Info()

@@ -199,19 +199,22 @@ static int plan_crs_vector(pwm_matrix* h, int G) {
 
 // CRS planning and the autotuner.  Candidates: the vector kernel with the
 // rows-per-wave heuristic G0 = pow2(avg row length / 2) in [2, 32] and with
-// G0/2 and 2 G0 (only G0 when the caller fixed group_size), and the tiles
-// kernel (unless crs_kernel excludes either).  With more than one candidate,
+// G0/2 and 2 G0 when the longest row fills them (only G0 when the caller
+// fixed group_size), and the tiles kernel (unless crs_kernel excludes
+// either).  With more than one candidate,
 // each is timed on x = 1 (2 warm-up + 5 timed SpMVs) and the fastest is kept.
 int plan_crs(pwm_matrix* h) {
     const double avg = h->nor > 0 ? (double)h->nnz / (double)h->nor : 0.0;
     const int G0 = h->group_opt > 0 ? h->group_opt : std::min(32, std::max(2, pow2_ceil((int64_t)ceil(avg / 2.0))));
+    // more lanes per row than the longest row has entries leave lanes idle
+    // in every row: such candidates (other than G0) are not timed
     std::vector<int> gs;
     if (h->crs_kernel != 2) {
         if (h->group_opt > 0) {
             gs.push_back(G0);
         } else {
             for (int g : {G0 / 2, G0, 2 * G0})
-                if (g >= 1 && g <= 64) gs.push_back(g);
+                if (g >= 1 && g <= 64 && (g == G0 || g <= std::max<int64_t>(1, h->stats.row_max))) gs.push_back(g);
         }
     }
     const bool tiles = h->crs_kernel != 1;
@@ -453,7 +456,8 @@ static void reset_matrix(pwm_matrix* h) {
 }
 
 static int finish_load(pwm_matrix* h, const int32_t* hrs, const int32_t* hci, const double* hval) {
-    int rc = PWM_OK;
+    int rc = compute_matrix_stats(h);
+    if (rc != PWM_OK) return rc;
     switch (h->format) {
         case PWM_FORMAT_CRS: rc = plan_crs(h); break;
         case PWM_FORMAT_CRS_MERGE: rc = plan_merge(h); break;

@@ -140,6 +140,7 @@ struct pwm_matrix {
     pwm_impl::DevBuf rs, ci, val;
     // host copy of row_start (long-row planning)
     std::vector<int32_t> h_rs;
+    pwm_stats_t stats{};  // computed at load (stats.hip)
 
     // CRS vector kernel plan
     int group_opt = 0;       // option: lanes per row, 0 = autotune
@@ -183,6 +184,9 @@ struct pwm_matrix {
 namespace pwm_impl {
 
 int fail(pwm_matrix* h, int code, const std::string& msg);
+// Statistics of the loaded CRS arrays (stats.hip) into h->stats; needs the
+// device row_start / col_ind.
+int compute_matrix_stats(pwm_matrix* h);
 // COO -> CRS on the device (coo.hip): row_start, padded col_ind and padded
 // FP64 values of the sorted matrix.  Indices must already be validated.
 int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows, const int32_t* cols,

@@ -273,10 +273,14 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     const int64_t S = c.slabs;
     const int64_t units = V * S;  // workgroups per column group
     // block-column groups: the caller's count, else the fastest of 1, 8 and
-    // 16 (timed below, once the arrays are on the device)
+    // 16 (timed below, once the arrays are on the device); only 1 when x
+    // (2 MB or less) already fits one XCD's L2, where groups cannot make the
+    // gathers cheaper and only add partial sums
     std::vector<int32_t> cands;
     if (h->csb_groups > 0)
         cands.push_back((int32_t)std::min<int64_t>(h->csb_groups, H));
+    else if (noc * (int64_t)vector_size(h->vtype) <= ((int64_t)2 << 20))
+        cands.push_back(1);
     else
         for (int64_t gcount : {(int64_t)1, (int64_t)8, (int64_t)16})
             if (std::find(cands.begin(), cands.end(), (int32_t)std::min(gcount, H)) == cands.end())

@@ -184,6 +184,10 @@ struct pwm_matrix {
 namespace pwm_impl {
 
 int fail(pwm_matrix* h, int code, const std::string& msg);
+// CSB slab-stream layout built on the device from the device CRS arrays
+// (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
+int csb_key_bits(int64_t cells, int bits);
+int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S);
 // Statistics of the loaded CRS arrays (stats.hip) into h->stats; needs the
 // device row_start / col_ind.
 int compute_matrix_stats(pwm_matrix* h);

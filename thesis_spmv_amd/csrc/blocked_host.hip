@@ -108,59 +108,25 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     c.H = H;
     c.V = V;
 
-    std::vector<uint32_t> ind(std::max<int64_t>(nnz, 1));
-    std::vector<double> data(std::max<int64_t>(nnz, 1));
-    std::vector<int32_t> blk_ptr(H * V + 1, 0);
-    std::vector<double> brn(V, 0.0);
-    // Per block row: bucket by block column (stable), then sort each block by
-    // Z-Morton key (stable, so duplicates keep their CRS order).
-    parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
-        std::vector<int64_t> cnt;
-        struct KP {
-            uint32_t key;
-            uint32_t lrow;
-            int64_t e;
-        };
-        std::vector<KP> kp;
-        for (int64_t br = b0; br < b1; ++br) {
-            const int64_t r0 = br * beta, r1 = std::min(nor, r0 + beta);
-            const int64_t e0 = rs[r0], e1 = rs[r1];
-            cnt.assign(H + 1, 0);
-            for (int64_t e = e0; e < e1; ++e) cnt[ci[e] / beta + 1]++;
-            for (int64_t bc = 0; bc < H; ++bc) cnt[bc + 1] += cnt[bc];
-            for (int64_t bc = 0; bc < H; ++bc) blk_ptr[br * H + bc + 1] = (int32_t)(e0 + cnt[bc + 1]);
-            brn[br] = (double)(e1 - e0);
-            kp.resize(e1 - e0);
-            std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-            for (int64_t r = r0; r < r1; ++r)
-                for (int64_t e = rs[r]; e < rs[r + 1]; ++e) {
-                    const int64_t bc = ci[e] / beta;
-                    kp[pos[bc]++] = {zkey((uint32_t)(r - r0), (uint32_t)(ci[e] - bc * beta)), (uint32_t)(r - r0), e};
-                }
-            for (int64_t bc = 0; bc < H; ++bc)
-                std::stable_sort(kp.begin() + cnt[bc], kp.begin() + cnt[bc + 1],
-                                 [](const KP& a, const KP& b) { return a.key < b.key; });
-            for (int64_t i = 0; i < e1 - e0; ++i) {
-                const int64_t e = kp[i].e;
-                ind[e0 + i] = (kp[i].lrow << 16) | (uint32_t)(ci[e] % beta);
-                data[e0 + i] = vals[e];
-            }
-        }
-    });
-    blk_ptr[0] = 0;
     // CSB.hpp:549-559: unbalanced when a block row holds more than twice the mean
     c.balanced = 1;
-    const double avg = std::accumulate(brn.begin(), brn.end(), 0.0) / (double)V;
-    for (int64_t i = 0; i < V; ++i)
-        if (brn[i] > 2 * avg) {
-            c.balanced = 0;
-            break;
-        }
+    {
+        std::vector<double> brn(V, 0.0);
+        for (int64_t br = 0; br < V; ++br) brn[br] = (double)(rs[std::min(nor, (br + 1) * beta)] - rs[br * beta]);
+        const double avg = std::accumulate(brn.begin(), brn.end(), 0.0) / (double)V;
+        for (int64_t i = 0; i < V; ++i)
+            if (brn[i] > 2 * avg) {
+                c.balanced = 0;
+                break;
+            }
+    }
 
     // Slab-stream layout: slabs of 2^rb rows, rb <= the LDS rows, beta and
     // 32 - (bits of a column index), so that (row in slab, column) packs into
-    // 32 bits.  Used unless the caller asks for pieces, or the slabs would be
-    // shorter than 256 rows (automatic choice only).
+    // 32 bits.  Used unless the caller asks for pieces, the slabs would be
+    // shorter than 256 rows (automatic choice only), or the device build's
+    // sort key (slab, block column, Z-Morton code) would need more than 64
+    // bits.
     const int lds_bits = blocked::lds_rows(h->vtype) == 8192 ? 13 : 12;
     int cb = 0;
     while (((int64_t)1 << cb) < noc) ++cb;
@@ -170,50 +136,51 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     while (fill_bits < lds_bits && (nor >> (fill_bits + 1)) >= 1024) ++fill_bits;
     const int rb = std::min(std::min(std::min(lds_bits, fill_bits), bits), 32 - cb);
     const bool use_stream = h->csb_layout == 2 || (h->csb_layout == 0 && rb >= std::min(8, bits));
-    if (use_stream) {
-        const int64_t S = beta >> rb, hrows = (int64_t)1 << rb;
-        const int64_t cells = V * S * H;
-        std::vector<int64_t> off(cells + 1, 0);
-        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
-            for (int64_t br = b0; br < b1; ++br)
-                for (int64_t bc = 0; bc < H; ++bc)
-                    for (int64_t e = blk_ptr[br * H + bc]; e < blk_ptr[br * H + bc + 1]; ++e)
-                        off[(br * S + ((ind[e] >> 16) >> rb)) * H + bc + 1]++;
-        });
-        for (int64_t i = 0; i < cells; ++i) off[i + 1] += off[i];
-        const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
-        std::vector<uint32_t> sind(zpad, 0u);
-        std::vector<double> sdat(zpad, 0.0);
-        // within a slab: block columns ascending, each block's entries in
-        // their stored (Z-Morton) order
-        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
-            std::vector<int64_t> pos(S);
-            for (int64_t br = b0; br < b1; ++br)
-                for (int64_t bc = 0; bc < H; ++bc) {
-                    for (int64_t sl = 0; sl < S; ++sl) pos[sl] = off[(br * S + sl) * H + bc];
-                    for (int64_t e = blk_ptr[br * H + bc]; e < blk_ptr[br * H + bc + 1]; ++e) {
-                        const uint32_t lrow = ind[e] >> 16;
-                        const int64_t p = pos[lrow >> rb]++;
-                        sind[p] = ((lrow & (uint32_t)(hrows - 1)) << cb) | (uint32_t)(bc * beta + (ind[e] & 0xffffu));
-                        sdat[p] = data[e];
-                    }
-                }
-        });
-        std::vector<int32_t> off32(off.begin(), off.end());
-        hipError_t e;
-        if ((e = upload(c.ind, sind.data(), zpad * 4)) != hipSuccess ||
-            (e = upload(c.blk_ptr, blk_ptr.data(), blk_ptr.size() * 4)) != hipSuccess ||
-            (e = upload(c.soff, off32.data(), off32.size() * 4)) != hipSuccess)
-            return hip_fail(h, e, "CSB upload");
-        rc = upload_blocked_values(h, c.data, sdat);
+    const int64_t S_stream = beta >> rb;
+    if (use_stream && csb_key_bits(V * S_stream * H, bits) <= 64) {
+        // the slab streams, sorted on the device from the device CRS arrays
+        rc = csb_stream_device(h, rb, cb, S_stream);
         if (rc != PWM_OK) return rc;
-        c.stream = 1;
-        c.col_bits = cb;
-        c.slab_bits = rb;
-        c.slabs = (int32_t)S;
-        c.slab_rows = hrows;
-        c.pieces_per_block = 1;
     } else {
+        // slab pieces over the stored order, built on the host
+        std::vector<uint32_t> ind(std::max<int64_t>(nnz, 1));
+        std::vector<double> data(std::max<int64_t>(nnz, 1));
+        std::vector<int32_t> blk_ptr(H * V + 1, 0);
+        // Per block row: bucket by block column (stable), then sort each block by
+        // Z-Morton key (stable, so duplicates keep their CRS order).
+        parallel_for(V, 1, [&](int64_t b0, int64_t b1, int) {
+            std::vector<int64_t> cnt;
+            struct KP {
+                uint32_t key;
+                uint32_t lrow;
+                int64_t e;
+            };
+            std::vector<KP> kp;
+            for (int64_t br = b0; br < b1; ++br) {
+                const int64_t r0 = br * beta, r1 = std::min(nor, r0 + beta);
+                const int64_t e0 = rs[r0], e1 = rs[r1];
+                cnt.assign(H + 1, 0);
+                for (int64_t e = e0; e < e1; ++e) cnt[ci[e] / beta + 1]++;
+                for (int64_t bc = 0; bc < H; ++bc) cnt[bc + 1] += cnt[bc];
+                for (int64_t bc = 0; bc < H; ++bc) blk_ptr[br * H + bc + 1] = (int32_t)(e0 + cnt[bc + 1]);
+                kp.resize(e1 - e0);
+                std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+                for (int64_t r = r0; r < r1; ++r)
+                    for (int64_t e = rs[r]; e < rs[r + 1]; ++e) {
+                        const int64_t bc = ci[e] / beta;
+                        kp[pos[bc]++] = {zkey((uint32_t)(r - r0), (uint32_t)(ci[e] - bc * beta)), (uint32_t)(r - r0), e};
+                    }
+                for (int64_t bc = 0; bc < H; ++bc)
+                    std::stable_sort(kp.begin() + cnt[bc], kp.begin() + cnt[bc + 1],
+                                     [](const KP& a, const KP& b) { return a.key < b.key; });
+                for (int64_t i = 0; i < e1 - e0; ++i) {
+                    const int64_t e = kp[i].e;
+                    ind[e0 + i] = (kp[i].lrow << 16) | (uint32_t)(ci[e] % beta);
+                    data[e0 + i] = vals[e];
+                }
+            }
+        });
+        blk_ptr[0] = 0;
         // Slab pieces: a workgroup owns `slab` rows of a block row (its y-slice
         // lives in LDS).  With slab = beta >> k, the entries of one block whose
         // top k row bits equal s form 2^(k-1) contiguous Z-order runs (one per

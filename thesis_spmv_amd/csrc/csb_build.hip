@@ -1,0 +1,193 @@
+// CSB slab-stream layout built on the device (the stored order of
+// SOA_Implementations/CSB.hpp:464-567, regrouped by row slab).
+//
+// Entry k of row r, column c gets the key
+//     cell << 2b | zkey(r mod beta, c mod beta),   cell = (br*S + s)*H + bc,
+// with br = r >> b, bc = c >> b, s = (r mod beta) >> rb the slab inside the
+// block row, and zkey the Z-Morton interleave (row bit above column bit,
+// CSB.hpp:147-173).  A stable radix sort of (key, k) puts every slab's entries
+// together, block columns ascending, each block's entries in Z-Morton order
+// and duplicates in CRS order -- the order the host build gives them.  The
+// packed indices and values are then gathered through the sorted positions,
+// each (slab, block column) run start is found by binary search, and the
+// reference's blk_ptr follows from the run lengths.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "api_internal.hpp"
+#include "blocked.hpp"
+#include "kernels.hpp"
+
+namespace pwmk {
+
+// bit i of a 16-bit value to bit 2i, and back
+__device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__device__ __forceinline__ uint32_t compact16(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+
+struct CsbKeyArgs {
+    int64_t nor, nnz, S, H;
+    int bits, rb, cb;
+};
+
+__global__ void k_csb_keys(CsbKeyArgs a, const int32_t* __restrict__ rs, const int32_t* __restrict__ ci,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ pos) {
+    const uint32_t bmask = (uint32_t)(((uint64_t)1 << a.bits) - 1);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = a.nor - 1;  // the last row starting at or before k
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (rs[mid] <= k)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t c = (uint32_t)ci[k];
+        const uint32_t lrow = (uint32_t)lo & bmask, lcol = c & bmask;
+        const uint64_t cell = ((uint64_t)(lo >> a.bits) * a.S + (lrow >> a.rb)) * a.H + (c >> a.bits);
+        keys[k] = (cell << (2 * a.bits)) | (((uint64_t)spread16(lrow) << 1) | spread16(lcol));
+        pos[k] = (uint32_t)k;
+    }
+}
+
+template <typename E>
+__global__ void k_csb_gather(CsbKeyArgs a, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ pos,
+                             const E* __restrict__ val, uint32_t* __restrict__ sind, E* __restrict__ sdat) {
+    const uint64_t zmask = (a.bits == 0) ? 0 : ((((uint64_t)1 << (2 * a.bits)) - 1));
+    const uint32_t smask = (uint32_t)(((uint64_t)1 << a.rb) - 1);
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = keys[p];
+        const uint32_t zk = (uint32_t)(key & zmask);
+        const uint64_t bc = (key >> (2 * a.bits)) % (uint64_t)a.H;
+        const uint32_t lrow = compact16(zk >> 1), lcol = compact16(zk);
+        const uint32_t col = (uint32_t)(bc << a.bits) + lcol;
+        sind[p] = ((lrow & smask) << a.cb) | col;
+        sdat[p] = val[pos[p]];
+    }
+}
+
+// soff[cell] = number of sorted keys below cell << 2b, cell < cells
+__global__ void k_csb_soff(const uint64_t* __restrict__ keys, int64_t nnz, int64_t cells, int bits,
+                           int32_t* __restrict__ soff) {
+    for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < cells;
+         cell += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = (uint64_t)cell << (2 * bits);
+        int64_t lo = 0, hi = nnz;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < t)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        soff[cell] = (int32_t)lo;
+    }
+}
+
+static unsigned grid_of(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
+}
+
+}  // namespace pwmk
+
+namespace pwm_impl {
+
+int csb_key_bits(int64_t cells, int bits) {
+    int cb = 0;
+    while (((int64_t)1 << cb) < cells) ++cb;
+    return cb + 2 * bits;
+}
+
+int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S) {
+    CsbDev& c = h->csb;
+    const int bits = c.bits;
+    const int64_t H = c.H, V = c.V, nnz = h->nnz, nor = h->nor;
+    const int64_t cells = V * S * H;
+    const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
+    const size_t vs = value_size(h->vtype);
+    hipError_t e;
+    if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess ||
+        (e = c.soff.alloc((cells + 1) * 4)) != hipSuccess)
+        return hip_fail(h, e, "CSB stream allocation");
+    if ((e = hipMemsetAsync(c.ind.p, 0, zpad * 4, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "CSB stream allocation");
+    pwmk::CsbKeyArgs a{nor, nnz, S, H, bits, rb, cb};
+    std::vector<int32_t> soff(cells + 1, 0);
+    if (nnz > 0) {
+        DevBuf keys, keys_out, pos, pos_out, tmp;
+        if ((e = keys.alloc(nnz * 8)) != hipSuccess || (e = keys_out.alloc(nnz * 8)) != hipSuccess ||
+            (e = pos.alloc(nnz * 4)) != hipSuccess || (e = pos_out.alloc(nnz * 4)) != hipSuccess)
+            return hip_fail(h, e, "CSB build allocation");
+        hipLaunchKernelGGL(pwmk::k_csb_keys, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream, a,
+                           h->rs.as<int32_t>(), h->ci.as<int32_t>(), keys.as<uint64_t>(), pos.as<uint32_t>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB keys");
+        const unsigned end_bit = (unsigned)std::max(csb_key_bits(cells, bits), 1);
+        size_t tb = 0;
+        e = rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
+                                      pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
+        if (e == hipSuccess) e = tmp.alloc(std::max<size_t>(tb, 4));
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                          pos.as<uint32_t>(), pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit,
+                                          h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "CSB sort");
+        const dim3 g(pwmk::grid_of(nnz));
+        if (vs == 8)
+            hipLaunchKernelGGL(pwmk::k_csb_gather<uint64_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint64_t>());
+        else if (vs == 4)
+            hipLaunchKernelGGL(pwmk::k_csb_gather<uint32_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(pwmk::k_csb_gather<uint16_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint16_t>());
+        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
+                           keys_out.as<uint64_t>(), nnz, cells, bits, c.soff.as<int32_t>());
+        if ((e = hipGetLastError()) == hipSuccess)
+            e = hipMemcpyAsync(soff.data(), c.soff.p, cells * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "CSB build");
+    }
+    soff[cells] = (int32_t)nnz;
+    // the stored order's blk_ptr: block (br, bc) holds the runs of its S slabs
+    std::vector<int32_t> blk_ptr(H * V + 1, 0);
+    for (int64_t br = 0; br < V; ++br)
+        for (int64_t bc = 0; bc < H; ++bc) {
+            int64_t n = 0;
+            for (int64_t sl = 0; sl < S; ++sl) {
+                const int64_t cell = (br * S + sl) * H + bc;
+                n += soff[cell + 1] - soff[cell];
+            }
+            blk_ptr[br * H + bc + 1] = (int32_t)(blk_ptr[br * H + bc] + n);
+        }
+    if ((e = hipMemcpy((char*)c.soff.p + cells * 4, &soff[cells], 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = c.blk_ptr.alloc(blk_ptr.size() * 4)) != hipSuccess ||
+        (e = hipMemcpy(c.blk_ptr.p, blk_ptr.data(), blk_ptr.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "CSB build");
+    c.stream = 1;
+    c.col_bits = cb;
+    c.slab_bits = rb;
+    c.slabs = (int32_t)S;
+    c.slab_rows = (int64_t)1 << rb;
+    c.pieces_per_block = 1;
+    return PWM_OK;
+}
+
+}  // namespace pwm_impl

@@ -130,10 +130,10 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     const int lds_bits = blocked::lds_rows(h->vtype) == 8192 ? 13 : 12;
     int cb = 0;
     while (((int64_t)1 << cb) < noc) ++cb;
-    // no longer than needed for about 1024 slabs (a few workgroups per CU),
+    // no longer than needed for about 4096 slabs (16 workgroups per CU),
     // but at least 256 rows
     int fill_bits = 8;
-    while (fill_bits < lds_bits && (nor >> (fill_bits + 1)) >= 1024) ++fill_bits;
+    while (fill_bits < lds_bits && (nor >> (fill_bits + 1)) >= 4096) ++fill_bits;
     const int rb = std::min(std::min(std::min(lds_bits, fill_bits), bits), 32 - cb);
     const bool use_stream = h->csb_layout == 2 || (h->csb_layout == 0 && rb >= std::min(8, bits));
     const int64_t S_stream = beta >> rb;

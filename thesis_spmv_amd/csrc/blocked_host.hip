@@ -358,8 +358,6 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
     check_bf16(h, vals, nnz, &rc);
     if (rc != PWM_OK) return rc;
     const int64_t lds_rows = blocked::lds_rows(h->vtype);
-    // thread count: the caller's, else the smallest count (>= 256 bands, so
-    // every CU gets work) whose bands all fit the LDS y-window
     std::vector<int64_t> bstart, bnnz;
     int threads = h->threads;
     auto clamp = [&](int t) {  // BlockCOH.hpp:325-328 (kept >= 1)
@@ -367,11 +365,12 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         if (nor / t < 8) t = (int)(nor / 8);
         return t < 1 ? 1 : t;
     };
-    // the caller's thread count, else about 16K entries per band (and at
-    // least one band per CU); bands the LDS window or one task cannot hold
-    // are split into atomic tasks below
+    // the caller's thread count, else about 16K entries per band and at
+    // least 640 bands (2.5 per CU: a 5M-entry Poisson matrix runs in 40 us
+    // with 639 bands against 57 us with 319); bands the LDS window or one
+    // task cannot hold are split into atomic tasks below
     threads = threads > 0 ? clamp(threads)
-                          : clamp((int)std::min<int64_t>(INT32_MAX, std::max<int64_t>(256, nnz / 16384)));
+                          : clamp((int)std::min<int64_t>(INT32_MAX, std::max<int64_t>(640, nnz / 16384)));
     coh_bands(nor, nnz, rs, threads, bstart, bnnz);
     CohDev& c = h->coh;
     c.bands.assign(threads, CohBandMeta{});

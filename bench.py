@@ -119,7 +119,9 @@ def bench_batch(args):
     """Batched mixed-precision power method: 512 SPD instances of uneven size
     (256 square 2D Poisson grids, side uniform in [32, 1024]; 256 symmetric
     power-law matrices, N log-uniform in [1e3, 2e5], diagonal = row |sum| + 1),
-    FP32 values and vectors, FP64 norms / Rayleigh quotients, stop at relative
+    FP32 values (with --batch-bf16-exact 1, BF16 for every instance whose
+    values are exact in BF16, i.e. the Poisson grids) and FP32 vectors, FP64
+    norms / Rayleigh quotients, stop at relative
     eigenvalue change <= 1e-7 or 2000 iterations; all unfinished instances
     advance together, one tile launch + one finish launch per step, chunks
     of steps replayed as a hipGraph.  Compared with the plain FP64 per-matrix
@@ -132,7 +134,9 @@ def bench_batch(args):
     ns = np.exp(rng.uniform(np.log(1e3), np.log(2e5), n_each)).astype(np.int64)
     t0 = time.time()
     spd = [pw.gen_spd_powerlaw(int(n), seed=1000 + i) for i, n in enumerate(ns)]
-    b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk)
+    mixed = args.batch_values == "f32" and args.batch_bf16_exact
+    opts = {"bf16_when_exact": 1} if mixed else {}
+    b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk, **opts)
     ids = [b.add_poisson(int(s), int(s)) for s in sides] + [b.add_crs(*m) for m in spd]
     n_inst, rows, nnz = b.size()
     print(f"# batch: {n_inst} instances, {rows} rows, {nnz} nnz, built in {time.time() - t0:.1f}s",
@@ -144,11 +148,12 @@ def bench_batch(args):
     pw.synchronize()
     wall = time.perf_counter() - t1
     solves = n_inst / wall
-    print(f"# batched {args.batch_values}: {wall * 1e3:.1f} ms wall ({ms:.1f} ms device), {solves:.1f} solves/s, "
+    label = "f32, bf16 where exact" if mixed else args.batch_values
+    print(f"# batched {label}: {wall * 1e3:.1f} ms wall ({ms:.1f} ms device), {solves:.1f} solves/s, "
           f"iterations min/median/max {its.min()}/{int(np.median(its))}/{its.max()}", file=sys.stderr, flush=True)
     out = {"metric": METRIC, "value": round(solves, 2), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
            "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": args.batch_values, "data": "synthetic",
+           "dtype": "f32+bf16" if mixed else args.batch_values, "data": "synthetic",
            "config": {"workload": "batched power method, 512 SPD instances (Poisson sides U[32,1024], power-law "
                                   "SPD N log-U[1e3,2e5])", "instances": n_inst, "rows": rows, "nnz": nnz,
                       "max_it": args.max_it, "tol": args.tol},
@@ -156,16 +161,22 @@ def bench_batch(args):
            "engine": b.stats()}
     # algorithmic bytes of every step actually taken (values, indices, row
     # offsets, x and y once per step)
-    vb = {"f64": 8, "f32": 4, "bf16": 2}[args.batch_values]
+    vb0 = {"f64": 8, "f32": 4, "bf16": 2}[args.batch_values]
     xb = 8 if args.batch_values == "f64" else 4
-    sizes = [(int(s) ** 2, 5 * int(s) ** 2 - 4 * int(s)) for s in sides] + [(len(m[0]) - 1, int(m[0][-1])) for m in spd]
-    traffic = sum(int(k) * ((vb + 4) * z + 4 * (n + 1) + 2 * xb * n) for k, (n, z) in zip(its, sizes))
+
+    def exact16(v):  # the library's test: float32-exact with a zero low half
+        f = np.asarray(v, np.float64).astype(np.float32)
+        return bool(np.all(f.astype(np.float64) == v) and np.all((f.view(np.uint32) & 0xFFFF) == 0))
+
+    sizes = [(int(s) ** 2, 5 * int(s) ** 2 - 4 * int(s), 2 if mixed else vb0) for s in sides] + \
+            [(len(m[0]) - 1, int(m[0][-1]), 2 if mixed and exact16(m[2]) else vb0) for m in spd]
+    traffic = sum(int(k) * ((vb + 4) * z + 4 * (n + 1) + 2 * xb * n) for k, (n, z, vb) in zip(its, sizes))
     out["achieved_GBps"] = round(traffic / wall / 1e9, 1)
     print(f"# algorithmic traffic {traffic / 1e12:.2f} TB over all steps -> {traffic / wall / 1e9:.0f} GB/s",
           file=sys.stderr, flush=True)
     # one step with every instance active: 128 steps without the stop test
     # (after a first such solve that captures their graphs)
-    full = sum((vb + 4) * z + 4 * (n + 1) + 2 * xb * n for n, z in sizes)
+    full = sum((vb + 4) * z + 4 * (n + 1) + 2 * xb * n for n, z, vb in sizes)
     b.solve(128, -1.0, seed=0)
     pw.synchronize()
     t2 = time.perf_counter()
@@ -424,7 +435,7 @@ def main():
     ap.add_argument("--workload", default="powerlaw")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
-    ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh")
+    ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh,csb_ref_beta")
     ap.add_argument("--crs-kernel", type=int, default=0,
                     help="CRS kernel: 0 = time the candidates at load, 1 = vector, 2 = row-aligned tiles")
     ap.add_argument("--csb-groups", type=int, default=0, help="CSB block-column groups (0 = time 1, 8, 16 at load)")
@@ -436,6 +447,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-values", default="f32")
     ap.add_argument("--batch-ipt", type=int, default=16, help="merge items per thread of a batch tile (4, 8, 16)")
+    ap.add_argument("--batch-bf16-exact", type=int, default=0,
+                    help="FP32 batch: store instances whose values are exact in BF16 as BF16 (1) or not (0)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="batch steps per graph replay")
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)
@@ -482,6 +495,9 @@ def main():
     csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
     formats = (("crs", pw.CRSHip, {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}),
                ("merge_path", pw.CRSMergeHip, {}), ("csb", pw.CSBHip, csb_opts), ("blockcoh", pw.BlockCOHHip, {}))
+    if args.workload == "stencil9":
+        # the workload names beta = 2^14; the reference's own rule gives 2^15
+        formats += (("csb_ref_beta", pw.CSBHip, {k: v for k, v in csb_opts.items() if k != "block_bits"}),)
     for name, cls, opts in formats:
         if name not in wanted:
             continue

@@ -281,7 +281,9 @@ int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_
 /* "items_per_thread" (4, 8, 16, default 16: tile = 256 x items merge items),
  * "chunk" (steps per graph replay, 1..4096), "graph" (0: plain launches),
  * "persistent" (0: one workgroup per tile instead of resident workgroups
- * that prefetch their next tile). */
+ * that prefetch their next tile), "bf16_when_exact" (FP32 batches: 1 stores
+ * each instance whose values are all exact in BF16 as BF16, the others as
+ * FP32; vectors stay FP32). */
 int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value);
 /* Kernel launches, graph instantiations and work-list builds so far. */
 int pwm_batch_stats(const pwm_batch* b, int64_t* launches, int64_t* graphs, int64_t* lists);

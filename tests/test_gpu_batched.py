@@ -185,3 +185,42 @@ def test_errors():
         pw.Batch("f32").solve(10)  # no instances
     with pytest.raises(pw.PwmError):
         pw.Batch("f32").add_crs([0, 1], [3], [1.0])  # column out of range
+
+
+def test_bf16_where_exact_is_bitwise_the_pure_batches():
+    """An FP32 batch with bf16_when_exact stores the Poisson instances (4 and
+    -1) as BF16 and the power-law ones as FP32, in one launch per step: each
+    instance's result is bit-identical to the same instance in a pure BF16
+    (Poisson) or pure FP32 (power-law) batch, and every eigenvalue is within
+    1e-6 of the FP64 oracle."""
+    rng = np.random.default_rng(9)
+    mats = _instances(rng)
+    mixed = pw.Batch("f32", bf16_when_exact=1)
+    ids = [mixed.add_crs(A.row_start, A.col_ind, A.data) for A in mats]
+    K = 50
+    lam_m, _, _ = mixed.solve(K, -1.0, seed=5)
+    f32 = pw.Batch("f32")
+    for A in mats:
+        f32.add_crs(A.row_start, A.col_ind, A.data)
+    lam_f, _, _ = f32.solve(K, -1.0, seed=5)
+    # _instances lists the four Poisson matrices first: in a BF16 batch of
+    # just those they keep their ids (and so their start vectors)
+    bf2 = pw.Batch("bf16")
+    for A in mats[:4]:
+        bf2.add_crs(A.row_start, A.col_ind, A.data)
+    lam_b, _, _ = bf2.solve(K, -1.0, seed=5)
+    for i, A in zip(ids, mats):
+        vm, _ = mixed.vector(i, A.nor)
+        if i < 4:  # the Poisson instances: BF16 values
+            vb, _ = bf2.vector(i, A.nor)
+            assert lam_m[i] == lam_b[i] and np.array_equal(vm, vb), i
+        else:
+            vf, _ = f32.vector(i, A.nor)
+            assert lam_m[i] == lam_f[i] and np.array_equal(vm, vf), i
+        x0 = mixed.start_vector(i, A.nor, seed=5)
+        xr, yr = oracle.crs_power_iteration(A, x0, K - 1)
+        ref = _rq(A, xr if (K - 1) % 2 == 0 else yr)
+        tl = 1e-6 if i < len(mats) - 1 else 1e-5
+        assert abs(lam_m[i] - ref) <= tl * abs(ref), (i, lam_m[i], ref)
+    with pytest.raises(pw.PwmError):
+        pw.Batch("f64", bf16_when_exact=1)

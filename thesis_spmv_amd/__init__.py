@@ -418,7 +418,8 @@ class Batch:
     """Many power methods stepped together (pwm_batch_* in pwm_hip.h).
 
     Options: items_per_thread (4, 8, 16), chunk (steps per graph replay),
-    graph (False: plain launches)."""
+    graph (False: plain launches), persistent, bf16_when_exact (FP32 batches:
+    BF16 storage for the instances whose values are exact in BF16)."""
 
     def __init__(self, values="f32", **options):
         self.h = VP()

@@ -7,6 +7,9 @@
 //
 // Values: FP32 or BF16 (exact values only) with FP32 vectors and per-row FP32
 // accumulation, or FP64 throughout; norms and Rayleigh quotients in FP64.
+// With the option bf16_when_exact an FP32 batch stores each instance whose
+// values are all exact in BF16 (every Poisson instance: 4 and -1) as BF16
+// and the others as FP32, in one launch per step.
 #include <math.h>
 #include <string.h>
 
@@ -27,6 +30,7 @@ inline uint64_t mix64h(uint64_t z) {
 struct BatchInst {
     int64_t n = 0, nnz = 0;
     int kind = 0;           // 0: CRS from the host, 1: Poisson generated on the device
+    int vt16 = 0;           // BF16 values in a bf16_when_exact batch
     int64_t pm = 0, pn = 0;  // Poisson grid
     std::vector<int32_t> rs, ci;
     std::vector<double> val;
@@ -51,6 +55,7 @@ struct pwm_batch {
     int chunk = 32;
     int use_graph = 1;
     int persistent = 0;
+    int bf16_exact = 0;     // FP32 batch: BF16 storage for instances exact in BF16
     pwm_impl::Arena arena;
     pwm_impl::Engine* eng = nullptr;
     std::vector<double> lambda, ynorm;
@@ -171,16 +176,38 @@ extern "C" int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value
         b->persistent = value != 0;
         return PWM_OK;
     }
+    if (k == "bf16_when_exact") {
+        if (value != 0 && b->vtype != PWM_VALUES_F32)
+            return bfail(b, PWM_ERR_INVALID, "pwm_batch_set_option: bf16_when_exact needs an FP32 batch");
+        if (b->eng && b->eng->active() != 0) return bfail(b, PWM_ERR_STATE, "pwm_batch_set_option: solve in flight");
+        b->bf16_exact = value != 0;
+        b->uploaded = false;
+        delete b->eng;  // the engine's value type changes with it
+        b->eng = nullptr;
+        return PWM_OK;
+    }
     return bfail(b, PWM_ERR_INVALID, "pwm_batch_set_option: unknown option " + k);
+}
+
+static bool exact_in_bf16(const std::vector<double>& v) {
+    for (double d : v) {
+        const float f = (float)d;
+        uint32_t bits;
+        memcpy(&bits, &f, 4);
+        if ((double)f != d || (bits & 0xffffu)) return false;
+    }
+    return true;
 }
 
 static int batch_upload(pwm_batch* b) {
     if (b->uploaded && b->planned_ipt == b->ipt) return PWM_OK;
-    const size_t vs = pwm_impl::value_size(b->vtype), xs = pwm_impl::vector_size(b->vtype);
+    const size_t xs = pwm_impl::vector_size(b->vtype);
     const size_t acc = b->vtype == PWM_VALUES_F64 ? 8 : 4;
     using pwm_impl::Arena;
     size_t total = 0;
     for (BatchInst& I : b->inst) {
+        I.vt16 = b->bf16_exact && (I.kind == 1 || exact_in_bf16(I.val));
+        const size_t vs = I.vt16 ? 2 : pwm_impl::value_size(b->vtype);
         I.n_tiles = pwmk::engine_tiles_for(I.n, I.nnz, b->ipt);
         const int64_t zp = I.nnz + pwm_impl::kPadEntries;  // tiles read aligned groups of 4
         total += Arena::round((I.n + 1) * 4) + Arena::round(zp * 4) + Arena::round(zp * vs) +
@@ -195,6 +222,8 @@ static int batch_upload(pwm_batch* b) {
     if (max_stage > 0 && (e = tmp.alloc(max_stage * 8)) != hipSuccess) return bhip(b, e, "batch staging");
     Arena& A = b->arena;
     for (BatchInst& I : b->inst) {
+        const int vt = I.vt16 ? PWM_VALUES_BF16 : b->vtype;
+        const size_t vs = pwm_impl::value_size(vt);
         const int64_t zp = I.nnz + pwm_impl::kPadEntries;
         bool ok = A.alloc((I.n + 1) * 4, &I.o_rs) && A.alloc(zp * 4, &I.o_ci) && A.alloc(zp * vs, &I.o_val) &&
                   A.alloc((size_t)I.n_tiles * sizeof(pwmk::ETile), &I.o_tiles) && A.alloc((size_t)I.n_tiles * 24, &I.o_part) &&
@@ -210,10 +239,10 @@ static int batch_upload(pwm_batch* b) {
             if (e == hipSuccess && I.nnz) e = hipMemcpy(A.at(I.o_ci), I.ci.data(), I.nnz * 4, hipMemcpyHostToDevice);
             if (e == hipSuccess && I.nnz) e = hipMemcpy(tmp.p, I.val.data(), I.nnz * 8, hipMemcpyHostToDevice);
             if (e == hipSuccess && I.nnz)
-                e = pwmk::launch_convert_values(tmp.as<double>(), A.at(I.o_val), I.nnz, b->vtype, b->stream);
+                e = pwmk::launch_convert_values(tmp.as<double>(), A.at(I.o_val), I.nnz, vt, b->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
         } else {
-            e = pwmk::launch_fill_poisson(I.pm, I.pn, rs, (int32_t*)A.at(I.o_ci), A.at(I.o_val), b->vtype, b->stream);
+            e = pwmk::launch_fill_poisson(I.pm, I.pn, rs, (int32_t*)A.at(I.o_ci), A.at(I.o_val), vt, b->stream);
         }
         if (e == hipSuccess)
             e = pwmk::engine_plan_tiles(rs, I.n, I.nnz, b->ipt, (pwmk::ETile*)A.at(I.o_tiles), b->stream);
@@ -244,7 +273,8 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
     if (!b->eng) {
         b->eng = new pwm_impl::Engine();
         std::string err;
-        if ((rc = b->eng->init(b->vtype, ni, b->stream, err)) != PWM_OK) {
+        const int evt = b->bf16_exact ? pwmk::VT_MIXED16 : b->vtype;
+        if ((rc = b->eng->init(evt, ni, b->stream, err)) != PWM_OK) {
             delete b->eng;
             b->eng = nullptr;
             return bfail(b, rc, err);
@@ -286,6 +316,7 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
         j.max_it = max_it;
         j.tol = tol;
         j.n_tiles = I.n_tiles;
+        j.vt16 = I.vt16;
         I.slot = E.add(j, I.tiles.data());
     }
     std::vector<int> fin;

@@ -183,6 +183,31 @@ __global__ __launch_bounds__(kEngBS) void k_eng_tiles(const EWork* __restrict__ 
     eng_tile<VT, ITEMS>(W, v, R, sm, red, [] {});
 }
 
+// FP32 vectors, each tile's values FP32 or BF16 by its job (a branch that is
+// uniform over the workgroup); the accumulator and LDS tile are FP32 either
+// way.
+template <int ITEMS>
+__global__ __launch_bounds__(kEngBS) void k_eng_tiles_mixed(const EWork* __restrict__ work,
+                                                            const EJob* __restrict__ jobs,
+                                                            const EState* __restrict__ st,
+                                                            const ECtl* __restrict__ ctl) {
+    __shared__ MergeTileSmem<float, kEngBS, ITEMS> sm;
+    __shared__ double red[48];
+    if ((int)blockIdx.x >= ctl->n_work) return;
+    const EWork W = work[blockIdx.x];
+    EView<float> v;
+    if (!eng_view(jobs, st, W.slot, v)) return;
+    if (W.vt16) {
+        TileRegs<bf16_t, ITEMS> R;
+        tile_load<bf16_t, kEngBS, ITEMS>(R, W.ci, (const bf16_t*)W.val, W.b0, W.b1);
+        eng_tile<bf16_t, ITEMS>(W, v, R, sm, red, [] {});
+    } else {
+        TileRegs<float, ITEMS> R;
+        tile_load<float, kEngBS, ITEMS>(R, W.ci, (const float*)W.val, W.b0, W.b1);
+        eng_tile<float, ITEMS>(W, v, R, sm, red, [] {});
+    }
+}
+
 // Persistent variant: gridDim.x (a multiple of 8, at most the resident
 // capacity) workgroups; workgroup b takes entries b, b+G, b+2G, ... (all on
 // its own XCD).  The next entry is read while the current tile is staged,
@@ -320,7 +345,23 @@ static int resident_v(int ipt) {
     }
 }
 
+template <int ITEMS>
+static hipError_t step_mixed(const EArgs& a, int gw, int gs, hipStream_t s) {
+    hipLaunchKernelGGL((k_eng_tiles_mixed<ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a.work, a.jobs, (const EState*)a.st,
+                       a.ctl);
+    hipLaunchKernelGGL((k_eng_finish<float>), dim3(gs), dim3(kEngBS), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t engine_step(const EArgs& a, int vtype, int ipt, int gw, int gp, int gs, hipStream_t s) {
+    if (vtype == VT_MIXED16) {  // no persistent variant
+        switch (ipt) {
+            case 4: return step_mixed<4>(a, gw, gs, s);
+            case 8: return step_mixed<8>(a, gw, gs, s);
+            case 16: return step_mixed<16>(a, gw, gs, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (vtype) {
         case VT_F64: return step_v<double>(a, ipt, gw, gp, gs, s);
         case VT_F32: return step_v<float>(a, ipt, gw, gp, gs, s);
@@ -568,6 +609,7 @@ int Engine::build_lists(std::string& err) {
             W.b0 = T.b0;
             W.b1 = T.b1;
             W.stream = T.stream;
+            W.vt16 = J.vt16;
         }
     int j = 0;
     for (int s = 0; s < max_slots_; ++s)

@@ -61,7 +61,7 @@ struct EJob {  // device view of one slot
     int64_t max_it;
     double tol;
     int32_t n_tiles;
-    int32_t pad;
+    int32_t vt16;  // mixed engines: 1 when the values are BF16 (else FP32)
 };
 
 struct EState {  // per slot, written by k_eng_finish
@@ -82,7 +82,8 @@ struct EWork {
     int32_t slot, tile;
     int32_t a0, a1, b0, b1;
     int32_t stream;
-    int32_t pad[3];
+    int32_t vt16;  // the job's vt16
+    int32_t pad[2];
 };
 
 struct ECtl {  // list lengths, read by the kernels
@@ -100,6 +101,9 @@ struct EArgs {
 };
 
 constexpr int kEngBS = 256;
+// Engine value type for FP32 vectors with per-job FP32 or BF16 values
+// (EJob::vt16), beside VT_F64 / VT_F32 / VT_BF16.
+constexpr int VT_MIXED16 = 3;
 
 int32_t engine_tiles_for(int64_t n, int64_t nnz, int ipt);
 hipError_t engine_plan_tiles(const int32_t* rs, int64_t n, int64_t nnz, int ipt, ETile* tiles, hipStream_t s);

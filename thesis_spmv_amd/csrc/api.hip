@@ -93,9 +93,11 @@ static int plan_crs_tiles(pwm_matrix* h) {
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "tiles plan");
     std::vector<int64_t> sa(nt + 1), sb(nt + 1);
+    h->tiles_carry = false;
     for (int32_t i = 0; i < nt; ++i) {
         sa[i] = t[i].a0;
         sb[i] = t[i].b0;
+        if (!t[i].stream) h->tiles_carry = true;
     }
     sa[nt] = h->nor;
     sb[nt] = h->nnz;
@@ -117,8 +119,10 @@ static hipError_t crs_kernel_launch(pwm_matrix* h, int which, const void* x, voi
     const CrsDev A = crs_of(h);
     hipError_t e;
     if (which == 2) {
+        // a plain SpMV whose tiles are all row-aligned stream tiles has no
+        // carries to add; power steps always finish in the fix-up (norms)
         e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, pw, h->stream);
-        if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, pw, h->stream);
+        if (e == hipSuccess && (pw || h->tiles_carry)) e = pwmk::launch_merge_fixup(A, h->mp, y, pw, h->stream);
         return e;
     }
     e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, pw, h->stream);
@@ -941,7 +945,7 @@ extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int6
                 if (h->crs_used == 2) {
                     e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, nullptr, h->stream);
                     (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                    if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+                    if (e == hipSuccess && h->tiles_carry) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
                     break;
                 }
                 e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, nullptr, h->stream);

@@ -157,6 +157,7 @@ struct pwm_matrix {
     int crs_kernel = 0;
     int crs_used = 1;
     pwm_impl::DevBuf tiles;
+    bool tiles_carry = true;  // some tile is a merge tile (its carry needs the fix-up)
 
     // merge plan
     int ipt = 8;

@@ -230,3 +230,33 @@ def test_power_method_eigenvalue_poisson(cls):
     lam, _, its = m.power_method_device(x, y, 20000, 1e-13)
     exact = 4 + 2 * np.cos(np.pi / 21) + 2 * np.cos(np.pi / 18)
     assert its < 20000 and abs(lam - exact) <= 1e-10 * exact
+
+
+@pytest.mark.parametrize("values,tol", [("f32", 1e-5), ("bf16", 1e-5)])
+def test_csb_device_build_narrow_values(values, tol):
+    """The slab streams gather FP32 and BF16 values too (2- and 4-byte
+    elements): Poisson (exact in BF16) and, for FP32, a power-law matrix."""
+    cases = [oracle.poisson_crs(300, 200)] + ([_powerlaw(20000, cap=3000)] if values == "f32" else [])
+    for A in cases:
+        x = pw.gen_uniform(A.noc, 4)
+        ref = oracle.crs_mv(A, x)
+        for opt in ({}, {"csb_groups": 8}):
+            m = _load(pw.CSBHip, A, values=values, **opt)
+            y = np.zeros(A.nor)
+            m.mv(x, y)
+            assert _relerr(y, ref) <= tol, (values, A.nor, opt)
+            gb, gind, gdata, gblk = m.csb_layout()
+            rb, rind, rdata, rblk, _ = oracle.csb_build(_triplets_of(A), 0)
+            assert np.array_equal(gind, rind) and np.array_equal(gblk, rblk)
+            assert np.array_equal(gdata, rdata.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_empty_triplets(cls):
+    """No entries at all, given as triplets: y = 0."""
+    m = cls()
+    m.loadFromTriplets(oracle.Triplets(50, 50, [], [], []))
+    y = np.full(50, np.nan)
+    m.mv(np.ones(50), y)
+    assert np.all(y == 0)
+    assert m.stats()["nnz"] == 0

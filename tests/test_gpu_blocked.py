@@ -51,7 +51,8 @@ def test_golden(goldens, cls, idx):
     case = goldens[idx]
     A, t = load_case_matrix(case)
     print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
-    runs = ([(t, {}) for t in (1, 2, 3, 4, 8, -1)] if cls is pw.BlockCOHHip
+    runs = ([(t, {}) for t in (1, 2, 3, 4, 8, -1)] + [(t, {"coh_build": 1}) for t in (1, 3, -1)]
+            if cls is pw.BlockCOHHip
             else [(-1, {"csb_layout": lay}) for lay in CSB_LAYOUTS])
     for threads, opts in runs:
         m = cls(threads, **opts)
@@ -107,22 +108,69 @@ def test_csb_layout_duplicates_and_unbalanced(layout):
     assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13
 
 
+def _check_bands(m, ref):
+    for p in range(len(ref)):
+        g, o = m.band(p), ref[p]
+        for k in ("nnz", "blocks", "start", "rows", "beta_bits", "horizontal_blocks", "vertical_blocks"):
+            assert g[k] == o[k], (p, k)
+        for k in ("row_jump", "col_jump", "data", "row_jump_block", "col_jump_block", "block_nnz"):
+            assert np.array_equal(g[k], o[k]), (p, k)
+
+
+# BlockCOH layouts: 0 = built on the device (default), 1 = built on the host
+@pytest.mark.parametrize("build", [0, 1])
 @pytest.mark.parametrize("threads", [1, 2, 3, 8, 64, -1])
-def test_blockcoh_layout_bit_exact(threads):
+def test_blockcoh_layout_bit_exact(threads, build):
     for A in (oracle.poisson_crs(40, 31), _powerlaw(5000, cap=600), _stencil(60)):
-        m = _load(pw.BlockCOHHip, A, threads)
+        m = _load(pw.BlockCOHHip, A, threads, coh_build=build)
         P = m.info()["threads"]
         if threads > 0:
             ref = oracle.blockcoh_build(A, threads)
             assert P == len(ref)
         else:
             ref = oracle.blockcoh_build(A, P)
-        for p in range(P):
-            g, o = m.band(p), ref[p]
-            for k in ("nnz", "blocks", "start", "rows", "beta_bits", "horizontal_blocks", "vertical_blocks"):
-                assert g[k] == o[k], (p, k)
-            for k in ("row_jump", "col_jump", "data", "row_jump_block", "col_jump_block", "block_nnz"):
-                assert np.array_equal(g[k], o[k]), (p, k)
+        _check_bands(m, ref)
+
+
+def test_blockcoh_device_build_load_paths():
+    """The device build from every load path (generated Poisson, triplets with
+    duplicates and empty rows, a tall matrix whose bands have their own beta,
+    a matrix whose first row outweighs a band share) and every value type,
+    against the oracle's layout and the host build."""
+    rng = np.random.default_rng(5)
+    for values in ("f64", "f32", "bf16"):
+        for threads in (1, 7, -1):
+            m = pw.BlockCOHHip(threads, values)
+            m.generatePoissonMatrix(50, 37, 1)
+            P = m.info()["threads"]
+            _check_bands(m, oracle.blockcoh_build(oracle.poisson_crs(50, 37), P))
+    n, nc = 3000, 700
+    rows = rng.integers(0, n, 40000).astype(np.int32)
+    rows[:5000] = 0
+    cols = rng.integers(0, nc, 40000).astype(np.int32)
+    vals = rng.integers(-8, 9, 40000).astype(np.float64)
+    t = oracle.Triplets(n, nc, rows, cols, vals)
+    A = oracle.triplet_to_crs(t)
+    for values in ("f64", "bf16"):
+        for threads in (1, 5, 40):
+            ms = [pw.BlockCOHHip(threads, values, coh_build=b) for b in (0, 1)]
+            for m in ms:
+                m.loadFromTriplets(t)
+            P = ms[0].info()["threads"]
+            _check_bands(ms[0], oracle.blockcoh_build(A, P))
+            x = rng.standard_normal(nc)
+            ref = oracle.crs_mv(A, x)
+            # BF16 storage computes in FP32 (x rounded, products and sums in
+            # FP32): per row at most (n_i + 2) * 2^-24 * (|A| |x|)_i off
+            absA = oracle.CRSMatrix(A.nor, A.noc, A.row_start, A.col_ind, np.abs(A.data))
+            bound = (np.diff(A.row_start) + 2) * 2.0 ** -24 * oracle.crs_mv(absA, np.abs(x))
+            for m in ms:
+                y = np.zeros(n)
+                m.mv(x, y)
+                if values == "f64":
+                    assert _relerr(y, ref) <= 1e-13
+                else:
+                    assert np.all(np.abs(y - ref) <= bound), np.max(np.abs(y - ref) / np.maximum(bound, 1e-300))
 
 
 # ---------------------------------------------------------------- SpMV values
@@ -138,7 +186,7 @@ def test_spmv_random_and_stencil(cls):
                     {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
                     {"csb_layout": 2, "block_bits": 3, "csb_groups": 5}, {"csb_unroll": 2, "csb_groups": 1},
                     {"csb_unroll": 4, "csb_groups": 8})
-        for opt in csb_opts if cls is pw.CSBHip else ({}, ):
+        for opt in csb_opts if cls is pw.CSBHip else ({}, {"coh_build": 1}):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)
                 y = np.zeros(A.nor)

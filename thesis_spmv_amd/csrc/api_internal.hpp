@@ -13,6 +13,10 @@
 #include "../../include/pwm_hip.h"
 #include "kernels.hpp"
 
+namespace blocked {
+struct CohTask;
+}
+
 namespace pwm_impl {
 
 // Host-side parallel loop over [0, n) in chunks (std::thread; the library
@@ -169,6 +173,7 @@ struct pwm_matrix {
     int32_t csb_groups = 0;  // 0 = autotune
     int32_t csb_layout = 0;  // 0 = streams when the packed index fits, 1 = pieces, 2 = streams
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
+    int32_t coh_build = 0;   // BlockCOH layout: 0 = built on the device, 1 = on the host
     pwm_impl::CsbDev csb;
     pwm_impl::CohDev coh;
 
@@ -189,6 +194,14 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
 int csb_key_bits(int64_t cells, int bits);
 int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S);
+// BlockCOH layout built on the device from the device CRS arrays
+// (coh_build.hip); h->coh.bands holds the band partition (start, rows, nnz,
+// beta_bits, H, V, off_nz) on entry and the per-band sizes and offsets on
+// return.  coh_plan_tasks: the kernel's work list for those bands, decode
+// states left zero.
+int coh_layout_device(pwm_matrix* h, int64_t lds_rows);
+void coh_plan_tasks(const std::vector<CohBandMeta>& B, int64_t lds_rows, std::vector<blocked::CohTask>& tasks,
+                    std::vector<int32_t>& atomic_bands);
 // Statistics of the loaded CRS arrays (stats.hip) into h->stats; needs the
 // device row_start / col_ind.
 int compute_matrix_stats(pwm_matrix* h);

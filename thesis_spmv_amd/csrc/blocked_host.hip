@@ -352,10 +352,33 @@ static void coh_bands(int64_t nor, int64_t nnz, const int32_t* rs, int threads, 
     bnnz[pid] = sum;
 }
 
+// Work list: one LDS task per band whose rows fit the LDS window and whose
+// entries fit one task, atomic tasks of at most kCohTaskChunks chunks
+// otherwise (decode states left zero for the caller to fill).
+void coh_plan_tasks(const std::vector<CohBandMeta>& B, int64_t lds_rows, std::vector<blocked::CohTask>& tasks,
+                    std::vector<int32_t>& atomic_bands) {
+    tasks.clear();
+    atomic_bands.clear();
+    for (int p = 0; p < (int)B.size(); ++p) {
+        const CohBandMeta& m = B[p];
+        const int64_t nch = (m.nnz + blocked::kCohChunk - 1) / blocked::kCohChunk;
+        if (m.rows <= lds_rows && nch <= blocked::kCohTaskChunks) {
+            if (m.rows > 0) tasks.push_back({p, 0, (int32_t)nch, 0, 0, 0});
+            continue;
+        }
+        atomic_bands.push_back(p);
+        for (int64_t c0 = 0; c0 < nch; c0 += blocked::kCohTaskChunks)
+            tasks.push_back({p, (int32_t)c0, (int32_t)std::min<int64_t>(blocked::kCohTaskChunks, nch - c0), 1, 0, 0});
+    }
+}
+
+// rs: host row_start; ci / vals: host CRS arrays, needed only by the host
+// layout build (option coh_build = 1; the device build reads the device CRS).
 int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
     const int64_t nor = h->nor, noc = h->noc, nnz = h->nnz;
+    const bool device = h->coh_build != 1;
     int rc = PWM_OK;
-    check_bf16(h, vals, nnz, &rc);
+    if (!device) check_bf16(h, vals, nnz, &rc);
     if (rc != PWM_OK) return rc;
     const int64_t lds_rows = blocked::lds_rows(h->vtype);
     std::vector<int64_t> bstart, bnnz;
@@ -390,6 +413,7 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         m.off_nz = tot_nz;
         tot_nz += m.nnz;
     }
+    if (device) return coh_layout_device(h, lds_rows);
     // Per band: Hilbert order of blocks (hilbert_size/beta grid), ICRS inside
     // each block, BICRS jumps between blocks.
     std::vector<uint16_t> col_jump(std::max<int64_t>(nnz, 1)), row_jump(std::max<int64_t>(nnz, 1));
@@ -521,42 +545,39 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
         all_bnz.insert(all_bnz.end(), bnz[p].begin(), bnz[p].end());
     }
     c.total_blocks = (int64_t)allb.size();
-    // work list: one LDS task per band that fits, atomic tasks of at most
-    // kCohTaskChunks chunks otherwise, each with the decode state (S, R) of
-    // the entry before its first one
+    // work list; each atomic task starts from the decode state (S, R) of the
+    // entry before its first one
     std::vector<blocked::CohTask> tasks;
     std::vector<int32_t> atomic_bands;
-    for (int p = 0; p < threads; ++p) {
-        const CohBandMeta& m = B[p];
-        const int64_t nch = (m.nnz + blocked::kCohChunk - 1) / blocked::kCohChunk;
-        if (m.rows <= lds_rows && nch <= blocked::kCohTaskChunks) {
-            if (m.rows > 0) tasks.push_back({p, 0, (int32_t)nch, 0, 0, 0});
-            continue;
-        }
-        atomic_bands.push_back(p);
-        if (nch == 0) continue;
-        const int64_t beta = (int64_t)1 << m.beta_bits;
-        const uint16_t* cj = col_jump.data() + m.off_nz;
-        const uint16_t* rj = row_jump.data() + m.off_nz;
-        const std::vector<blocked::CohBlock>& bl = blocks[p];
-        int64_t S = 0, R = 0, b = 0;
-        for (int64_t c0 = 0; c0 < nch; c0 += blocked::kCohTaskChunks) {
-            // decode up to the entry before c0*kCohChunk
-            const int64_t stop = c0 * blocked::kCohChunk;
-            for (int64_t e = (c0 == 0 ? 0 : (c0 - blocked::kCohTaskChunks) * blocked::kCohChunk); e < stop; ++e) {
-                while (b + 1 < (int64_t)bl.size() && bl[b + 1].entry <= e) ++b;
-                if (bl[b].entry == e) {
-                    S = cj[e];
-                    R = rj[bl[b].row_jump];
-                } else {
-                    S += cj[e];
-                    if ((S >> m.beta_bits) != ((S - cj[e]) >> m.beta_bits)) R += rj[bl[b].row_jump + (S >> m.beta_bits)];
+    coh_plan_tasks(B, lds_rows, tasks, atomic_bands);
+    for (size_t t0 = 0; t0 < tasks.size();) {
+        size_t t1 = t0 + 1;
+        while (t1 < tasks.size() && tasks[t1].band == tasks[t0].band) ++t1;
+        if (tasks[t0].atomic) {
+            const int p = tasks[t0].band;
+            const CohBandMeta& m = B[p];
+            const uint16_t* cj = col_jump.data() + m.off_nz;
+            const uint16_t* rj = row_jump.data() + m.off_nz;
+            const std::vector<blocked::CohBlock>& bl = blocks[p];
+            int64_t S = 0, R = 0, b = 0, e = 0;
+            for (size_t t = t0; t < t1; ++t) {
+                // decode up to the entry before the task's first one
+                const int64_t stop = (int64_t)tasks[t].chunk0 * blocked::kCohChunk;
+                for (; e < stop; ++e) {
+                    while (b + 1 < (int64_t)bl.size() && bl[b + 1].entry <= e) ++b;
+                    if (bl[b].entry == e) {
+                        S = cj[e];
+                        R = rj[bl[b].row_jump];
+                    } else {
+                        S += cj[e];
+                        if ((S >> m.beta_bits) != ((S - cj[e]) >> m.beta_bits)) R += rj[bl[b].row_jump + (S >> m.beta_bits)];
+                    }
                 }
+                tasks[t].carry_s = (int32_t)S;
+                tasks[t].carry_r = (int32_t)R;
             }
-            (void)beta;
-            tasks.push_back({p, (int32_t)c0, (int32_t)std::min<int64_t>(blocked::kCohTaskChunks, nch - c0), 1,
-                             (int32_t)S, (int32_t)R});
         }
+        t0 = t1;
     }
     c.n_atomic = (int64_t)atomic_bands.size();
     hipError_t e;

@@ -10,8 +10,9 @@ Poisson-arrival requests on the continuous-batching server (p50/p99 latency
 and solves/s against the FP64 per-request path).  Otherwise:
 
 A step is one SpMV y = A x over the whole matrix with A, x, y resident in HBM.
-W untimed steps, then K timed steps between two device synchronisations;
-`value` = algorithmic bytes per SpMV / wall time per step.  The algorithmic
+W untimed steps, then K timed steps launched back to back (no event or
+marker between them) between two device synchronisations; `value` =
+algorithmic bytes per SpMV / wall time per step.  The algorithmic
 bytes of an FP64/int32 CRS SpMV are 12 Z + 4 (N+1) + 16 N (values and
 column indices once, row offsets once, x and y once).
 
@@ -507,9 +508,12 @@ def main():
         t_load = time.time() - t_load
         for _ in range(args.warmup):
             m.time_mv(x, y, 1)
+        # per-step events: the SpMV's device time and its dominant kernel
+        ms_ev, kms = m.time_mv(x, y, args.steps)
+        # wall: the same SpMVs back to back, no markers between the steps
         pw.synchronize()
         t0 = time.perf_counter()
-        ms_ev, kms = m.time_mv(x, y, args.steps)
+        m.time_mv(x, y, args.steps, per_step=False)
         pw.synchronize()
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)

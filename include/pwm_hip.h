@@ -203,7 +203,9 @@ int pwm_device_copy(void* dst, const void* src, int64_t bytes);
  * bytes is rewritten before each SpMV (outside the events' spans) so the
  * matrix is not served from the 256 MB Infinity Cache.  Writes the mean
  * milliseconds per SpMV of the whole launch sequence and, if kernel_ms is
- * not NULL, of the dominant kernel alone. */
+ * not NULL, of the dominant kernel alone.  With kernel_ms NULL and no flush
+ * the reps SpMVs run back to back between one pair of events (no markers
+ * between steps). */
 int pwm_time_mv(pwm_matrix* h, const void* x_dev, void* y_dev, int reps, int64_t flush_bytes,
                 double* ms_per_mv, double* kernel_ms);
 

@@ -445,3 +445,20 @@ def test_gather_rate_probe():
         pw.gather_rate(x, 1 << 14, True, 1 << 16)  # eight windows exceed x
     with pytest.raises(pw.PwmError):
         pw.gather_rate(x, 1 << 13, True, 100)  # fewer entries than one workgroup
+
+
+@pytest.mark.parametrize("cls", [pw.CRSHip, pw.CRSMergeHip, pw.CSBHip, pw.BlockCOHHip], ids=lambda c: c.__name__)
+def test_time_mv_back_to_back(cls):
+    """pwm_time_mv without the kernel time runs the SpMVs back to back and
+    leaves the SpMV's result in y, as with per-step events."""
+    A = oracle.poisson_crs(120, 90)
+    m = cls()
+    m.generatePoissonMatrix(120, 90, 1)
+    xh = pw.gen_uniform(A.noc, 3)
+    x, y = pw.DeviceVector.from_array(xh), pw.DeviceVector(A.nor)
+    ms, kms = m.time_mv(x, y, 7, per_step=False)
+    assert ms > 0 and kms is None
+    ref = oracle.crs_mv(A, xh)
+    assert np.max(np.abs(y.download() - ref)) <= 1e-13 * np.max(np.abs(ref))
+    ms2, kms2 = m.time_mv(x, y, 3)
+    assert ms2 > 0 and 0 < kms2 <= ms2 * 1.5

@@ -274,11 +274,14 @@ class SparseMatrixHip:
                                              C.byref(nrm), C.byref(its)), self.h)
         return lam.value, nrm.value, its.value
 
-    def time_mv(self, x, y, reps, flush_bytes=0):
+    def time_mv(self, x, y, reps, flush_bytes=0, per_step=True):
+        """(ms per SpMV, ms of its dominant kernel) over `reps` SpMVs timed
+        with HIP events per step; per_step=False: the SpMVs run back to back
+        between one pair of events and the kernel time is None."""
         ms, kms = F64(), F64()
-        _check(lib().pwm_time_mv(self.h, x.ptr, y.ptr, int(reps), int(flush_bytes), C.byref(ms), C.byref(kms)),
-               self.h)
-        return ms.value, kms.value
+        _check(lib().pwm_time_mv(self.h, x.ptr, y.ptr, int(reps), int(flush_bytes), C.byref(ms),
+                                 C.byref(kms) if per_step else None), self.h)
+        return ms.value, (kms.value if per_step else None)
 
     def crs_export(self):
         """The CRS arrays held on the device (values widened to double)."""

@@ -185,7 +185,8 @@ def test_spmv_random_and_stencil(cls):
                     {"csb_groups": 16}, {"block_bits": 7, "csb_groups": 16}, {"csb_layout": 1},
                     {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
                     {"csb_layout": 2, "block_bits": 3, "csb_groups": 5}, {"csb_unroll": 2, "csb_groups": 1},
-                    {"csb_unroll": 4, "csb_groups": 8})
+                    {"csb_unroll": 4, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 1},
+                    {"csb_nt": 0, "csb_unroll": 4})
         for opt in csb_opts if cls is pw.CSBHip else ({}, {"coh_build": 1}):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)

@@ -359,14 +359,15 @@ def test_cpp_dropin_classes():
 # ---------------------------------------------------------------- CRS kernels
 # The CRS format runs either the vector kernel (lanes per row) or the
 # row-aligned tiles kernel (tiles of whole short rows, long rows split with
-# carries); by default it times both at load.  Each is checked on its own.
+# carries); by default it times both at load, the tiles kernel with plain and
+# with non-temporal entry loads.  Each is checked on its own.
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel,nt", [(1, -1), (2, 0), (2, 1)])
 @pytest.mark.parametrize("idx", range(24))
-def test_golden_each_crs_kernel(goldens, kernel, idx):
+def test_golden_each_crs_kernel(goldens, kernel, nt, idx):
     case = goldens[idx]
     A, t = load_case_matrix(case)
-    m = pw.CRSHip(-1, "f64", crs_kernel=kernel)
+    m = pw.CRSHip(-1, "f64", crs_kernel=kernel, crs_nt=nt)
     if case["matrix"]["kind"] == "poisson":
         m.generatePoissonMatrix(case["matrix"]["m"], case["matrix"]["n"], 1)
     else:
@@ -392,14 +393,14 @@ def _long_row_matrix(rng, n=16000):
     return oracle.CRSMatrix(n, n, rs, ci, rng.standard_normal(int(rs[-1])))
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel,nt", [(1, -1), (2, 0), (2, 1)])
 @pytest.mark.parametrize("values,tol", [("f64", 1e-13), ("f32", 5e-5), ("bf16", 5e-5)])
-def test_crs_kernels_long_rows(kernel, values, tol):
+def test_crs_kernels_long_rows(kernel, nt, values, tol):
     rng = np.random.default_rng(17)
     A = _long_row_matrix(rng)
     if values == "bf16":
         A = oracle.CRSMatrix(A.nor, A.noc, A.row_start, A.col_ind, np.round(A.data * 8) / 8)
-    m = _load(pw.CRSHip, A, values=values, crs_kernel=kernel)
+    m = _load(pw.CRSHip, A, values=values, crs_kernel=kernel, crs_nt=nt)
     info = m.info()
     assert (info["group_size"] == 0) == (kernel == 2)
     x = rng.uniform(-1, 1, A.nor)
@@ -410,12 +411,12 @@ def test_crs_kernels_long_rows(kernel, values, tol):
     assert _relerr(y, oracle.crs_mv(A, x)) <= tol
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
-def test_crs_kernels_power_method(kernel):
+@pytest.mark.parametrize("kernel,nt", [(1, -1), (2, 0), (2, 1)])
+def test_crs_kernels_power_method(kernel, nt):
     rng = np.random.default_rng(5)
     A = _long_row_matrix(rng, 4000)
     A = oracle.CRSMatrix(A.nor, A.noc, A.row_start, A.col_ind, np.abs(A.data) + 0.1)
-    m = _load(pw.CRSHip, A, crs_kernel=kernel)
+    m = _load(pw.CRSHip, A, crs_kernel=kernel, crs_nt=nt)
     x0 = rng.uniform(0, 1, A.nor)
     xd, yd = pw.DeviceVector.from_array(x0), pw.DeviceVector(A.nor)
     lam, yn, its = m.power_method_device(xd, yd, 40, -1.0)

@@ -67,6 +67,7 @@ static CrsDev crs_of(const pwm_matrix* h) {
     A.ci = h->ci.as<int32_t>();
     A.val = h->val.p;
     A.vtype = h->vtype;
+    A.nt = h->crs_nt_used;
     return A;
 }
 
@@ -224,7 +225,8 @@ int plan_crs(pwm_matrix* h) {
     const bool tiles = h->crs_kernel != 1;
     int rc;
     if (tiles && (rc = plan_crs_tiles(h)) != PWM_OK) return rc;
-    if ((int)gs.size() + (tiles ? 1 : 0) == 1) {
+    h->crs_nt_used = h->crs_nt > 0 ? 1 : 0;
+    if ((int)gs.size() + (tiles ? 1 : 0) == 1 && (!tiles || h->crs_nt >= 0)) {
         h->crs_used = tiles ? 2 : 1;
         return plan_crs_vector(h, tiles ? G0 : gs[0]);
     }
@@ -237,13 +239,22 @@ int plan_crs(pwm_matrix* h) {
     e = pwmk::launch_fill(x.p, h->noc, h->vtype == PWM_VALUES_F64 ? 1 : 0, 1.0, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "CRS autotune vectors");
     float best = -1.0f;
-    int best_kernel = 1, best_g = G0;
+    int best_kernel = 1, best_g = G0, best_nt = 0;
     if (tiles) {
-        float ms = 0.0f;
-        if ((e = time_crs_kernel(h, 2, x.p, y.p, &ms)) != hipSuccess) return hip_fail(h, e, "CRS autotune");
-        best = ms;
-        best_kernel = 2;
+        // entry loads plain or non-temporal: the caller's choice, else both
+        for (int nt : {0, 1}) {
+            if (h->crs_nt >= 0 && nt != h->crs_nt) continue;
+            h->crs_nt_used = nt;
+            float ms = 0.0f;
+            if ((e = time_crs_kernel(h, 2, x.p, y.p, &ms)) != hipSuccess) return hip_fail(h, e, "CRS autotune");
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                best_kernel = 2;
+                best_nt = nt;
+            }
+        }
     }
+    h->crs_nt_used = best_nt;
     for (int g : gs) {
         if ((rc = plan_crs_vector(h, g)) != PWM_OK) return rc;
         float ms = 0.0f;
@@ -401,6 +412,12 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_unroll") {
         if (value != 0 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_unroll must be 0, 2 or 4");
         h->csb_unroll = (int32_t)value;
+    } else if (k == "crs_nt") {
+        if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "crs_nt must be -1 (timed), 0 or 1");
+        h->crs_nt = (int)value;
+    } else if (k == "csb_nt") {
+        if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "csb_nt must be -1 (timed), 0 or 1");
+        h->csb_nt = (int32_t)value;
     } else if (k == "coh_build") {
         if (value < 0 || value > 1) return fail(h, PWM_ERR_INVALID, "coh_build must be 0 (device) or 1 (host)");
         h->coh_build = (int32_t)value;
@@ -410,7 +427,8 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else {
         return fail(h, PWM_ERR_UNSUPPORTED, "unknown option " + k);
     }
-    if (h->loaded && (k == "group_size" || k == "vec_blocks" || k == "long_row" || k == "long_chunk" || k == "crs_kernel") &&
+    if (h->loaded && (k == "group_size" || k == "vec_blocks" || k == "long_row" || k == "long_chunk" || k == "crs_kernel" ||
+                      k == "crs_nt") &&
         h->format == PWM_FORMAT_CRS)
         return plan_crs(h);
     if (h->loaded && k == "items_per_thread" && h->format == PWM_FORMAT_CRS_MERGE) return plan_merge(h);

@@ -85,6 +85,7 @@ struct CsbDev {
     int32_t stream = 0;
     int32_t col_bits = 0;
     int32_t unroll = 2;          // stream kernel groups of 4 per thread per step
+    int32_t nt = 0;              // stream kernel: non-temporal entry loads / partial stores
     DevBuf soff;
     void clear() {
         bits = 0;
@@ -100,6 +101,7 @@ struct CsbDev {
         stream = 0;
         col_bits = 0;
         unroll = 2;
+        nt = 0;
         n_tasks = 0;
         slab_bits = 0;
     }
@@ -161,6 +163,8 @@ struct pwm_matrix {
     int crs_kernel = 0;
     int crs_used = 1;
     pwm_impl::DevBuf tiles;
+    int crs_nt = -1;          // tiles kernel: non-temporal entry loads (-1 = timed at load)
+    int crs_nt_used = 0;
     bool tiles_carry = true;  // some tile is a merge tile (its carry needs the fix-up)
 
     // merge plan
@@ -173,6 +177,7 @@ struct pwm_matrix {
     int32_t csb_groups = 0;  // 0 = autotune
     int32_t csb_layout = 0;  // 0 = streams when the packed index fits, 1 = pieces, 2 = streams
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
+    int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
     int32_t coh_build = 0;   // BlockCOH layout: 0 = built on the device, 1 = on the host
     pwm_impl::CsbDev csb;
     pwm_impl::CohDev coh;

@@ -44,6 +44,7 @@ struct CsbArgs {
     // (row in slab << col_bits) | column; ind/val extend 4 zero entries past
     // the last one.  slabs/slab_rows describe these slabs; pieces is unused.
     int32_t stream = 0;
+    int32_t nt = 0;          // slab streams: non-temporal entry loads and partial-slab stores
     int32_t col_bits = 0;
     const int32_t* soff = nullptr;
     int32_t unroll = 2;  // groups of 4 entries per thread per step: 2 or 4

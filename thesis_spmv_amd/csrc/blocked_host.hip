@@ -262,13 +262,24 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
         unrolls.push_back(h->csb_unroll);
     else
         unrolls = {2, 4};
+    // stream kernel loads and partial stores: non-temporal (the entries are
+    // read once; keeps more of the x window in L2) or plain (keeps a matrix
+    // that fits the Infinity Cache there); the caller's choice, else both
+    std::vector<int32_t> nts;
+    if (!c.stream)
+        nts.push_back(0);
+    else if (h->csb_nt >= 0)
+        nts.push_back(h->csb_nt);
+    else
+        nts = {0, 1};
     c.groups = cands[0];
     c.unroll = unrolls[0];
+    c.nt = nts[0];
     c.n_tasks = units * c.groups;
     hipError_t e;
     if (gmax > 1 && (e = c.ypart.alloc((size_t)gmax * nor * vector_size(h->vtype))) != hipSuccess)
         return hip_fail(h, e, "CSB partial sums");
-    if (cands.size() * unrolls.size() > 1) {
+    if (cands.size() * unrolls.size() * nts.size() > 1) {
         // autotune: a few timed SpMVs per candidate on x = 1
         DevBuf tx, ty;
         const size_t vs = vector_size(h->vtype);
@@ -279,29 +290,33 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
         (void)hipEventCreate(&e0);
         (void)hipEventCreate(&e1);
         float best = 1e30f;
-        int32_t best_g = cands[0], best_u = unrolls[0];
+        int32_t best_g = cands[0], best_u = unrolls[0], best_nt = nts[0];
         for (int32_t gc : cands)
-            for (int32_t uc : unrolls) {
-                c.groups = gc;
-                c.unroll = uc;
-                c.n_tasks = units * gc;
-                for (int w = 0; w < 2; ++w) (void)csb_launch(h, tx.p, ty.p, nullptr);
-                (void)hipEventRecord(e0, h->stream);
-                for (int r = 0; r < 5; ++r) (void)csb_launch(h, tx.p, ty.p, nullptr);
-                (void)hipEventRecord(e1, h->stream);
-                (void)hipEventSynchronize(e1);
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best) {
-                    best = ms;
-                    best_g = gc;
-                    best_u = uc;
+            for (int32_t uc : unrolls)
+                for (int32_t ntc : nts) {
+                    c.groups = gc;
+                    c.unroll = uc;
+                    c.nt = ntc;
+                    c.n_tasks = units * gc;
+                    for (int w = 0; w < 2; ++w) (void)csb_launch(h, tx.p, ty.p, nullptr);
+                    (void)hipEventRecord(e0, h->stream);
+                    for (int r = 0; r < 5; ++r) (void)csb_launch(h, tx.p, ty.p, nullptr);
+                    (void)hipEventRecord(e1, h->stream);
+                    (void)hipEventSynchronize(e1);
+                    float ms = 0;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    if (ms < best) {
+                        best = ms;
+                        best_g = gc;
+                        best_u = uc;
+                        best_nt = ntc;
+                    }
                 }
-            }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c.groups = best_g;
         c.unroll = best_u;
+        c.nt = best_nt;
         c.n_tasks = units * best_g;
         if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(h, e, "CSB autotune");
     }
@@ -628,6 +643,7 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.col_bits = h->csb.col_bits;
     a.soff = h->csb.soff.as<int32_t>();
     a.unroll = h->csb.unroll;
+    a.nt = h->csb.nt;
     hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream, part);
     if (e == hipSuccess && pw && part != 1)
         e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);

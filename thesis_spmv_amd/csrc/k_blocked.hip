@@ -118,7 +118,7 @@ __device__ __forceinline__ void csb_task(const CsbArgs& a, int& g, int64_t& t) {
 
 // Writes a slab's LDS sums: to ypart[g] with several column groups, else to
 // y (POWER: scaled, with the slab's |y|^2, x.y, x.x in partial slot t).
-template <typename X, typename Acc, int BS, bool POWER>
+template <typename X, typename Acc, int BS, bool POWER, bool NT = false>
 __device__ __forceinline__ void csb_write_slab(const CsbArgs& a, const Acc* ylds, int h, int64_t row0, int g, int64_t t,
                                                const X* __restrict__ x, X* __restrict__ y, double scale,
                                                const PowerArgs& pw, double* red) {
@@ -128,7 +128,10 @@ __device__ __forceinline__ void csb_write_slab(const CsbArgs& a, const Acc* ylds
         for (int i = tid; i < h; i += BS) {
             const int64_t row = row0 + i;
             if (row >= a.nor) break;
-            yp[row] = ylds[i];
+            if constexpr (NT)
+                __builtin_nontemporal_store(ylds[i], &yp[row]);
+            else
+                yp[row] = ylds[i];
         }
         return;
     }
@@ -263,8 +266,9 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
 // so the loads carry no branch.
 constexpr int kSBS = 256;
 
-// kSU: groups of 4 entries per thread per step (2 or 4)
-template <typename VT, bool POWER, int kSU>
+// kSU: groups of 4 entries per thread per step (2 or 4); NT: non-temporal
+// entry loads and partial-slab stores
+template <typename VT, bool POWER, int kSU, bool NT>
 __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                                      typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
     using X = typename Arith<VT>::X;
@@ -305,8 +309,13 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
         for (int u = 0; u < kSU; ++u) {
             const int32_t q = qb + tid + u * kSBS;
             const int64_t qc = q < q1 ? q : q1 - 1;
-            w[u] = *pwmk::G((const pwmk::vi4*)(ind + 4 * qc));
-            v[u].load(val + 4 * qc);
+            if constexpr (NT) {
+                w[u] = __builtin_nontemporal_load(pwmk::G((const pwmk::vi4*)(ind + 4 * qc)));
+                v[u].load_nt(val + 4 * qc);
+            } else {
+                w[u] = *pwmk::G((const pwmk::vi4*)(ind + 4 * qc));
+                v[u].load(val + 4 * qc);
+            }
         }
         X xv[kSU][4];
 #pragma unroll
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
     __syncthreads();
     const int64_t beta = (int64_t)1 << a.bits;
     const int64_t row0 = (t / a.slabs) * beta + (t % a.slabs) * (int64_t)h;
-    csb_write_slab<X, Acc, kSBS, POWER>(a, ylds, h, row0, g, t, x, y, scale, pw, red);
+    csb_write_slab<X, Acc, kSBS, POWER, NT>(a, ylds, h, row0, g, t, x, y, scale, pw, red);
 }
 
 // y[r] = sum over the column groups of ypart[g][r], groups in order (and the
@@ -627,18 +636,22 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     } else if (a.stream) {
         const size_t lds = (size_t)a.slab_rows * sizeof(Acc);
         const dim3 grid((unsigned)a.n_tasks);
-        if (pw && a.unroll == 4)
-            hipLaunchKernelGGL((k_csb_stream<VT, true, 4>), grid, dim3(kSBS), lds, s, a, (const X*)nullptr,
-                               (X*)nullptr, *pw);
-        else if (pw)
-            hipLaunchKernelGGL((k_csb_stream<VT, true, 2>), grid, dim3(kSBS), lds, s, a, (const X*)nullptr,
-                               (X*)nullptr, *pw);
-        else if (a.unroll == 4)
-            hipLaunchKernelGGL((k_csb_stream<VT, false, 4>), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
-                               PowerArgs{});
-        else
-            hipLaunchKernelGGL((k_csb_stream<VT, false, 2>), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
-                               PowerArgs{});
+        const bool four = a.unroll == 4;
+        if (pw) {
+            if (a.nt)
+                hipLaunchKernelGGL((four ? k_csb_stream<VT, true, 4, true> : k_csb_stream<VT, true, 2, true>), grid,
+                                   dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
+            else
+                hipLaunchKernelGGL((four ? k_csb_stream<VT, true, 4, false> : k_csb_stream<VT, true, 2, false>),
+                                   grid, dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
+        } else {
+            if (a.nt)
+                hipLaunchKernelGGL((four ? k_csb_stream<VT, false, 4, true> : k_csb_stream<VT, false, 2, true>),
+                                   grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
+            else
+                hipLaunchKernelGGL((four ? k_csb_stream<VT, false, 4, false> : k_csb_stream<VT, false, 2, false>),
+                                   grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
+        }
     } else if (pw) {
         hipLaunchKernelGGL((k_csb<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
                            (X*)nullptr, *pw);

@@ -17,7 +17,7 @@ namespace pwmk {
 constexpr int kTBS = 256;
 constexpr int kTilesIPT = 8;
 
-template <typename VT, bool POWER>
+template <typename VT, bool POWER, bool NT>
 __global__ __launch_bounds__(kTBS) void k_tiles_spmv(CrsDev A, const ETile* __restrict__ tiles, MergePlan mp,
                                                      const typename Arith<VT>::X* __restrict__ x_arg,
                                                      typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kTBS) void k_tiles_spmv(CrsDev A, const ETile* __re
     const int na = T.a1 - T.a0;
     TileRegs<VT, kTilesIPT> R;
     X xr[kTilesIPT];
-    tile_load<VT, kTBS, kTilesIPT>(R, A.ci, (const VT*)A.val, T.b0, T.b1);
+    tile_load<VT, kTBS, kTilesIPT, NT>(R, A.ci, (const VT*)A.val, T.b0, T.b1);
     if (POWER && T.stream) stream_xrows<VT, kTBS, kTilesIPT>(x, T.a0, na, xr);
     tile_stage<VT, kTBS, kTilesIPT>(R, A.rs, x, T.a0, na, T.b0, T.b1, sm);
     __syncthreads();
@@ -69,11 +69,11 @@ static hipError_t launch_tiles_t(const CrsDev& A, const ETile* tiles, const Merg
                                  const PowerArgs* pw, hipStream_t s) {
     using X = typename Arith<VT>::X;
     if (pw)
-        hipLaunchKernelGGL((k_tiles_spmv<VT, true>), dim3((unsigned)mp.P), dim3(kTBS), 0, s, A, tiles, mp,
-                           (const X*)nullptr, (X*)nullptr, *pw);
+        hipLaunchKernelGGL((A.nt ? k_tiles_spmv<VT, true, true> : k_tiles_spmv<VT, true, false>), dim3((unsigned)mp.P),
+                           dim3(kTBS), 0, s, A, tiles, mp, (const X*)nullptr, (X*)nullptr, *pw);
     else
-        hipLaunchKernelGGL((k_tiles_spmv<VT, false>), dim3((unsigned)mp.P), dim3(kTBS), 0, s, A, tiles, mp, (const X*)x,
-                           (X*)y, PowerArgs{});
+        hipLaunchKernelGGL((A.nt ? k_tiles_spmv<VT, false, true> : k_tiles_spmv<VT, false, false>),
+                           dim3((unsigned)mp.P), dim3(kTBS), 0, s, A, tiles, mp, (const X*)x, (X*)y, PowerArgs{});
     return hipGetLastError();
 }
 

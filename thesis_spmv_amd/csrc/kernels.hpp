@@ -18,6 +18,7 @@ struct CrsDev {
     const int32_t* ci = nullptr;
     const void* val = nullptr;
     int vtype = VT_F64;
+    int nt = 0;  // tiles kernel: non-temporal col_ind / value loads
 };
 
 // A slice of one long row handled by one workgroup: entries [k0, k1) of row

@@ -54,6 +54,7 @@ template <typename VT> struct Quad;
 template <> struct Quad<float> {
     vf4 q;
     __device__ __forceinline__ void load(const float* p) { q = *G((const vf4*)p); }
+    __device__ __forceinline__ void load_nt(const float* p) { q = __builtin_nontemporal_load(G((const vf4*)p)); }
     __device__ __forceinline__ void set(const float* t) { q = vf4{t[0], t[1], t[2], t[3]}; }
     __device__ __forceinline__ float get(int e) const { return q[e]; }
 };
@@ -62,6 +63,10 @@ template <> struct Quad<double> {
     __device__ __forceinline__ void load(const double* p) {
         a = *G((const vd2*)p);
         b = *G((const vd2*)(p + 2));
+    }
+    __device__ __forceinline__ void load_nt(const double* p) {
+        a = __builtin_nontemporal_load(G((const vd2*)p));
+        b = __builtin_nontemporal_load(G((const vd2*)(p + 2)));
     }
     __device__ __forceinline__ void set(const double* t) {
         a = vd2{t[0], t[1]};
@@ -72,6 +77,7 @@ template <> struct Quad<double> {
 template <> struct Quad<bf16_t> {
     vu2 q;
     __device__ __forceinline__ void load(const bf16_t* p) { q = *G((const vu2*)p); }
+    __device__ __forceinline__ void load_nt(const bf16_t* p) { q = __builtin_nontemporal_load(G((const vu2*)p)); }
     __device__ __forceinline__ void set(const bf16_t* t) {
         q.x = (uint32_t)t[0].u | ((uint32_t)t[1].u << 16);
         q.y = (uint32_t)t[2].u | ((uint32_t)t[3].u << 16);
@@ -100,7 +106,8 @@ struct TileRegs {
 // Every lane loads a group (lanes past the tile's last group reload that
 // group), so the loads need no branch and nothing waits on them until
 // tile_stage.  Requires kTilePad zero entries after col_ind/values.
-template <typename VT, int BS, int ITEMS>
+// NT: non-temporal loads (entries read once leave the L2 to the x gathers).
+template <typename VT, int BS, int ITEMS, bool NT = false>
 __device__ __forceinline__ void tile_load(TileRegs<VT, ITEMS>& R, const int32_t* __restrict__ ci,
                                           const VT* __restrict__ val, int64_t b0, int64_t b1) {
     constexpr int VU = TileRegs<VT, ITEMS>::VU;
@@ -112,8 +119,13 @@ __device__ __forceinline__ void tile_load(TileRegs<VT, ITEMS>& R, const int32_t*
     for (int u = 0; u < VU; ++u) {
         const int q = tid + u * BS < qmax ? tid + u * BS : qmax;
         const int64_t g = g0 + 4 * (int64_t)q;
-        R.jq[u] = *G((const vi4*)(ci + g));
-        R.vq[u].load(val + g);
+        if constexpr (NT) {
+            R.jq[u] = __builtin_nontemporal_load(G((const vi4*)(ci + g)));
+            R.vq[u].load_nt(val + g);
+        } else {
+            R.jq[u] = *G((const vi4*)(ci + g));
+            R.vq[u].load(val + g);
+        }
     }
 }
 

@@ -441,6 +441,10 @@ def main():
                     help="CRS kernel: 0 = time the candidates at load, 1 = vector, 2 = row-aligned tiles")
     ap.add_argument("--csb-groups", type=int, default=0, help="CSB block-column groups (0 = time 1, 8, 16 at load)")
     ap.add_argument("--csb-unroll", type=int, default=0, help="CSB stream kernel unroll (0 = time 2 and 4 at load)")
+    ap.add_argument("--csb-nt", type=int, default=-1,
+                    help="CSB stream kernel non-temporal entry loads: 1, 0, or -1 = time both at load")
+    ap.add_argument("--crs-nt", type=int, default=-1,
+                    help="CRS tiles kernel non-temporal entry loads: 1, 0, or -1 = time both at load")
     ap.add_argument("--values", default=None,
                     help="f64 and/or f32 SpMV passes (default f64; f64,f32 for stencil9)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -494,7 +498,12 @@ def main():
     wanted = set(args.formats.split(","))
     csb_opts = {"block_bits": 14} if args.workload == "stencil9" else {}
     csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
-    formats = (("crs", pw.CRSHip, {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}),
+    if args.csb_nt >= 0:
+        csb_opts["csb_nt"] = args.csb_nt
+    crs_opts = {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}
+    if args.crs_nt >= 0:
+        crs_opts["crs_nt"] = args.crs_nt
+    formats = (("crs", pw.CRSHip, crs_opts),
                ("merge_path", pw.CRSMergeHip, {}), ("csb", pw.CSBHip, csb_opts), ("blockcoh", pw.BlockCOHHip, {}))
     if args.workload == "stencil9":
         # the workload names beta = 2^14; the reference's own rule gives 2^15

@@ -636,22 +636,18 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     } else if (a.stream) {
         const size_t lds = (size_t)a.slab_rows * sizeof(Acc);
         const dim3 grid((unsigned)a.n_tasks);
-        const bool four = a.unroll == 4;
-        if (pw) {
-            if (a.nt)
-                hipLaunchKernelGGL((four ? k_csb_stream<VT, true, 4, true> : k_csb_stream<VT, true, 2, true>), grid,
-                                   dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
-            else
-                hipLaunchKernelGGL((four ? k_csb_stream<VT, true, 4, false> : k_csb_stream<VT, true, 2, false>),
-                                   grid, dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
-        } else {
-            if (a.nt)
-                hipLaunchKernelGGL((four ? k_csb_stream<VT, false, 4, true> : k_csb_stream<VT, false, 2, true>),
-                                   grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
-            else
-                hipLaunchKernelGGL((four ? k_csb_stream<VT, false, 4, false> : k_csb_stream<VT, false, 2, false>),
-                                   grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
-        }
+        // kernel instance for (power step, unroll, non-temporal)
+        auto pick = [&](auto power) {
+            constexpr bool P = decltype(power)::value;
+            if (a.unroll == 4) return a.nt ? k_csb_stream<VT, P, 4, true> : k_csb_stream<VT, P, 4, false>;
+            return a.nt ? k_csb_stream<VT, P, 2, true> : k_csb_stream<VT, P, 2, false>;
+        };
+        if (pw)
+            hipLaunchKernelGGL(pick(std::true_type{}), grid, dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr,
+                               *pw);
+        else
+            hipLaunchKernelGGL(pick(std::false_type{}), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
+                               PowerArgs{});
     } else if (pw) {
         hipLaunchKernelGGL((k_csb<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
                            (X*)nullptr, *pw);

@@ -646,6 +646,7 @@ def main():
                         "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
                         **({"crs_kernel": "tiles" if v["info"]["group_size"] == 0 else "vector"} if k == "crs" else {}),
                         "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
+                        "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"],
                         "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"]}
                     for k, v in results.items()},
         "power_method_us_per_iteration": round(power_ms * 1e3, 2),

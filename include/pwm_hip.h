@@ -70,6 +70,8 @@ typedef struct {
     int64_t long_rows;         /* rows routed to the long-row path */
     int32_t block_bits;        /* CSB beta = 2^block_bits (0 when not CSB) */
     int32_t balanced;          /* CSB balanced flag (CSB.hpp:549-559) */
+    int32_t unroll;            /* CSB slab-stream kernel: groups of 4 entries per thread per step */
+    int32_t non_temporal;      /* CSB slab streams / CRS tiles: 1 = non-temporal entry loads */
 } pwm_info_t;
 
 /* Constructor SparseMatrix(int threads) (Matrix/SparseMatrix.hpp:30) through

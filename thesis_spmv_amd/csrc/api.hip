@@ -670,6 +670,10 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
                                                       : h->vec_blocks;
     out->block_bits = h->format == PWM_FORMAT_CSB ? h->csb.bits : 0;
     out->balanced = h->format == PWM_FORMAT_CSB ? h->csb.balanced : 0;
+    out->unroll = h->format == PWM_FORMAT_CSB && h->csb.stream ? h->csb.unroll : 0;
+    out->non_temporal = h->format == PWM_FORMAT_CSB                         ? h->csb.nt
+                        : (h->format == PWM_FORMAT_CRS && h->crs_used == 2) ? h->crs_nt_used
+                                                                           : 0;
     const pwm_impl::DevBuf* bufs[] = {&h->rs, &h->ci, &h->val, &h->lr_chunks, &h->lr_rows, &h->lr_begin,
                                       &h->lr_partial, &h->split_a, &h->split_b, &h->carry_row, &h->carry_val, &h->tiles,
                                       &h->csb.ind, &h->csb.data, &h->csb.blk_ptr, &h->csb.work,

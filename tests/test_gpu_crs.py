@@ -463,3 +463,19 @@ def test_time_mv_back_to_back(cls):
     assert np.max(np.abs(y.download() - ref)) <= 1e-13 * np.max(np.abs(ref))
     ms2, kms2 = m.time_mv(x, y, 3)
     assert ms2 > 0 and 0 < kms2 <= ms2 * 1.5
+
+
+def test_info_reports_tuned_choices():
+    """pwm_info reports the kernel variant a handle runs: the options the
+    caller fixed, and a valid choice when the load-time tuner picked it."""
+    rs, ci, dv = pw.gen_powerlaw(50000, struct_seed=3, value_seed=4, cap=3000)
+    A = oracle.CRSMatrix(50000, 50000, rs, ci, dv)  # CSB: 25 block columns of 2^11, slab streams
+    for nt in (0, 1):
+        inf = _load(pw.CRSHip, A, crs_kernel=2, crs_nt=nt).info()
+        assert inf["group_size"] == 0 and inf["non_temporal"] == nt
+        inf = _load(pw.CSBHip, A, csb_groups=8, csb_unroll=4, csb_nt=nt).info()
+        assert (inf["group_size"], inf["unroll"], inf["non_temporal"]) == (8, 4, nt)
+    inf = _load(pw.CSBHip, A).info()
+    assert inf["unroll"] in (2, 4) and inf["non_temporal"] in (0, 1) and inf["group_size"] in (1, 8, 16)
+    inf = _load(pw.CRSHip, A, crs_kernel=1).info()
+    assert inf["group_size"] > 0 and inf["non_temporal"] == 0

@@ -532,8 +532,8 @@ extern "C" int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32
 // COO -> CRS (Util/TripletToCRS.hpp:135-172): rows ascending, columns
 // ascending inside a row, duplicates kept in input order.  The indices are
 // validated on the host (in parallel); the sort runs on the device
-// (coo_to_crs_device).  CSB and a host-built BlockCOH then take the sorted
-// arrays back; the device BlockCOH build needs only row_start.
+// (coo_to_crs_device).  A host-built BlockCOH then takes the sorted arrays
+// back; the other builds need only row_start on the host.
 extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, const int32_t* rows,
                                  const int32_t* cols, const double* vals, int partitions) {
     (void)partitions;
@@ -593,8 +593,9 @@ extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_
     if ((e = hipMemcpy(hrs.data(), h->rs.p, (nor + 1) * 4, hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(h, e, "row_start download");
     h->h_rs = hrs;
-    if (h->format == PWM_FORMAT_BLOCKCOH && h->coh_build != 1) return finish_load(h, h->h_rs.data(), nullptr, nullptr);
-    if (h->format == PWM_FORMAT_CSB || h->format == PWM_FORMAT_BLOCKCOH) {
+    if (h->format == PWM_FORMAT_CSB || (h->format == PWM_FORMAT_BLOCKCOH && h->coh_build != 1))
+        return finish_load(h, h->h_rs.data(), nullptr, nullptr);
+    if (h->format == PWM_FORMAT_BLOCKCOH) {
         std::vector<int32_t> hci(std::max<int64_t>(nnz, 1));
         std::vector<double> hdv(std::max<int64_t>(nnz, 1));
         const DevBuf& dsrc = f64 ? h->val : dtmp;
@@ -625,15 +626,16 @@ extern "C" int pwm_generate_poisson(pwm_matrix* h, int64_t m, int64_t n, int par
     e = pwmk::launch_fill_poisson(m, n, h->rs.as<int32_t>(), h->ci.as<int32_t>(), h->val.p, h->vtype, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "Poisson fill");
-    if (h->format == PWM_FORMAT_BLOCKCOH && h->coh_build != 1) {
-        // the device build needs only row_start on the host (band cuts)
+    if (h->format == PWM_FORMAT_CSB || (h->format == PWM_FORMAT_BLOCKCOH && h->coh_build != 1)) {
+        // the device builds need only row_start on the host (block-row sums,
+        // band cuts); CSB's host-built fallback downloads the rest itself
         h->h_rs.resize(N + 1);
         if ((e = hipMemcpy(h->h_rs.data(), h->rs.p, (N + 1) * 4, hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(h, e, "Poisson download");
         return finish_load(h, h->h_rs.data(), nullptr, nullptr);
     }
-    if (h->format == PWM_FORMAT_CSB || h->format == PWM_FORMAT_BLOCKCOH) {
-        // the host builders work from host CRS arrays
+    if (h->format == PWM_FORMAT_BLOCKCOH) {
+        // the host builder works from host CRS arrays
         std::vector<int32_t> rs(N + 1), ci(Z);
         std::vector<double> dv(Z);
         DevBuf tmp;

@@ -95,11 +95,33 @@ static void check_bf16(pwm_matrix* h, const double* vals, int64_t n, int* rc) {
 
 // ---------------------------------------------------------------- CSB
 
+// Host copies of the device CRS col_ind and values (widened to double), for
+// the host-built layouts when the caller has none.
+static int download_crs(pwm_matrix* h, std::vector<int32_t>& ci, std::vector<double>& vals) {
+    const int64_t nnz = h->nnz;
+    ci.resize(std::max<int64_t>(nnz, 1));
+    vals.resize(std::max<int64_t>(nnz, 1));
+    if (nnz == 0) return PWM_OK;
+    DevBuf tmp;
+    hipError_t e = hipMemcpy(ci.data(), h->ci.p, nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = tmp.alloc(nnz * 8);
+    if (e == hipSuccess) e = pwmk::launch_copy_values_f64(h->val.p, h->vtype, tmp.as<double>(), nnz, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(vals.data(), tmp.p, nnz * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "CRS download");
+    return PWM_OK;
+}
+
+// rs: host row_start; ci / vals: host CRS arrays, or NULL (values already
+// validated for the storage type) -- they are then downloaded only if the
+// host-built slab-piece layout is used.
 int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
     const int64_t nor = h->nor, noc = h->noc, nnz = h->nnz;
     int rc = PWM_OK;
-    check_bf16(h, vals, nnz, &rc);
+    if (vals) check_bf16(h, vals, nnz, &rc);
     if (rc != PWM_OK) return rc;
+    std::vector<int32_t> ci_dl;
+    std::vector<double> vals_dl;
     const int bits = h->csb_bits_override > 0 ? h->csb_bits_override : csb_reference_bits(nor, noc);
     const int64_t beta = (int64_t)1 << bits;
     const int64_t H = (noc + beta - 1) / beta, V = (nor + beta - 1) / beta;
@@ -143,6 +165,11 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
         if (rc != PWM_OK) return rc;
     } else {
         // slab pieces over the stored order, built on the host
+        if (!ci || !vals) {
+            if ((rc = download_crs(h, ci_dl, vals_dl)) != PWM_OK) return rc;
+            ci = ci_dl.data();
+            vals = vals_dl.data();
+        }
         std::vector<uint32_t> ind(std::max<int64_t>(nnz, 1));
         std::vector<double> data(std::max<int64_t>(nnz, 1));
         std::vector<int32_t> blk_ptr(H * V + 1, 0);

@@ -12,8 +12,11 @@
  *
  * Index arrays are 32-bit.  The reference instantiates int_type as `int` and
  * `unsigned int` (Tests/poisson.cpp:28,146); every index it stores is
- * non-negative and below 2^31, so both have the bits this ABI reads.  Host
- * vectors are double (the reference's data_t, Util/Constants.hpp:32).
+ * non-negative and below 2^31, so both have the bits this ABI reads.  A
+ * matrix holds at most PWM_MAX_NNZ = 2^31 - 65536 nonzeros (the kernels'
+ * 32-bit entry offsets keep that much headroom); loading more fails with
+ * PWM_ERR_INVALID.  Host vectors are double (the reference's data_t,
+ * Util/Constants.hpp:32).
  *
  * Threading: calls on one handle are not re-entrant (like the reference's
  * objects, whose CRS_Merge carry buffers are mutable members,
@@ -32,6 +35,8 @@ extern "C" {
 
 /* ---- status codes ------------------------------------------------------ */
 #define PWM_OK               0
+#define PWM_MAX_NNZ         2147418112LL  /* 2^31 - 65536 */
+
 #define PWM_ERR_INVALID     -1  /* bad argument (sizes, indices, aliasing) */
 #define PWM_ERR_IO          -2  /* file cannot be read / is malformed */
 #define PWM_ERR_HIP         -3  /* HIP runtime error (no device, launch failure) */

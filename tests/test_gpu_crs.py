@@ -479,3 +479,17 @@ def test_info_reports_tuned_choices():
     assert inf["unroll"] in (2, 4) and inf["non_temporal"] in (0, 1) and inf["group_size"] in (1, 8, 16)
     inf = _load(pw.CRSHip, A, crs_kernel=1).info()
     assert inf["group_size"] > 0 and inf["non_temporal"] == 0
+
+
+def test_nnz_limit_rejected_before_reading():
+    """A matrix over PWM_MAX_NNZ = 2^31 - 65536 nonzeros is refused with
+    PWM_ERR_INVALID before any array is read or allocated."""
+    max_nnz = (1 << 31) - 65536
+    m = pw.CRSHip()
+    tiny = np.zeros(4, np.int32)
+    rc = pw.lib().pwm_load_triplets(m.h, 10, 10, max_nnz + 1, tiny.ctypes.data, tiny.ctypes.data,
+                                    np.zeros(4).ctypes.data, 1)
+    assert rc == -1 and b"PWM_MAX_NNZ" in pw.lib().pwm_last_error(m.h)
+    assert pw.lib().pwm_generate_poisson(m.h, 20724, 20724, 1) == -1   # 5 m n just above the limit
+    m.generatePoissonMatrix(40, 30, 1)                                  # the handle stays usable
+    assert m.info()["nnz"] == oracle.poisson_crs(40, 30).nnz

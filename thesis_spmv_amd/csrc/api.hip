@@ -522,6 +522,7 @@ extern "C" int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32
     for (int64_t r = 0; r < nor; ++r)
         if (rs[r + 1] < rs[r]) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: row_start must be non-decreasing");
     const int64_t nnz = rs[nor];
+    if (nnz > PWM_MAX_NNZ) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: more than PWM_MAX_NNZ nonzeros");
     if (nnz > 0 && (!ci || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: NULL col_ind/values");
     for (int64_t k = 0; k < nnz; ++k)
         if (ci[k] < 0 || ci[k] >= noc) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: column index out of range");
@@ -538,8 +539,8 @@ extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_
                                  const int32_t* cols, const double* vals, int partitions) {
     (void)partitions;
     if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_load_triplets: NULL handle");
-    if (nor <= 0 || noc <= 0 || nnz < 0 || nor > INT32_MAX - 1 || noc > INT32_MAX || nnz > INT32_MAX)
-        return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: need 0 < nor, noc < 2^31 and 0 <= nnz < 2^31");
+    if (nor <= 0 || noc <= 0 || nnz < 0 || nor > INT32_MAX - 1 || noc > INT32_MAX || nnz > PWM_MAX_NNZ)
+        return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: need 0 < nor, noc < 2^31 and 0 <= nnz <= PWM_MAX_NNZ");
     if (nnz > 0 && (!rows || !cols || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_triplets: NULL arrays");
     // first out-of-range entry and first value not exact in BF16 (when BF16 storage)
     std::atomic<int64_t> bad_idx{INT64_MAX}, bad_val{INT64_MAX};
@@ -613,8 +614,8 @@ extern "C" int pwm_load_triplets(pwm_matrix* h, int64_t nor, int64_t noc, int64_
 extern "C" int pwm_generate_poisson(pwm_matrix* h, int64_t m, int64_t n, int partitions) {
     (void)partitions;
     if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_generate_poisson: NULL handle");
-    if (m <= 0 || n <= 0 || m * n > INT32_MAX - 1 || 5 * m * n > INT32_MAX)
-        return fail(h, PWM_ERR_INVALID, "pwm_generate_poisson: need m, n > 0 and 5*m*n < 2^31");
+    if (m <= 0 || n <= 0 || m * n > INT32_MAX - 1 || 5 * m * n > PWM_MAX_NNZ)
+        return fail(h, PWM_ERR_INVALID, "pwm_generate_poisson: need m, n > 0 and 5*m*n <= PWM_MAX_NNZ");
     reset_matrix(h);
     const int64_t N = m * n, Z = n * (m + 2 * (m - 1)) + 2 * (n - 1) * m;
     h->nor = h->noc = N;

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
     const int cb = a.col_bits;
     const uint32_t cmask = (uint32_t)((1ull << cb) - 1);
     __syncthreads();
-    const int32_t q0 = e0 >> 2, q1 = (e1 + 3) >> 2;
+    const int32_t q0 = e0 >> 2, q1 = (int32_t)(((int64_t)e1 + 3) >> 2);
     for (int32_t qb = q0; qb < q1; qb += kSBS * kSU) {
         pwmk::vi4 w[kSU];
         pwmk::Quad<VT> v[kSU];

@@ -1,8 +1,10 @@
-// Host-side builders of the CSB and BlockCOH device layouts, plus their
-// parity exports.  The stored arrays follow the reference's layouts exactly
-// (SOA_Implementations/CSB.hpp:464-567, BlockCOH.hpp:319-455); the extra
-// tables the kernels need (slab pieces, per-block offsets and coordinates)
-// are derived from them here.
+// Host side of the CSB and BlockCOH builds and their parity exports.  The
+// stored arrays follow the reference's layouts exactly
+// (SOA_Implementations/CSB.hpp:464-567, BlockCOH.hpp:319-455).  Here: the CSB
+// block size, balanced flag, layout choice and load-time tuning, the BlockCOH
+// band cuts and task list, and the host-built layouts (CSB slab pieces; the
+// BlockCOH layout when coh_build = 1).  The default layouts are sorted on the
+// device (csb_build.hip, coh_build.hip).
 #include <math.h>
 #include <string.h>
 

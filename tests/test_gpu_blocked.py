@@ -15,8 +15,10 @@ pytestmark = pytest.mark.gpu
 
 FORMATS = [pw.CSBHip, pw.BlockCOHHip]
 # CSB device layouts: 1 = slab pieces over the stored block order, 2 = one
-# contiguous stream per slab (the automatic choice when its index packs)
-CSB_LAYOUTS = [1, 2]
+# contiguous stream per slab (the automatic choice when its index packs),
+# 3 = the same streams with column and row in slab stored apart (the
+# automatic choice past 2^24 columns)
+CSB_LAYOUTS = [1, 2, 3]
 
 
 def _relerr(a, b):
@@ -186,7 +188,8 @@ def test_spmv_random_and_stencil(cls):
                     {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
                     {"csb_layout": 2, "block_bits": 3, "csb_groups": 5}, {"csb_unroll": 2, "csb_groups": 1},
                     {"csb_unroll": 4, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 1},
-                    {"csb_nt": 0, "csb_unroll": 4})
+                    {"csb_nt": 0, "csb_unroll": 4}, {"csb_layout": 3}, {"csb_layout": 3, "csb_groups": 8},
+                    {"csb_layout": 3, "csb_nt": 1, "csb_unroll": 4, "csb_groups": 16})
         for opt in csb_opts if cls is pw.CSBHip else ({}, {"coh_build": 1}):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)
@@ -217,7 +220,7 @@ def test_edge_shapes(cls):
     for A in cases:
         x = rng.standard_normal(A.noc)
         ref = oracle.crs_mv(A, x)
-        for opt in ({"csb_layout": 1}, {"csb_layout": 2}, {"csb_layout": 2, "csb_groups": 8}) \
+        for opt in ({"csb_layout": 1}, {"csb_layout": 2}, {"csb_layout": 2, "csb_groups": 8}, {"csb_layout": 3}) \
                 if cls is pw.CSBHip else ({},):
             m = _load(cls, A, **opt)
             y = np.full(A.nor, np.nan)
@@ -262,7 +265,8 @@ def test_power_iteration_state(cls, it):
     xr, yr = oracle.crs_power_iteration(A, x0, it)
     for opt in (({"csb_groups": 1}, {"csb_groups": 8, "block_bits": 9}, {"csb_groups": 16, "block_bits": 8},
                  {"csb_unroll": 4, "csb_groups": 1}, {"csb_unroll": 2, "csb_groups": 8},
-                 {"csb_layout": 1, "csb_groups": 1}, {"csb_layout": 1, "csb_groups": 8, "block_bits": 9})
+                 {"csb_layout": 1, "csb_groups": 1}, {"csb_layout": 1, "csb_groups": 8, "block_bits": 9},
+                 {"csb_layout": 3, "csb_groups": 1}, {"csb_layout": 3, "csb_groups": 8, "block_bits": 9})
                 if cls is pw.CSBHip else ({}, {"long_chunk": 256})[:1]):
         m = _load(cls, A, **opt)
         x, y = x0.copy(), np.zeros(A.nor)
@@ -289,7 +293,7 @@ def test_csb_device_build_narrow_values(values, tol):
     for A in cases:
         x = pw.gen_uniform(A.noc, 4)
         ref = oracle.crs_mv(A, x)
-        for opt in ({}, {"csb_groups": 8}):
+        for opt in ({}, {"csb_groups": 8}, {"csb_layout": 3}):
             m = _load(pw.CSBHip, A, values=values, **opt)
             y = np.zeros(A.nor)
             m.mv(x, y)
@@ -309,3 +313,28 @@ def test_empty_triplets(cls):
     m.mv(np.ones(50), y)
     assert np.all(y == 0)
     assert m.stats()["nnz"] == 0
+
+
+def test_csb_wide_streams_past_2_24_columns():
+    """With more than 2^24 columns the packed (row in slab, column) index would
+    leave slabs of under 256 rows, so the automatic choice builds wide slab
+    streams on the device (not the host-built slab pieces): stored layout
+    bit-exact, y within 1e-13, and the packed layout on the same matrix."""
+    rng = np.random.default_rng(11)
+    nor, noc = 3000, 1 << 25
+    lens = rng.integers(0, 40, nor)
+    lens[7] = 3000
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(noc, int(k), replace=False)) for k in lens]).astype(np.int32)
+    A = oracle.CRSMatrix(nor, noc, rs, ci, rng.standard_normal(int(rs[-1])))
+    x = rng.standard_normal(noc)
+    ref = oracle.crs_mv(A, x)
+    rb, rind, rdata, rblk, _ = oracle.csb_build(_triplets_of(A), 0)
+    for opt, layout in (({}, 3), ({"csb_groups": 8}, 3), ({"csb_layout": 2}, 2)):
+        m = _load(pw.CSBHip, A, **opt)
+        assert m.info()["layout"] == layout, opt
+        y = np.zeros(nor)
+        m.mv(x, y)
+        assert _relerr(y, ref) <= 1e-13, opt
+        gb, gind, gdata, gblk = m.csb_layout()
+        assert gb == rb and np.array_equal(gind, rind) and np.array_equal(gblk, rblk) and np.array_equal(gdata, rdata)

@@ -86,7 +86,8 @@ struct CsbDev {
     int32_t col_bits = 0;
     int32_t unroll = 2;          // stream kernel groups of 4 per thread per step
     int32_t nt = 0;              // stream kernel: non-temporal entry loads / partial stores
-    DevBuf soff;
+    int32_t wide = 0;            // wide streams: ind = column, srows = row in slab (u16)
+    DevBuf soff, srows;
     void clear() {
         bits = 0;
         H = V = 0;
@@ -98,6 +99,8 @@ struct CsbDev {
         work.release();
         ypart.release();
         soff.release();
+        srows.release();
+        wide = 0;
         stream = 0;
         col_bits = 0;
         unroll = 2;
@@ -198,7 +201,7 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // CSB slab-stream layout built on the device from the device CRS arrays
 // (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
 int csb_key_bits(int64_t cells, int bits);
-int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S);
+int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide);
 // BlockCOH layout built on the device from the device CRS arrays
 // (coh_build.hip); h->coh.bands holds the band partition (start, rows, nnz,
 // beta_bits, H, V, off_nz) on entry and the per-band sizes and offsets on

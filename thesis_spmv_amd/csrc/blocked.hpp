@@ -48,6 +48,10 @@ struct CsbArgs {
     int32_t col_bits = 0;
     const int32_t* soff = nullptr;
     int32_t unroll = 2;  // groups of 4 entries per thread per step: 2 or 4
+    // Wide slab streams (wide = 1): ind holds the column itself and srows the
+    // row in the slab (u16), for column indices too wide to pack.
+    int32_t wide = 0;
+    const uint16_t* srows = nullptr;
 };
 
 // One BlockCOH block (offsets are relative to its band's arrays).

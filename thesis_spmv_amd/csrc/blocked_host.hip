@@ -145,12 +145,14 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
             }
     }
 
-    // Slab-stream layout: slabs of 2^rb rows, rb <= the LDS rows, beta and
-    // 32 - (bits of a column index), so that (row in slab, column) packs into
-    // 32 bits.  Used unless the caller asks for pieces, the slabs would be
-    // shorter than 256 rows (automatic choice only), or the device build's
-    // sort key (slab, block column, Z-Morton code) would need more than 64
-    // bits.
+    // Slab-stream layout: slabs of 2^rb rows, rb <= the LDS rows and beta.
+    // Packed streams store (row in slab) << cb | column in 32 bits, so rb <=
+    // 32 - cb (cb = bits of a column index); when that leaves slabs of fewer
+    // than 256 rows (more than 2^24 columns) the automatic choice takes wide
+    // streams, which store the column and the row in slab (u16) apart.  The
+    // host-built slab pieces are used when the caller asks for them or the
+    // device build's sort key (slab, block column, Z-Morton code) would need
+    // more than 64 bits.
     const int lds_bits = blocked::lds_rows(h->vtype) == 8192 ? 13 : 12;
     int cb = 0;
     while (((int64_t)1 << cb) < noc) ++cb;
@@ -158,12 +160,15 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     // but at least 256 rows
     int fill_bits = 8;
     while (fill_bits < lds_bits && (nor >> (fill_bits + 1)) >= 4096) ++fill_bits;
-    const int rb = std::min(std::min(std::min(lds_bits, fill_bits), bits), 32 - cb);
-    const bool use_stream = h->csb_layout == 2 || (h->csb_layout == 0 && rb >= std::min(8, bits));
+    const int rb_fit = std::min(std::min(lds_bits, fill_bits), bits);
+    const int rb_packed = std::min(rb_fit, 32 - cb);
+    const bool wide = h->csb_layout == 3 || (h->csb_layout == 0 && rb_packed < std::min(8, bits));
+    const bool use_stream = h->csb_layout != 1;
+    const int rb = wide ? rb_fit : rb_packed;
     const int64_t S_stream = beta >> rb;
     if (use_stream && csb_key_bits(V * S_stream * H, bits) <= 64) {
         // the slab streams, sorted on the device from the device CRS arrays
-        rc = csb_stream_device(h, rb, cb, S_stream);
+        rc = csb_stream_device(h, rb, cb, S_stream, wide);
         if (rc != PWM_OK) return rc;
     } else {
         // slab pieces over the stored order, built on the host
@@ -673,6 +678,8 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.soff = h->csb.soff.as<int32_t>();
     a.unroll = h->csb.unroll;
     a.nt = h->csb.nt;
+    a.wide = h->csb.wide;
+    a.srows = h->csb.srows.as<uint16_t>();
     hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream, part);
     if (e == hipSuccess && pw && part != 1)
         e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);
@@ -715,10 +722,12 @@ static int csb_stream_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t
     const uint32_t cmask = (uint32_t)(((uint64_t)1 << cb) - 1);
     std::vector<int32_t> off(V * S * H + 1);
     std::vector<uint32_t> sind(std::max<int64_t>(nnz, 1));
+    std::vector<uint16_t> srow(c.wide ? std::max<int64_t>(nnz, 1) : 0);
     std::vector<double> sdat(std::max<int64_t>(nnz, 1));
     hipError_t e = hipMemcpy(blk_ptr, c.blk_ptr.p, (H * V + 1) * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(off.data(), c.soff.p, off.size() * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && nnz) e = hipMemcpy(sind.data(), c.ind.p, nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nnz && c.wide) e = hipMemcpy(srow.data(), c.srows.p, nnz * 2, hipMemcpyDeviceToHost);
     if (e == hipSuccess && nnz) {
         DevBuf tmp;
         e = tmp.alloc(nnz * 8);
@@ -740,8 +749,8 @@ static int csb_stream_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t
                     const int64_t cell = (br * S + sl) * H + bc;
                     for (int64_t p = off[cell]; p < off[cell + 1]; ++p) {
                         const uint32_t w = sind[p];
-                        const uint32_t lrow = (uint32_t)(sl << rb) + (cb < 32 ? (w >> cb) : 0u);
-                        const uint32_t col = (uint32_t)((w & cmask) - bc * beta);
+                        const uint32_t lrow = (uint32_t)(sl << rb) + (c.wide ? srow[p] : (cb < 32 ? (w >> cb) : 0u));
+                        const uint32_t col = (uint32_t)((c.wide ? w : (w & cmask)) - bc * beta);
                         tmp.push_back({zkey(lrow, col), (lrow << 16) | col, sdat[p]});
                     }
                 }

@@ -62,9 +62,11 @@ __global__ void k_csb_keys(CsbKeyArgs a, const int32_t* __restrict__ rs, const i
     }
 }
 
+// srow != NULL: wide streams (sind = column, srow = row in slab)
 template <typename E>
 __global__ void k_csb_gather(CsbKeyArgs a, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ pos,
-                             const E* __restrict__ val, uint32_t* __restrict__ sind, E* __restrict__ sdat) {
+                             const E* __restrict__ val, uint32_t* __restrict__ sind, uint16_t* __restrict__ srow,
+                             E* __restrict__ sdat) {
     const uint64_t zmask = (a.bits == 0) ? 0 : ((((uint64_t)1 << (2 * a.bits)) - 1));
     const uint32_t smask = (uint32_t)(((uint64_t)1 << a.rb) - 1);
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.nnz; p += (int64_t)gridDim.x * blockDim.x) {
@@ -73,7 +75,12 @@ __global__ void k_csb_gather(CsbKeyArgs a, const uint64_t* __restrict__ keys, co
         const uint64_t bc = (key >> (2 * a.bits)) % (uint64_t)a.H;
         const uint32_t lrow = compact16(zk >> 1), lcol = compact16(zk);
         const uint32_t col = (uint32_t)(bc << a.bits) + lcol;
-        sind[p] = ((lrow & smask) << a.cb) | col;
+        if (srow) {
+            sind[p] = col;
+            srow[p] = (uint16_t)(lrow & smask);
+        } else {
+            sind[p] = ((lrow & smask) << a.cb) | col;
+        }
         sdat[p] = val[pos[p]];
     }
 }
@@ -111,7 +118,7 @@ int csb_key_bits(int64_t cells, int bits) {
     return cb + 2 * bits;
 }
 
-int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S) {
+int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide) {
     CsbDev& c = h->csb;
     const int bits = c.bits;
     const int64_t H = c.H, V = c.V, nnz = h->nnz, nor = h->nor;
@@ -120,11 +127,13 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S) {
     const size_t vs = value_size(h->vtype);
     hipError_t e;
     if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess ||
-        (e = c.soff.alloc((cells + 1) * 4)) != hipSuccess)
+        (e = c.soff.alloc((cells + 1) * 4)) != hipSuccess || (wide && (e = c.srows.alloc(zpad * 2)) != hipSuccess))
         return hip_fail(h, e, "CSB stream allocation");
     if ((e = hipMemsetAsync(c.ind.p, 0, zpad * 4, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess)
+        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess ||
+        (wide && (e = hipMemsetAsync(c.srows.p, 0, zpad * 2, h->stream)) != hipSuccess))
         return hip_fail(h, e, "CSB stream allocation");
+    if (!wide) c.srows.release();
     pwmk::CsbKeyArgs a{nor, nnz, S, H, bits, rb, cb};
     std::vector<int32_t> soff(cells + 1, 0);
     if (nnz > 0) {
@@ -148,15 +157,15 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S) {
         const dim3 g(pwmk::grid_of(nnz));
         if (vs == 8)
             hipLaunchKernelGGL(pwmk::k_csb_gather<uint64_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
                                c.data.as<uint64_t>());
         else if (vs == 4)
             hipLaunchKernelGGL(pwmk::k_csb_gather<uint32_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
                                c.data.as<uint32_t>());
         else
             hipLaunchKernelGGL(pwmk::k_csb_gather<uint16_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
                                c.data.as<uint16_t>());
         hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
                            keys_out.as<uint64_t>(), nnz, cells, bits, c.soff.as<int32_t>());
@@ -182,6 +191,7 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S) {
         (e = hipMemcpy(c.blk_ptr.p, blk_ptr.data(), blk_ptr.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(h, e, "CSB build");
     c.stream = 1;
+    c.wide = wide ? 1 : 0;
     c.col_bits = cb;
     c.slab_bits = rb;
     c.slabs = (int32_t)S;

@@ -267,8 +267,9 @@ __global__ __launch_bounds__(kBS) void k_csb(CsbArgs a, const typename Arith<VT>
 constexpr int kSBS = 256;
 
 // kSU: groups of 4 entries per thread per step (2 or 4); NT: non-temporal
-// entry loads and partial-slab stores
-template <typename VT, bool POWER, int kSU, bool NT>
+// entry loads and partial-slab stores; WIDE: ind holds the column and srows
+// the row in the slab.
+template <typename VT, bool POWER, int kSU, bool NT, bool WIDE>
 __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                                      typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
     using X = typename Arith<VT>::X;
@@ -304,6 +305,7 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
     const int32_t q0 = e0 >> 2, q1 = (int32_t)(((int64_t)e1 + 3) >> 2);
     for (int32_t qb = q0; qb < q1; qb += kSBS * kSU) {
         pwmk::vi4 w[kSU];
+        pwmk::vu2 r[kSU];
         pwmk::Quad<VT> v[kSU];
 #pragma unroll
         for (int u = 0; u < kSU; ++u) {
@@ -311,9 +313,11 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
             const int64_t qc = q < q1 ? q : q1 - 1;
             if constexpr (NT) {
                 w[u] = __builtin_nontemporal_load(pwmk::G((const pwmk::vi4*)(ind + 4 * qc)));
+                if constexpr (WIDE) r[u] = __builtin_nontemporal_load(pwmk::G((const pwmk::vu2*)(a.srows + 4 * qc)));
                 v[u].load_nt(val + 4 * qc);
             } else {
                 w[u] = *pwmk::G((const pwmk::vi4*)(ind + 4 * qc));
+                if constexpr (WIDE) r[u] = *pwmk::G((const pwmk::vu2*)(a.srows + 4 * qc));
                 v[u].load(val + 4 * qc);
             }
         }
@@ -321,14 +325,18 @@ __global__ __launch_bounds__(kSBS) void k_csb_stream(CsbArgs a, const typename A
 #pragma unroll
         for (int u = 0; u < kSU; ++u)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xv[u][j] = x[(uint32_t)w[u][j] & cmask];
+            for (int j = 0; j < 4; ++j) xv[u][j] = x[WIDE ? (uint32_t)w[u][j] : ((uint32_t)w[u][j] & cmask)];
 #pragma unroll
         for (int u = 0; u < kSU; ++u)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int64_t k = 4 * (int64_t)(qb + tid + u * kSBS) + j;
-                if (k >= e0 && k < e1)
-                    atomicAdd(&ylds[(uint32_t)w[u][j] >> cb], (Acc)v[u].get(j) * (Acc)xv[u][j]);
+                uint32_t row;
+                if constexpr (WIDE)
+                    row = ((j < 2 ? r[u].x : r[u].y) >> (16 * (j & 1))) & 0xffffu;
+                else
+                    row = (uint32_t)w[u][j] >> cb;
+                if (k >= e0 && k < e1) atomicAdd(&ylds[row], (Acc)v[u].get(j) * (Acc)xv[u][j]);
             }
     }
     __syncthreads();
@@ -637,17 +645,18 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
         const size_t lds = (size_t)a.slab_rows * sizeof(Acc);
         const dim3 grid((unsigned)a.n_tasks);
         // kernel instance for (power step, unroll, non-temporal)
-        auto pick = [&](auto power) {
-            constexpr bool P = decltype(power)::value;
-            if (a.unroll == 4) return a.nt ? k_csb_stream<VT, P, 4, true> : k_csb_stream<VT, P, 4, false>;
-            return a.nt ? k_csb_stream<VT, P, 2, true> : k_csb_stream<VT, P, 2, false>;
+        auto pick = [&](auto power, auto wide) {
+            constexpr bool P = decltype(power)::value, W = decltype(wide)::value;
+            if (a.unroll == 4) return a.nt ? k_csb_stream<VT, P, 4, true, W> : k_csb_stream<VT, P, 4, false, W>;
+            return a.nt ? k_csb_stream<VT, P, 2, true, W> : k_csb_stream<VT, P, 2, false, W>;
         };
         if (pw)
-            hipLaunchKernelGGL(pick(std::true_type{}), grid, dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr,
-                               *pw);
+            hipLaunchKernelGGL(a.wide ? pick(std::true_type{}, std::true_type{}) : pick(std::true_type{}, std::false_type{}),
+                               grid, dim3(kSBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
         else
-            hipLaunchKernelGGL(pick(std::false_type{}), grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y,
-                               PowerArgs{});
+            hipLaunchKernelGGL(a.wide ? pick(std::false_type{}, std::true_type{})
+                                      : pick(std::false_type{}, std::false_type{}),
+                               grid, dim3(kSBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
     } else if (pw) {
         hipLaunchKernelGGL((k_csb<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
                            (X*)nullptr, *pw);

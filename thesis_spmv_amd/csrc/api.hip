@@ -44,6 +44,58 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+// Pinned staging for host -> device uploads: two process-wide buffers of
+// kStageBytes, held until the process ends; one upload at a time uses them.
+namespace {
+constexpr size_t kStageBytes = (size_t)64 << 20;
+std::mutex g_stage_mu;
+void* g_stage[2] = {nullptr, nullptr};
+hipEvent_t g_stage_ev[2] = {nullptr, nullptr};
+bool g_stage_failed = false;
+}  // namespace
+
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!g_stage[0] && !g_stage_failed) {
+        for (int i = 0; i < 2 && !g_stage_failed; ++i)
+            if (hipHostMalloc(&g_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&g_stage_ev[i], hipEventDisableTiming) != hipSuccess)
+                g_stage_failed = true;
+        if (g_stage_failed) {
+            for (int i = 0; i < 2; ++i) {
+                if (g_stage[i]) (void)hipHostFree(g_stage[i]);
+                if (g_stage_ev[i]) (void)hipEventDestroy(g_stage_ev[i]);
+                g_stage[i] = nullptr;
+                g_stage_ev[i] = nullptr;
+            }
+            (void)hipGetLastError();
+        }
+    }
+    if (g_stage_failed) {  // no pinned memory: a plain copy from pageable memory
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    // host threads fill one buffer while the other is in flight
+    hipError_t e = hipSuccess;
+    bool busy[2] = {false, false};
+    int b = 0;
+    for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageBytes, b ^= 1) {
+        const size_t n = std::min(kStageBytes, bytes - off);
+        if (busy[b] && (e = hipEventSynchronize(g_stage_ev[b])) != hipSuccess) break;
+        char* stage = (char*)g_stage[b];
+        const char* from = (const char*)src + off;
+        parallel_for((int64_t)n, (int64_t)4 << 20,
+                     [&](int64_t a0, int64_t a1, int) { memcpy(stage + a0, from + a0, (size_t)(a1 - a0)); });
+        e = hipMemcpyAsync((char*)dst + off, stage, n, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(g_stage_ev[b], s);
+        busy[b] = true;
+    }
+    // the buffers are the next upload's: wait until the copies have read them
+    const hipError_t e2 = hipStreamSynchronize(s);
+    return e != hipSuccess ? e : e2;
+}
+
 int fail(pwm_matrix* h, int code, const std::string& msg) {
     if (h)
         h->err = msg;
@@ -443,7 +495,7 @@ static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
     if (e != hipSuccess) return hip_fail(h, e, "value allocation");
     if (nnz == 0) return PWM_OK;
     if (h->vtype == PWM_VALUES_F64) {
-        e = hipMemcpy(h->val.p, vals, nnz * 8, hipMemcpyHostToDevice);
+        e = upload_staged(h->val.p, vals, nnz * 8, h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "value upload");
         return PWM_OK;
     }
@@ -459,7 +511,7 @@ static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
     }
     DevBuf tmp;
     if ((e = tmp.alloc(nnz * 8)) != hipSuccess) return hip_fail(h, e, "staging allocation");
-    if ((e = hipMemcpy(tmp.p, vals, nnz * 8, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = upload_staged(tmp.p, vals, nnz * 8, h->stream)) != hipSuccess)
         return hip_fail(h, e, "value upload");
     e = pwmk::launch_convert_values(tmp.as<double>(), h->val.p, nnz, h->vtype, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -504,8 +556,8 @@ static int load_crs_host(pwm_matrix* h, int64_t nor, int64_t noc, const int32_t*
     hipError_t e;
     if ((e = h->rs.alloc((nor + 1) * 4)) != hipSuccess || (e = alloc_padded(h->ci, nnz, 4)) != hipSuccess)
         return hip_fail(h, e, "CRS allocation");
-    if ((e = hipMemcpy(h->rs.p, rs, (nor + 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (nnz && (e = hipMemcpy(h->ci.p, ci, nnz * 4, hipMemcpyHostToDevice)) != hipSuccess))
+    if ((e = upload_staged(h->rs.p, rs, (nor + 1) * 4, h->stream)) != hipSuccess ||
+        (e = upload_staged(h->ci.p, ci, nnz * 4, h->stream)) != hipSuccess)
         return hip_fail(h, e, "CRS upload");
     int rc = upload_values(h, vals, nnz);
     if (rc != PWM_OK) return rc;

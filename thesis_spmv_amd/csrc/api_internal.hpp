@@ -198,6 +198,10 @@ struct pwm_matrix {
 namespace pwm_impl {
 
 int fail(pwm_matrix* h, int code, const std::string& msg);
+// Host -> device copy through pinned staging buffers (double-buffered,
+// threads fill one while the other is in flight); returns once the copy has
+// completed.  A plain copy when pinned memory cannot be allocated.
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
 // CSB slab-stream layout built on the device from the device CRS arrays
 // (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
 int csb_key_bits(int64_t cells, int bits);

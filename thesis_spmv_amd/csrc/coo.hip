@@ -80,9 +80,9 @@ int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, cons
         (e = keys_out.alloc(nnz * 8)) != hipSuccess || (e = pos.alloc(nnz * 4)) != hipSuccess ||
         (e = pos_out.alloc(nnz * 4)) != hipSuccess)
         return hip_fail(h, e, "COO conversion allocation");
-    if ((e = hipMemcpyAsync(d_rows.p, rows, nnz * 4, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_cols.p, cols, nnz * 4, hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(d_vals.p, vals, nnz * 8, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+    if ((e = upload_staged(d_rows.p, rows, nnz * 4, h->stream)) != hipSuccess ||
+        (e = upload_staged(d_cols.p, cols, nnz * 4, h->stream)) != hipSuccess ||
+        (e = upload_staged(d_vals.p, vals, nnz * 8, h->stream)) != hipSuccess)
         return hip_fail(h, e, "COO upload");
     hipLaunchKernelGGL(pwmk::k_coo_keys, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream, d_rows.as<int32_t>(),
                        d_cols.as<int32_t>(), nnz, cb, keys.as<uint64_t>(), pos.as<uint32_t>());

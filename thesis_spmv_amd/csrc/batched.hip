@@ -235,9 +235,9 @@ static int batch_upload(pwm_batch* b) {
             e = hipMemsetAsync((char*)A.at(I.o_val) + I.nnz * vs, 0, pwm_impl::kPadEntries * vs, b->stream);
         if (e != hipSuccess) return bhip(b, e, "batch padding");
         if (I.kind == 0) {
-            e = hipMemcpy(rs, I.rs.data(), (I.n + 1) * 4, hipMemcpyHostToDevice);
-            if (e == hipSuccess && I.nnz) e = hipMemcpy(A.at(I.o_ci), I.ci.data(), I.nnz * 4, hipMemcpyHostToDevice);
-            if (e == hipSuccess && I.nnz) e = hipMemcpy(tmp.p, I.val.data(), I.nnz * 8, hipMemcpyHostToDevice);
+            e = pwm_impl::upload_staged(rs, I.rs.data(), (I.n + 1) * 4, b->stream);
+            if (e == hipSuccess) e = pwm_impl::upload_staged(A.at(I.o_ci), I.ci.data(), I.nnz * 4, b->stream);
+            if (e == hipSuccess) e = pwm_impl::upload_staged(tmp.p, I.val.data(), I.nnz * 8, b->stream);
             if (e == hipSuccess && I.nnz)
                 e = pwmk::launch_convert_values(tmp.as<double>(), A.at(I.o_val), I.nnz, vt, b->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(b->stream);

@@ -299,12 +299,11 @@ static int make_resident(pwm_server* s, int mi) {
     if (e == hipSuccess && M.kind == 1) {
         e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, s->vtype, s->stream);
     } else if (e == hipSuccess) {
-        e = hipMemcpyAsync(rs, M.rs_p, (M.n + 1) * 4, hipMemcpyHostToDevice, s->stream);
-        if (e == hipSuccess && M.nnz) e = hipMemcpyAsync(base + M.o_ci, M.ci_p, M.nnz * 4, hipMemcpyHostToDevice, s->stream);
+        e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->stream);
+        if (e == hipSuccess) e = pwm_impl::upload_staged(base + M.o_ci, M.ci_p, M.nnz * 4, s->stream);
         if (e == hipSuccess && M.nnz) {
             if (s->vtype == PWM_VALUES_F64) {
-                e = hipMemcpyAsync(base + M.o_val, M.val_p, M.nnz * 8, hipMemcpyHostToDevice, s->stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+                e = pwm_impl::upload_staged(base + M.o_val, M.val_p, M.nnz * 8, s->stream);
             } else {
                 // narrow on the host (FP32, or BF16 = the upper half of the
                 // exact FP32 value), then one copy
@@ -335,8 +334,7 @@ static int make_resident(pwm_server* s, int mi) {
                     src = b16.data();
                     nb = M.nnz * 2;
                 }
-                e = hipMemcpyAsync(base + M.o_val, src, nb, hipMemcpyHostToDevice, s->stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(s->stream);  // the staging dies here
+                e = pwm_impl::upload_staged(base + M.o_val, src, nb, s->stream);  // returns once copied
             }
         }
     }

@@ -526,6 +526,14 @@ def main():
         pw.synchronize()
         wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
         ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
+        # the drop-in host API: host x in, host y out (PCIe included; never `value`)
+        yh = np.empty(n)
+        host_ms = []
+        for _ in range(6):
+            th = time.perf_counter()
+            m.mv(x_host, yh)
+            host_ms.append((time.perf_counter() - th) * 1e3)
+        host_ms = float(np.median(host_ms[1:]))
         inf = m.info()
         mstats = m.stats()
         # parity spot check of this run's result against the first format's
@@ -533,9 +541,10 @@ def main():
         results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
                          "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush,
                          "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf, "load_s": t_load,
-                         "y_checksum": float(np.sum(np.abs(yv))), "y": yv}
+                         "y_checksum": float(np.sum(np.abs(yv))), "y": yv, "host_ms": host_ms}
         print(f"# {name}: wall {wall_ms * 1e3:.1f} us/step, events {ms_ev * 1e3:.1f} us, dominant kernel "
-              f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us",
+              f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us, "
+              f"host-array mv {host_ms * 1e3:.0f} us",
               file=sys.stderr, flush=True)
         del m
 
@@ -647,7 +656,8 @@ def main():
                         **({"crs_kernel": "tiles" if v["info"]["group_size"] == 0 else "vector"} if k == "crs" else {}),
                         "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
                         "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"],
-                        "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"]}
+                        "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"],
+                        "host_array_mv_us": round(v["host_ms"] * 1e3, 1)}
                     for k, v in results.items()},
         "power_method_us_per_iteration": round(power_ms * 1e3, 2),
     }

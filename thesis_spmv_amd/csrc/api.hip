@@ -54,9 +54,9 @@ hipEvent_t g_stage_ev[2] = {nullptr, nullptr};
 bool g_stage_failed = false;
 }  // namespace
 
-hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (bytes == 0) return hipSuccess;
-    std::lock_guard<std::mutex> lk(g_stage_mu);
+// Allocates the staging pair once (the caller holds g_stage_mu); false when
+// pinned memory cannot be had.
+static bool stage_ready() {
     if (!g_stage[0] && !g_stage_failed) {
         for (int i = 0; i < 2 && !g_stage_failed; ++i)
             if (hipHostMalloc(&g_stage[i], kStageBytes, hipHostMallocDefault) != hipSuccess ||
@@ -72,7 +72,13 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
             (void)hipGetLastError();
         }
     }
-    if (g_stage_failed) {  // no pinned memory: a plain copy from pageable memory
+    return !g_stage_failed;
+}
+
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!stage_ready()) {  // no pinned memory: a plain copy from pageable memory
         hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
         return e == hipSuccess ? hipStreamSynchronize(s) : e;
     }
@@ -92,6 +98,36 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
         busy[b] = true;
     }
     // the buffers are the next upload's: wait until the copies have read them
+    const hipError_t e2 = hipStreamSynchronize(s);
+    return e != hipSuccess ? e : e2;
+}
+
+hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!stage_ready()) {
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    // the copy of chunk k+1 into one buffer runs while host threads empty
+    // chunk k from the other
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t k) {
+        const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+        hipError_t e = hipMemcpyAsync(g_stage[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(g_stage_ev[k & 1], s);
+        return e;
+    };
+    hipError_t e = issue(0);
+    for (size_t k = 0; k < nch && e == hipSuccess; ++k) {
+        if (k + 1 < nch && (e = issue(k + 1)) != hipSuccess) break;
+        if ((e = hipEventSynchronize(g_stage_ev[k & 1])) != hipSuccess) break;
+        const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+        const char* stage = (const char*)g_stage[k & 1];
+        char* to = (char*)dst + off;
+        parallel_for((int64_t)n, (int64_t)4 << 20,
+                     [&](int64_t a0, int64_t a1, int) { memcpy(to + a0, stage + a0, (size_t)(a1 - a0)); });
+    }
     const hipError_t e2 = hipStreamSynchronize(s);
     return e != hipSuccess ? e : e2;
 }
@@ -776,11 +812,10 @@ static int stage_vectors(pwm_matrix* h) {
 static int upload_vec(pwm_matrix* h, const double* src, void* dst, int64_t n) {
     hipError_t e;
     if (h->vtype == PWM_VALUES_F64) {
-        e = hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, h->stream);
+        e = upload_staged(dst, src, n * 8, h->stream);
     } else {
         std::vector<float> f(src, src + n);
-        e = hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        e = upload_staged(dst, f.data(), n * 4, h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "vector upload");
     return PWM_OK;
@@ -788,12 +823,10 @@ static int upload_vec(pwm_matrix* h, const double* src, void* dst, int64_t n) {
 static int download_vec(pwm_matrix* h, const void* src, double* dst, int64_t n) {
     hipError_t e;
     if (h->vtype == PWM_VALUES_F64) {
-        e = hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        e = download_staged(dst, src, n * 8, h->stream);
     } else {
         std::vector<float> f(n);
-        e = hipMemcpyAsync(f.data(), src, n * 4, hipMemcpyDeviceToHost, h->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        e = download_staged(f.data(), src, n * 4, h->stream);
         for (int64_t i = 0; i < n; ++i) dst[i] = f[i];
     }
     if (e != hipSuccess) return hip_fail(h, e, "vector download");

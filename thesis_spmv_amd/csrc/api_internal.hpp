@@ -202,6 +202,8 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // threads fill one while the other is in flight); returns once the copy has
 // completed.  A plain copy when pinned memory cannot be allocated.
 hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Device -> host the same way (after the work queued on s).
+hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
 // CSB slab-stream layout built on the device from the device CRS arrays
 // (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
 int csb_key_bits(int64_t cells, int bits);

@@ -414,6 +414,10 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
     constexpr int LR = LdsRows<Acc>::v;
     __shared__ Acc ylds[LR];
     __shared__ int32_t bstart[kCohChunk + 2];  // up to kCohChunk+1 blocks and a sentinel
+    // the records of the chunk's blocks when there are at most kCohStage of
+    // them (the usual case), so the decode reads them from LDS
+    constexpr int kCohStage = 256;
+    __shared__ CohBlock bstage[kCohStage];
     __shared__ int tv[kWaves], tf[kWaves];
     __shared__ int carry_s, carry_r;
     __shared__ double red[16];
@@ -452,7 +456,12 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
         const int b_lo = ctab[c], b_hi = ctab[c + 1];  // blocks touching this chunk
         const int nbl = b_hi - b_lo + 1;
         __syncthreads();
-        for (int i = tid; i < nbl; i += kBS) bstart[i] = blocks[b_lo + i].entry;
+        const bool staged = nbl <= kCohStage;
+        for (int i = tid; i < nbl; i += kBS) {
+            const CohBlock cb = blocks[b_lo + i];
+            bstart[i] = cb.entry;
+            if (staged) bstage[i] = cb;
+        }
         if (tid == 0) bstart[nbl] = 0x7fffffff;
         __syncthreads();
         const int e0 = base + tid * kCohIPT;
@@ -505,7 +514,7 @@ __global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>
             const int e = e0 + j;
             dl[j] = 0;
             if (e < nnz) {
-                blk[j] = blocks[b_lo + bj[j]];
+                blk[j] = staged ? bstage[bj[j]] : blocks[b_lo + bj[j]];
                 const int kk = S[j] >> bits;
                 const bool newrow = rst[j] || (kk != ((S[j] - cj[j]) >> bits));
                 // a band holds at most nnz row segments: the clamp keeps a

@@ -81,12 +81,14 @@ def make_workload(name):
     return n, rs, ci, dv, desc, time.time() - t0
 
 
-def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0):
+def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0, y_gpu=None):
     """The oracle's restatements of CRSOMP::mv (Env_Implementations/CRSOMP.hpp:110-123,
     OpenMP dynamic,512) and CRS_Merge::mv (SOA_Implementations/CRS_Merge.hpp:95-141,
     P = threads) on `threads` host cores, each bounded to about `budget_s`
     seconds of full SpMVs of the same matrix, plus the power iteration
-    (CRSOMP.hpp:135-157) per iteration.  `value` is the faster SpMV."""
+    (CRSOMP.hpp:135-157) per iteration.  `value` is the faster SpMV.  As the
+    checker, the CRSOMP result (per row the sequential CRS::mv order) also
+    gives the relative max-norm error of the GPU's y (`y_gpu`)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     A = oracle.CRSMatrix(n, n, rs, ci, dv)
@@ -108,7 +110,12 @@ def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0):
     L = oracle.lib()
     t_pow, k_pow = timed(lambda: L.orc_crsomp_power_iteration(threads, n, A.row_start, A.col_ind, A.data, px, py, it))
     best = min(t_omp, t_mrg)
+    err = None
+    if y_gpu is not None:
+        y_ref = oracle.crsomp_mv(A, x, threads)
+        err = float(np.max(np.abs(y_gpu - y_ref)) / max(float(np.max(np.abs(y_ref))), 1e-300))
     return {"value": round(bytes_per_mv / best / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+            "gpu_y_rel_err_vs_oracle": err,
             "sample": f"median of {k_omp} CRSOMP::mv and {k_mrg} CRS_Merge::mv (P={threads}) full SpMVs of the same "
                       f"matrix: {t_omp * 1e3:.2f} ms and {t_mrg * 1e3:.2f} ms (oracle restatements, g++ -O2 -fopenmp); "
                       f"power iteration {t_pow / it * 1e3:.2f} ms/iteration ({k_pow} calls of {it})",
@@ -458,7 +465,7 @@ def main():
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-baseline", action="store_true")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 1000.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 1000.0, 2000.0])
     ap.add_argument("--serve-requests", type=int, default=120, help="requests per arrival rate")
     ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
     ap.add_argument("--serve-max-n", type=float, default=1e7)
@@ -643,6 +650,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "format": best, "nnz": z, "rows": n, "index": "int32",
                    "algorithmic_bytes_per_spmv": bytes_mv, "pct_of_hbm_peak": round(100 * r["GBs"] / HBM_PEAK_GBS, 1)},
+        "gflops": round(2.0 * z / (r["ms_per_step"] * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "timed": (f"csb slab kernel + column-group reduction, {round(t_roof * 1e3, 2)} us" if two_kernels
@@ -671,7 +679,7 @@ def main():
             cores = len(os.sched_getaffinity(0))
         except AttributeError:
             cores = os.cpu_count() or 1
-        out["cpu_baseline"] = cpu_baseline(n, rs, ci, dv, x_host, bytes_mv, min(16, cores))
+        out["cpu_baseline"] = cpu_baseline(n, rs, ci, dv, x_host, bytes_mv, min(16, cores), y_gpu=r["y"])
     print(json.dumps(out), flush=True)
 
 

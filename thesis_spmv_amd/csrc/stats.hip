@@ -28,6 +28,32 @@ __global__ void k_block_hist(const int32_t* __restrict__ rs, const int32_t* __re
     }
 }
 
+// The same counts a workgroup of 256 consecutive rows at a time: with blocks
+// of at least 256 rows those rows share one block row, so the workgroup's
+// threads stride over the rows' (contiguous) entries and count into an LDS
+// histogram of that block row (nb bins); the workgroup then adds its
+// non-zero bins to the table once.
+constexpr int kHistRows = 256;
+__global__ __launch_bounds__(kHistRows) void k_block_hist_rows(const int32_t* __restrict__ rs,
+                                                               const int32_t* __restrict__ ci, int64_t nor, int bits,
+                                                               int64_t nb, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t lh[];
+    const int tid = threadIdx.x;
+    for (int64_t i = tid; i < nb; i += kHistRows) lh[i] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * kHistRows;
+    const int64_t r1 = r0 + kHistRows < nor ? r0 + kHistRows : nor;
+    for (int64_t e = rs[r0] + tid, e1 = rs[r1]; e < e1; e += kHistRows) {
+        const int64_t bc = (int64_t)ci[e] >> bits;
+        if (bc < nb) atomicAdd(&lh[bc], 1u);
+    }
+    __syncthreads();
+    const int64_t br = r0 >> bits;
+    if (br < nb)
+        for (int64_t i = tid; i < nb; i += kHistRows)
+            if (lh[i]) atomicAdd(&hist[br * nb + i], lh[i]);
+}
+
 }  // namespace pwmk
 
 namespace pwm_impl {
@@ -74,10 +100,16 @@ int compute_matrix_stats(pwm_matrix* h) {
         DevBuf d;
         if ((e = d.alloc(nb * nb * 4)) != hipSuccess) return hip_fail(h, e, "statistics table");
         e = hipMemsetAsync(d.p, 0, nb * nb * 4, h->stream);
-        const int64_t blocks = std::min<int64_t>((Z + 255) / 256, 8192);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(pwmk::k_block_hist, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->rs.as<int32_t>(),
-                               h->ci.as<int32_t>(), N, Z, bits, nb, d.as<uint32_t>());
+            if (bits >= 8) {  // 256-row workgroups stay inside one block row
+                const int64_t wgs = (N + pwmk::kHistRows - 1) / pwmk::kHistRows;
+                hipLaunchKernelGGL(pwmk::k_block_hist_rows, dim3((unsigned)wgs), dim3(pwmk::kHistRows), nb * 4,
+                                   h->stream, h->rs.as<int32_t>(), h->ci.as<int32_t>(), N, bits, nb, d.as<uint32_t>());
+            } else {
+                const int64_t blocks = std::min<int64_t>((Z + 255) / 256, 8192);
+                hipLaunchKernelGGL(pwmk::k_block_hist, dim3((unsigned)blocks), dim3(256), 0, h->stream,
+                                   h->rs.as<int32_t>(), h->ci.as<int32_t>(), N, Z, bits, nb, d.as<uint32_t>());
+            }
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipMemcpyAsync(hist.data(), d.p, nb * nb * 4, hipMemcpyDeviceToHost, h->stream);

@@ -259,6 +259,14 @@ def test_device_vectors_and_errors():
         _load(pw.CRSHip, oracle.CRSMatrix(2, 2, [0, 1, 2], [0, 2], [1.0, 1.0]))
     with pytest.raises(pw.PwmError):
         pw.CRSHip().loadCRS(2, 2, [0, 2, 1], [0, 1], [1.0, 1.0])
+    with pytest.raises(pw.PwmError, match="column index"):
+        _load(pw.CRSHip, oracle.CRSMatrix(2, 2, [0, 1, 2], [0, -1], [1.0, 1.0]))
+    # one bad index deep inside a large array (the check runs in parallel chunks)
+    n = 3_000_000
+    ci_big = (np.arange(n) % n).astype(np.int32)
+    ci_big[2_500_001] = n
+    with pytest.raises(pw.PwmError, match="column index"):
+        pw.CRSHip().loadCRS(n, n, np.arange(n + 1, dtype=np.int32), ci_big, np.ones(n))
     # calls before a load
     with pytest.raises(pw.PwmError) as e:
         pw.CRSHip().mv(np.ones(2), np.zeros(2))

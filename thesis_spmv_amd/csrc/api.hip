@@ -613,8 +613,13 @@ extern "C" int pwm_load_crs(pwm_matrix* h, int64_t nor, int64_t noc, const int32
     const int64_t nnz = rs[nor];
     if (nnz > PWM_MAX_NNZ) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: more than PWM_MAX_NNZ nonzeros");
     if (nnz > 0 && (!ci || !vals)) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: NULL col_ind/values");
-    for (int64_t k = 0; k < nnz; ++k)
-        if (ci[k] < 0 || ci[k] >= noc) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: column index out of range");
+    std::atomic<bool> bad{false};
+    parallel_for(nnz, 1 << 20, [&](int64_t k0, int64_t k1, int) {
+        bool b = false;
+        for (int64_t k = k0; k < k1; ++k) b |= (ci[k] < 0) | (ci[k] >= noc);
+        if (b) bad = true;
+    });
+    if (bad) return fail(h, PWM_ERR_INVALID, "pwm_load_crs: column index out of range");
     reset_matrix(h);
     return load_crs_host(h, nor, noc, rs, ci, vals);
 }

@@ -349,6 +349,30 @@ def test_mixed_precision_error_reported(cls, values):
     assert abs(lam - lam64) <= 1e-6 * lam64
 
 
+@pytest.mark.parametrize("values", ["f32", "bf16"])
+def test_mixed_precision_layout_variants(values):
+    """The narrow-storage SpMV and power method of the non-default layouts:
+    CSB slab pieces and wide slab streams, BlockCOH built on the host."""
+    A = oracle.poisson_crs(150, 120)
+    x0 = pw.gen_uniform(A.nor, 2)
+    ref = oracle.crs_mv(A, x0)
+    lam64 = None
+    for cls, opts in ((pw.CSBHip, {"csb_layout": 1}), (pw.CSBHip, {"csb_layout": 3}),
+                      (pw.CSBHip, {"csb_layout": 3, "csb_groups": 8}), (pw.BlockCOHHip, {"coh_build": 1})):
+        m = cls(-1, values, **opts)
+        m.generatePoissonMatrix(150, 120, 1)
+        y = np.zeros(A.nor)
+        m.mv(x0, y)
+        assert _relerr(y, ref) <= 1e-6, (cls.__name__, opts)
+        lam, _, _ = m.power_method_device(pw.DeviceVector.from_array(x0, np.float32),
+                                          pw.DeviceVector(A.nor, np.float32), 150, -1.0)
+        if lam64 is None:
+            m64 = pw.CRSHip()
+            m64.generatePoissonMatrix(150, 120, 1)
+            lam64, _, _ = m64.power_method_device(pw.DeviceVector.from_array(x0), pw.DeviceVector(A.nor), 150, -1.0)
+        assert abs(lam - lam64) <= 1e-6 * lam64, (cls.__name__, opts, lam, lam64)
+
+
 def test_bf16_rejects_inexact_values():
     A = oracle.CRSMatrix(2, 2, [0, 1, 2], [0, 1], [1.0, 0.1])
     with pytest.raises(pw.PwmError):

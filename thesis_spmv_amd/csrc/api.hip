@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <mutex>
+#include <thread>
 #include <numeric>
 #include <vector>
 
@@ -42,6 +44,105 @@ void DevBuf::release() {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+}
+
+// ---- host thread pool behind parallel_for
+namespace {
+constexpr int kMaxWorkers = 16;  // the GPU host grants 16 cores per GPU
+using LoopFn = std::function<void(int64_t, int64_t, int)>;
+
+void run_chunks(std::atomic<int64_t>& next, int64_t n, int64_t chunk, const LoopFn& fn, int worker) {
+    for (;;) {
+        const int64_t b = next.fetch_add(chunk);
+        if (b >= n) break;
+        fn(b, std::min(n, b + chunk), worker);
+    }
+}
+
+// one job at a time: workers 1..nworkers join the caller (worker 0)
+struct HostPool {
+    std::mutex run_mu;  // held by the caller whose job the pool runs
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    const LoopFn* fn = nullptr;
+    int64_t n = 0, chunk = 0;
+    std::atomic<int64_t> next{0};
+    int nworkers = 0, active = 0, pending = 0;
+    uint64_t epoch = 0;
+};
+HostPool* g_pool = nullptr;  // created once, never destroyed (its threads are detached)
+std::once_flag g_pool_once;
+
+void pool_worker(HostPool* p, int id) {
+    uint64_t seen = 0;
+    for (;;) {
+        const LoopFn* fn;
+        int64_t n, chunk;
+        {
+            std::unique_lock<std::mutex> lk(p->mu);
+            p->cv_work.wait(lk, [&] { return p->epoch != seen; });
+            seen = p->epoch;
+            if (id > p->active) continue;  // not needed for this job
+            fn = p->fn;
+            n = p->n;
+            chunk = p->chunk;
+        }
+        run_chunks(p->next, n, chunk, *fn, id);
+        std::lock_guard<std::mutex> lk(p->mu);
+        if (--p->pending == 0) p->cv_done.notify_all();
+    }
+}
+
+void spawn_run(int64_t n, int64_t chunk, const LoopFn& fn, int nt) {
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> ts;
+    for (int t = 1; t < nt; ++t) ts.emplace_back([&, t]() { run_chunks(next, n, chunk, fn, t); });
+    run_chunks(next, n, chunk, fn, 0);
+    for (auto& th : ts) th.join();
+}
+}  // namespace
+
+void parallel_run(int64_t n, int64_t chunk, const LoopFn& fn) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int64_t pieces = (n + chunk - 1) / chunk;
+    const int nt = (int)std::min<int64_t>(std::min<int64_t>(hw ? hw : 4, kMaxWorkers), pieces);
+    if (nt <= 1) {
+        fn(0, n, 0);
+        return;
+    }
+    std::call_once(g_pool_once, [&] {
+        HostPool* p = new HostPool();
+        const int want = (int)std::min<unsigned>(hw ? hw : 4, kMaxWorkers) - 1;
+        for (int i = 1; i <= want; ++i) {
+            try {
+                std::thread(pool_worker, p, i).detach();
+                p->nworkers = i;
+            } catch (...) {
+                break;
+            }
+        }
+        g_pool = p;
+    });
+    HostPool* p = g_pool;
+    std::unique_lock<std::mutex> run(p->run_mu, std::try_to_lock);
+    if (!run.owns_lock() || p->nworkers == 0) {
+        spawn_run(n, chunk, fn, nt);
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->fn = &fn;
+        p->n = n;
+        p->chunk = chunk;
+        p->next = 0;
+        p->active = std::min(p->nworkers, nt - 1);
+        p->pending = p->active;
+        ++p->epoch;
+    }
+    p->cv_work.notify_all();
+    run_chunks(p->next, n, chunk, fn, 0);
+    std::unique_lock<std::mutex> lk(p->mu);
+    p->cv_done.wait(lk, [&] { return p->pending == 0; });
 }
 
 // Pinned staging for host -> device uploads: two process-wide buffers of

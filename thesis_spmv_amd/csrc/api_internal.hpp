@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -19,29 +20,21 @@ struct CohTask;
 
 namespace pwm_impl {
 
-// Host-side parallel loop over [0, n) in chunks (std::thread; the library
-// does not depend on an OpenMP runtime).  fn(begin, end, worker).
+// Host-side parallel loop over [0, n) in chunks of `chunk`, on at most 16
+// threads (std::thread; the library does not depend on an OpenMP runtime).
+// fn(begin, end, worker) with worker < 16, each worker id used by one thread
+// at a time.  A persistent pool of 15 threads plus the caller runs it; a call
+// made while the pool is busy (another thread, or from inside fn) starts its
+// own threads instead.
+void parallel_run(int64_t n, int64_t chunk, const std::function<void(int64_t, int64_t, int)>& fn);
 template <typename F>
 void parallel_for(int64_t n, int64_t chunk, F fn) {
     if (n <= 0) return;
-    unsigned hw = std::thread::hardware_concurrency();
-    int64_t nt = std::min<int64_t>(hw ? hw : 4, 16);  // the GPU host grants 16 cores per GPU
-    nt = std::min<int64_t>(nt, (n + chunk - 1) / chunk);
-    if (nt <= 1) {
+    if ((n + chunk - 1) / chunk <= 1) {
         fn((int64_t)0, n, 0);
         return;
     }
-    std::atomic<int64_t> next(0);
-    std::vector<std::thread> pool;
-    for (int64_t t = 0; t < nt; ++t)
-        pool.emplace_back([&, t]() {
-            for (;;) {
-                const int64_t b = next.fetch_add(chunk);
-                if (b >= n) break;
-                fn(b, std::min(n, b + chunk), (int)t);
-            }
-        });
-    for (auto& th : pool) th.join();
+    parallel_run(n, chunk, std::function<void(int64_t, int64_t, int)>(fn));
 }
 
 // Thread-local message for calls without a handle.

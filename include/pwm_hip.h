@@ -152,8 +152,10 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
  * per thread per step, 2 or 4; 0 = time both with each group count),
  * "csb_nt" (slab-stream kernel: 1 = non-temporal entry loads and partial-slab
  * stores, 0 = plain, -1 = time both at load, the default),
- * "coh_build" (BlockCOH layout built on the device, 0, or on the host, 1;
- * both give the same arrays), "crs_kernel" (CRS: 0 = time the vector kernel at
+ * "csb_build" (CSB slab-piece layout built on the device, 0, or on the host,
+ * 1; both give the same arrays; the slab streams are always built on the
+ * device), "coh_build" (BlockCOH layout built on the device, 0, or on the
+ * host, 1; both give the same arrays), "crs_kernel" (CRS: 0 = time the vector kernel at
  * three lane counts and the row-aligned tiles kernel at load and keep the
  * fastest, 1 = vector, 2 = tiles), "crs_nt" (CRS tiles kernel: 1 =
  * non-temporal col_ind/value loads, 0 = plain, -1 = time both at load, the

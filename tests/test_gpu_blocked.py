@@ -18,7 +18,9 @@ FORMATS = [pw.CSBHip, pw.BlockCOHHip]
 # contiguous stream per slab (the automatic choice when its index packs),
 # 3 = the same streams with column and row in slab stored apart (the
 # automatic choice past 2^24 columns)
-CSB_LAYOUTS = [1, 2, 3]
+# layout options as stored: slab pieces built on the device and on the host,
+# packed and wide slab streams
+CSB_VARIANTS = [{"csb_layout": 1}, {"csb_layout": 1, "csb_build": 1}, {"csb_layout": 2}, {"csb_layout": 3}]
 
 
 def _relerr(a, b):
@@ -55,7 +57,7 @@ def test_golden(goldens, cls, idx):
     print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
     runs = ([(t, {}) for t in (1, 2, 3, 4, 8, -1)] + [(t, {"coh_build": 1}) for t in (1, 3, -1)]
             if cls is pw.BlockCOHHip
-            else [(-1, {"csb_layout": lay}) for lay in CSB_LAYOUTS])
+            else [(-1, dict(v)) for v in CSB_VARIANTS])
     for threads, opts in runs:
         m = cls(threads, **opts)
         if case["matrix"]["kind"] == "poisson":
@@ -76,11 +78,11 @@ def test_golden(goldens, cls, idx):
 
 # ---------------------------------------------------------------- layouts
 
-@pytest.mark.parametrize("layout", CSB_LAYOUTS)
-@pytest.mark.parametrize("bits", [0, 2, 3, 5, 9])
-def test_csb_layout_bit_exact(bits, layout):
+@pytest.mark.parametrize("variant", CSB_VARIANTS, ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()))
+@pytest.mark.parametrize("bits", [0, 2, 3, 5, 9, 13])
+def test_csb_layout_bit_exact(bits, variant):
     for A in (oracle.poisson_crs(9, 7), oracle.poisson_crs(40, 31), _powerlaw(3000, cap=400)):
-        m = _load(pw.CSBHip, A, block_bits=bits, csb_layout=layout)
+        m = _load(pw.CSBHip, A, block_bits=bits, **variant)
         gb, gind, gdata, gblk = m.csb_layout()
         rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), bits)
         assert gb == rb
@@ -88,8 +90,8 @@ def test_csb_layout_bit_exact(bits, layout):
         assert m.info()["balanced"] == rbal
 
 
-@pytest.mark.parametrize("layout", CSB_LAYOUTS)
-def test_csb_layout_duplicates_and_unbalanced(layout):
+@pytest.mark.parametrize("variant", CSB_VARIANTS, ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()))
+def test_csb_layout_duplicates_and_unbalanced(variant):
     rng = np.random.default_rng(4)
     N = 700
     r = np.concatenate([rng.integers(0, 40, 3000), rng.integers(0, N, 500)]).astype(np.int32)
@@ -98,7 +100,7 @@ def test_csb_layout_duplicates_and_unbalanced(layout):
     v = rng.integers(1, 5, 3500).astype(np.float64)
     t = oracle.Triplets(N, N, r, c, v)
     A = oracle.triplet_to_crs(t)
-    m = pw.CSBHip(csb_layout=layout)
+    m = pw.CSBHip(**variant)
     m.loadFromTriplets(t)
     gb, gind, gdata, gblk = m.csb_layout()
     rb, rind, rdata, rblk, rbal = oracle.csb_build(_triplets_of(A), 0)
@@ -108,6 +110,30 @@ def test_csb_layout_duplicates_and_unbalanced(layout):
     x = rng.standard_normal(N)
     m.mv(x, y)
     assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13
+
+
+@pytest.mark.parametrize("values", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("bits", [12, 13, 14, 15])
+def test_csb_pieces_device_equals_host(values, bits):
+    """Slab pieces with blocks of up to 8 LDS slabs (beta >> k rows, k up to
+    3 at these block sizes): the device build and the host build store the
+    same arrays, and their pieces give the same y up to the rounding of the
+    summation order (a wrong piece would drop or repeat whole entries)."""
+    A = oracle.poisson_crs(230, 190) if values == "bf16" else _powerlaw(45000, seed=5, cap=300)
+    ms = [_load(pw.CSBHip, A, values=values, block_bits=bits, csb_layout=1, csb_build=b, csb_groups=8)
+          for b in (0, 1)]
+    assert [m.info()["layout"] for m in ms] == [1, 1]
+    e0, e1 = ms[0].csb_layout(), ms[1].csb_layout()
+    assert e0[0] == e1[0] == bits
+    for a, b in zip(e0[1:], e1[1:]):
+        assert np.array_equal(a, b)
+    x = np.random.default_rng(bits).standard_normal(A.nor)
+    ys = [np.zeros(A.nor), np.zeros(A.nor)]
+    for m, y in zip(ms, ys):
+        m.mv(x, y)
+    assert _relerr(ys[0], ys[1]) <= (1e-13 if values == "f64" else 1e-5)
+    if values == "f64":
+        assert _relerr(ys[0], oracle.crs_mv(A, x)) <= 1e-13
 
 
 def _check_bands(m, ref):
@@ -185,7 +211,8 @@ def test_spmv_random_and_stencil(cls):
         ref = oracle.crs_mv(A, x)
         csb_opts = ({}, {"block_bits": 7}, {"csb_groups": 1}, {"csb_groups": 3}, {"csb_groups": 8},
                     {"csb_groups": 16}, {"block_bits": 7, "csb_groups": 16}, {"csb_layout": 1},
-                    {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
+                    {"csb_layout": 1, "csb_groups": 8}, {"csb_layout": 1, "csb_build": 1, "csb_groups": 8},
+                    {"csb_layout": 1, "block_bits": 7, "csb_groups": 3},
                     {"csb_layout": 2, "block_bits": 3, "csb_groups": 5}, {"csb_unroll": 2, "csb_groups": 1},
                     {"csb_unroll": 4, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 1},
                     {"csb_nt": 0, "csb_unroll": 4}, {"csb_layout": 3}, {"csb_layout": 3, "csb_groups": 8},

@@ -626,6 +626,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_nt") {
         if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "csb_nt must be -1 (timed), 0 or 1");
         h->csb_nt = (int32_t)value;
+    } else if (k == "csb_build") {
+        if (value < 0 || value > 1) return fail(h, PWM_ERR_INVALID, "csb_build must be 0 (device) or 1 (host)");
+        h->csb_build = (int32_t)value;
     } else if (k == "coh_build") {
         if (value < 0 || value > 1) return fail(h, PWM_ERR_INVALID, "coh_build must be 0 (device) or 1 (host)");
         h->coh_build = (int32_t)value;

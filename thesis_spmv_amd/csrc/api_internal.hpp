@@ -174,6 +174,7 @@ struct pwm_matrix {
     int32_t csb_layout = 0;  // 0 = streams when the packed index fits, 1 = pieces, 2 = streams
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
+    int32_t csb_build = 0;   // CSB slab pieces: 0 = built on the device, 1 = on the host
     int32_t coh_build = 0;   // BlockCOH layout: 0 = built on the device, 1 = on the host
     pwm_impl::CsbDev csb;
     pwm_impl::CohDev coh;
@@ -201,6 +202,9 @@ hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t
 // (csb_build.hip); h->csb.bits/H/V set.  csb_key_bits: bits of its sort key.
 int csb_key_bits(int64_t cells, int bits);
 int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide);
+// CSB slab-piece layout (the stored order, blk_ptr and the pieces of slabs of
+// beta >> k rows) built on the device the same way; h->csb.bits/H/V set.
+int csb_pieces_device(pwm_matrix* h, int k);
 // BlockCOH layout built on the device from the device CRS arrays
 // (coh_build.hip); h->coh.bands holds the band partition (start, rows, nnz,
 // beta_bits, H, V, off_nz) on entry and the per-band sizes and offsets on

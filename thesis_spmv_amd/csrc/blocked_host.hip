@@ -2,9 +2,9 @@
 // stored arrays follow the reference's layouts exactly
 // (SOA_Implementations/CSB.hpp:464-567, BlockCOH.hpp:319-455).  Here: the CSB
 // block size, balanced flag, layout choice and load-time tuning, the BlockCOH
-// band cuts and task list, and the host-built layouts (CSB slab pieces; the
-// BlockCOH layout when coh_build = 1).  The default layouts are sorted on the
-// device (csb_build.hip, coh_build.hip).
+// band cuts and task list, and the host-built layouts (CSB slab pieces when
+// csb_build = 1, the BlockCOH layout when coh_build = 1).  The default
+// layouts are sorted on the device (csb_build.hip, coh_build.hip).
 #include <math.h>
 #include <string.h>
 
@@ -150,9 +150,10 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     // 32 - cb (cb = bits of a column index); when that leaves slabs of fewer
     // than 256 rows (more than 2^24 columns) the automatic choice takes wide
     // streams, which store the column and the row in slab (u16) apart.  The
-    // host-built slab pieces are used when the caller asks for them or the
-    // device build's sort key (slab, block column, Z-Morton code) would need
-    // more than 64 bits.
+    // slab pieces are used when the caller asks for them or the stream sort
+    // key (slab, block column, Z-Morton code) would need more than 64 bits;
+    // they are sorted on the device too (key: block, Z-Morton code), or
+    // built on the host with csb_build = 1.
     const int lds_bits = blocked::lds_rows(h->vtype) == 8192 ? 13 : 12;
     int cb = 0;
     while (((int64_t)1 << cb) < noc) ++cb;
@@ -169,6 +170,13 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     if (use_stream && csb_key_bits(V * S_stream * H, bits) <= 64) {
         // the slab streams, sorted on the device from the device CRS arrays
         rc = csb_stream_device(h, rb, cb, S_stream, wide);
+        if (rc != PWM_OK) return rc;
+    } else if (h->csb_build != 1 && csb_key_bits(V * H, bits) <= 64) {
+        // slab pieces over the stored order, sorted on the device
+        const int64_t lds_rows = (int64_t)blocked::lds_rows(h->vtype);
+        int k = 0;
+        while ((beta >> k) > lds_rows) ++k;
+        rc = csb_pieces_device(h, k);
         if (rc != PWM_OK) return rc;
     } else {
         // slab pieces over the stored order, built on the host

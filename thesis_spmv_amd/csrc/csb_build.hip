@@ -11,6 +11,11 @@
 // packed indices and values are then gathered through the sorted positions,
 // each (slab, block column) run start is found by binary search, and the
 // reference's blk_ptr follows from the run lengths.
+//
+// The slab-piece layout (the stored order itself, CSB.hpp:464-567) sorts the
+// same keys with one slab per block row (cell = br*H + bc), packs each index
+// as (row in block << 16) | column in block, and finds each block's pieces by
+// binary search for its Z-order prefixes.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "api_internal.hpp"
@@ -40,6 +45,7 @@ __device__ __forceinline__ uint32_t compact16(uint32_t v) {
 struct CsbKeyArgs {
     int64_t nor, nnz, S, H;
     int bits, rb, cb;
+    int local;  // 1: index = (row in block << 16) | column in block
 };
 
 __global__ void k_csb_keys(CsbKeyArgs a, const int32_t* __restrict__ rs, const int32_t* __restrict__ ci,
@@ -75,7 +81,9 @@ __global__ void k_csb_gather(CsbKeyArgs a, const uint64_t* __restrict__ keys, co
         const uint64_t bc = (key >> (2 * a.bits)) % (uint64_t)a.H;
         const uint32_t lrow = compact16(zk >> 1), lcol = compact16(zk);
         const uint32_t col = (uint32_t)(bc << a.bits) + lcol;
-        if (srow) {
+        if (a.local) {
+            sind[p] = (lrow << 16) | lcol;
+        } else if (srow) {
             sind[p] = col;
             srow[p] = (uint16_t)(lrow & smask);
         } else {
@@ -85,12 +93,12 @@ __global__ void k_csb_gather(CsbKeyArgs a, const uint64_t* __restrict__ keys, co
     }
 }
 
-// soff[cell] = number of sorted keys below cell << 2b, cell < cells
-__global__ void k_csb_soff(const uint64_t* __restrict__ keys, int64_t nnz, int64_t cells, int bits,
+// soff[cell] = number of sorted keys below cell << shift, cell < cells
+__global__ void k_csb_soff(const uint64_t* __restrict__ keys, int64_t nnz, int64_t cells, int shift,
                            int32_t* __restrict__ soff) {
     for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < cells;
          cell += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = (uint64_t)cell << (2 * bits);
+        const uint64_t t = (uint64_t)cell << shift;
         int64_t lo = 0, hi = nnz;
         while (lo < hi) {
             const int64_t mid = (lo + hi) >> 1;
@@ -118,6 +126,47 @@ int csb_key_bits(int64_t cells, int bits) {
     return cb + 2 * bits;
 }
 
+// Keys, one stable sort and the gather of indices and values into c.ind /
+// c.data (and c.srows for wide streams); the sorted keys are left in keys_out.
+static int csb_sort_entries(pwm_matrix* h, const pwmk::CsbKeyArgs& a, unsigned end_bit, DevBuf& keys_out) {
+    CsbDev& c = h->csb;
+    const int64_t nnz = h->nnz;
+    const size_t vs = value_size(h->vtype);
+    hipError_t e;
+    DevBuf keys, pos, pos_out, tmp;
+    if ((e = keys.alloc(nnz * 8)) != hipSuccess || (e = keys_out.alloc(nnz * 8)) != hipSuccess ||
+        (e = pos.alloc(nnz * 4)) != hipSuccess || (e = pos_out.alloc(nnz * 4)) != hipSuccess)
+        return hip_fail(h, e, "CSB build allocation");
+    hipLaunchKernelGGL(pwmk::k_csb_keys, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream, a,
+                       h->rs.as<int32_t>(), h->ci.as<int32_t>(), keys.as<uint64_t>(), pos.as<uint32_t>());
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB keys");
+    size_t tb = 0;
+    e = rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
+                                  pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
+    if (e == hipSuccess) e = tmp.alloc(std::max<size_t>(tb, 4));
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                      pos.as<uint32_t>(), pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit,
+                                      h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "CSB sort");
+    const dim3 g(pwmk::grid_of(nnz));
+    if (vs == 8)
+        hipLaunchKernelGGL(pwmk::k_csb_gather<uint64_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                           pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
+                           c.data.as<uint64_t>());
+    else if (vs == 4)
+        hipLaunchKernelGGL(pwmk::k_csb_gather<uint32_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                           pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
+                           c.data.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(pwmk::k_csb_gather<uint16_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
+                           pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
+                           c.data.as<uint16_t>());
+    if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(h->stream);  // before the temporaries go
+    if (e != hipSuccess) return hip_fail(h, e, "CSB gather");
+    return PWM_OK;
+}
+
 int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide) {
     CsbDev& c = h->csb;
     const int bits = c.bits;
@@ -134,41 +183,14 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide) {
         (wide && (e = hipMemsetAsync(c.srows.p, 0, zpad * 2, h->stream)) != hipSuccess))
         return hip_fail(h, e, "CSB stream allocation");
     if (!wide) c.srows.release();
-    pwmk::CsbKeyArgs a{nor, nnz, S, H, bits, rb, cb};
+    pwmk::CsbKeyArgs a{nor, nnz, S, H, bits, rb, cb, 0};
     std::vector<int32_t> soff(cells + 1, 0);
     if (nnz > 0) {
-        DevBuf keys, keys_out, pos, pos_out, tmp;
-        if ((e = keys.alloc(nnz * 8)) != hipSuccess || (e = keys_out.alloc(nnz * 8)) != hipSuccess ||
-            (e = pos.alloc(nnz * 4)) != hipSuccess || (e = pos_out.alloc(nnz * 4)) != hipSuccess)
-            return hip_fail(h, e, "CSB build allocation");
-        hipLaunchKernelGGL(pwmk::k_csb_keys, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream, a,
-                           h->rs.as<int32_t>(), h->ci.as<int32_t>(), keys.as<uint64_t>(), pos.as<uint32_t>());
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB keys");
-        const unsigned end_bit = (unsigned)std::max(csb_key_bits(cells, bits), 1);
-        size_t tb = 0;
-        e = rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
-                                      pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
-        if (e == hipSuccess) e = tmp.alloc(std::max<size_t>(tb, 4));
-        if (e == hipSuccess)
-            e = rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
-                                          pos.as<uint32_t>(), pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit,
-                                          h->stream);
-        if (e != hipSuccess) return hip_fail(h, e, "CSB sort");
-        const dim3 g(pwmk::grid_of(nnz));
-        if (vs == 8)
-            hipLaunchKernelGGL(pwmk::k_csb_gather<uint64_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
-                               c.data.as<uint64_t>());
-        else if (vs == 4)
-            hipLaunchKernelGGL(pwmk::k_csb_gather<uint32_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
-                               c.data.as<uint32_t>());
-        else
-            hipLaunchKernelGGL(pwmk::k_csb_gather<uint16_t>, g, dim3(256), 0, h->stream, a, keys_out.as<uint64_t>(),
-                               pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(), c.srows.as<uint16_t>(),
-                               c.data.as<uint16_t>());
+        DevBuf keys_out;
+        const int rc = csb_sort_entries(h, a, (unsigned)std::max(csb_key_bits(cells, bits), 1), keys_out);
+        if (rc != PWM_OK) return rc;
         hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
-                           keys_out.as<uint64_t>(), nnz, cells, bits, c.soff.as<int32_t>());
+                           keys_out.as<uint64_t>(), nnz, cells, 2 * bits, c.soff.as<int32_t>());
         if ((e = hipGetLastError()) == hipSuccess)
             e = hipMemcpyAsync(soff.data(), c.soff.p, cells * 4, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -197,6 +219,78 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide) {
     c.slabs = (int32_t)S;
     c.slab_rows = (int64_t)1 << rb;
     c.pieces_per_block = 1;
+    return PWM_OK;
+}
+
+int csb_pieces_device(pwm_matrix* h, int k) {
+    CsbDev& c = h->csb;
+    const int bits = c.bits;
+    const int64_t H = c.H, V = c.V, nnz = h->nnz, nor = h->nor;
+    const int64_t cells = V * H;
+    // a slab is the top k row bits of a block; its entries in one block form
+    // 2^(k-1) Z-order runs, one per value of the top k-1 column bits (k = 0:
+    // the whole block).  Prefix q of a key's top 2k-1 Z bits is r c r c ... r.
+    const int64_t S = (int64_t)1 << k;
+    const int64_t np = k == 0 ? 1 : ((int64_t)1 << (k - 1));
+    const int64_t nprefix = k == 0 ? 1 : ((int64_t)1 << (2 * k - 1));
+    const int shift = 2 * bits - (k == 0 ? 0 : 2 * k - 1);
+    const int64_t sub = cells * nprefix;
+    const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
+    const size_t vs = value_size(h->vtype);
+    hipError_t e;
+    DevBuf soff;
+    if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess ||
+        (e = soff.alloc((sub + 1) * 4)) != hipSuccess)
+        return hip_fail(h, e, "CSB allocation");
+    if ((e = hipMemsetAsync(c.ind.p, 0, zpad * 4, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "CSB allocation");
+    c.srows.release();
+    c.soff.release();
+    const pwmk::CsbKeyArgs a{nor, nnz, 1, H, bits, bits, 16, 1};
+    std::vector<int32_t> off(sub + 1, 0);
+    if (nnz > 0) {
+        DevBuf keys_out;
+        const int rc = csb_sort_entries(h, a, (unsigned)std::max(csb_key_bits(cells, bits), 1), keys_out);
+        if (rc != PWM_OK) return rc;
+        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(sub)), dim3(256), 0, h->stream,
+                           keys_out.as<uint64_t>(), nnz, sub, shift, soff.as<int32_t>());
+        if ((e = hipGetLastError()) == hipSuccess)
+            e = hipMemcpyAsync(off.data(), soff.p, sub * 4, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "CSB build");
+    }
+    off[sub] = (int32_t)nnz;
+    std::vector<int32_t> blk_ptr(cells + 1);
+    for (int64_t cell = 0; cell <= cells; ++cell) blk_ptr[cell] = off[cell * nprefix];
+    // task t = br*S + s; pieces[t][bc][j] = (start, end)
+    std::vector<blocked::Piece> pieces(V * S * H * np);
+    parallel_for(cells, 4096, [&](int64_t c0, int64_t c1, int) {
+        for (int64_t cell = c0; cell < c1; ++cell) {
+            const int64_t br = cell / H, bc = cell % H;
+            for (int64_t q = 0; q < nprefix; ++q) {
+                int64_t sr = 0, pc = 0;
+                for (int lvl = 0; lvl < k; ++lvl) {
+                    sr = (sr << 1) | ((q >> (2 * k - 2 - 2 * lvl)) & 1);
+                    if (lvl < k - 1) pc = (pc << 1) | ((q >> (2 * k - 3 - 2 * lvl)) & 1);
+                }
+                pieces[((br * S + sr) * H + bc) * np + pc] = {off[cell * nprefix + q], off[cell * nprefix + q + 1]};
+            }
+        }
+    });
+    if ((e = c.blk_ptr.alloc(blk_ptr.size() * 4)) != hipSuccess ||
+        (e = hipMemcpy(c.blk_ptr.p, blk_ptr.data(), blk_ptr.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = c.work.alloc(pieces.size() * sizeof(blocked::Piece))) != hipSuccess ||
+        (e = hipMemcpy(c.work.p, pieces.data(), pieces.size() * sizeof(blocked::Piece), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+        return hip_fail(h, e, "CSB build");
+    c.stream = 0;
+    c.wide = 0;
+    c.col_bits = 0;
+    c.slab_bits = bits - k;
+    c.slabs = (int32_t)S;
+    c.slab_rows = ((int64_t)1 << bits) >> k;
+    c.pieces_per_block = (int32_t)np;
     return PWM_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <numeric>
@@ -941,8 +942,11 @@ static int upload_vec(pwm_matrix* h, const double* src, void* dst, int64_t n) {
     if (h->vtype == PWM_VALUES_F64) {
         e = upload_staged(dst, src, n * 8, h->stream);
     } else {
-        std::vector<float> f(src, src + n);
-        e = upload_staged(dst, f.data(), n * 4, h->stream);
+        std::unique_ptr<float[]> f(new float[std::max<int64_t>(n, 1)]);
+        parallel_for(n, (int64_t)1 << 18, [&](int64_t i0, int64_t i1, int) {
+            for (int64_t i = i0; i < i1; ++i) f[i] = (float)src[i];
+        });
+        e = upload_staged(dst, f.get(), n * 4, h->stream);
     }
     if (e != hipSuccess) return hip_fail(h, e, "vector upload");
     return PWM_OK;
@@ -952,9 +956,12 @@ static int download_vec(pwm_matrix* h, const void* src, double* dst, int64_t n) 
     if (h->vtype == PWM_VALUES_F64) {
         e = download_staged(dst, src, n * 8, h->stream);
     } else {
-        std::vector<float> f(n);
-        e = download_staged(f.data(), src, n * 4, h->stream);
-        for (int64_t i = 0; i < n; ++i) dst[i] = f[i];
+        std::unique_ptr<float[]> f(new float[std::max<int64_t>(n, 1)]);
+        e = download_staged(f.get(), src, n * 4, h->stream);
+        if (e == hipSuccess)
+            parallel_for(n, (int64_t)1 << 18, [&](int64_t i0, int64_t i1, int) {
+                for (int64_t i = i0; i < i1; ++i) dst[i] = f[i];
+            });
     }
     if (e != hipSuccess) return hip_fail(h, e, "vector download");
     return PWM_OK;

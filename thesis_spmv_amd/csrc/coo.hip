@@ -81,8 +81,7 @@ int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, cons
         (e = pos_out.alloc(nnz * 4)) != hipSuccess)
         return hip_fail(h, e, "COO conversion allocation");
     if ((e = upload_staged(d_rows.p, rows, nnz * 4, h->stream)) != hipSuccess ||
-        (e = upload_staged(d_cols.p, cols, nnz * 4, h->stream)) != hipSuccess ||
-        (e = upload_staged(d_vals.p, vals, nnz * 8, h->stream)) != hipSuccess)
+        (e = upload_staged(d_cols.p, cols, nnz * 4, h->stream)) != hipSuccess)
         return hip_fail(h, e, "COO upload");
     hipLaunchKernelGGL(pwmk::k_coo_keys, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream, d_rows.as<int32_t>(),
                        d_cols.as<int32_t>(), nnz, cb, keys.as<uint64_t>(), pos.as<uint32_t>());
@@ -97,6 +96,18 @@ int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, cons
         e = rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
                                       pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "COO sort");
+    // the values are needed only by the gather: their upload runs on a stream
+    // of its own while the sort runs, and has completed when this returns
+    {
+        hipStream_t vs = nullptr;
+        e = hipStreamCreateWithFlags(&vs, hipStreamNonBlocking);
+        if (e == hipSuccess) e = upload_staged(d_vals.p, vals, nnz * 8, vs);
+        if (vs) {
+            const hipError_t e2 = hipStreamDestroy(vs);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return hip_fail(h, e, "COO upload");
+    }
     hipLaunchKernelGGL(pwmk::k_coo_gather, dim3(pwmk::grid_of(nnz)), dim3(256), 0, h->stream,
                        keys_out.as<uint64_t>(), pos_out.as<uint32_t>(), d_vals.as<double>(), nnz,
                        ((uint64_t)1 << cb) - 1, ci.as<int32_t>(), dv.as<double>());

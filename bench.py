@@ -96,7 +96,7 @@ def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0, y_gpu=No
     def timed(fn):
         fn()  # warm-up
         times, t_start = [], time.time()
-        while len(times) < 30 and time.time() - t_start < budget_s:
+        while len(times) < 100 and time.time() - t_start < budget_s:
             t0 = time.perf_counter()
             fn()
             times.append(time.perf_counter() - t0)

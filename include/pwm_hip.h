@@ -163,7 +163,12 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
 int pwm_set_option(pwm_matrix* h, const char* key, int64_t value);
 
 /* ---- device-resident additions ----------------------------------------
- * Vectors live in HBM: double for PWM_VALUES_F64, float otherwise. */
+ * Vectors live in HBM: double for PWM_VALUES_F64, float otherwise.  A device
+ * vector must be coarse-grained device memory (hipMalloc, or
+ * pwm_device_alloc) holding the whole vector from the pointer on; host
+ * memory (pinned or not), managed memory and fine-grained device memory are
+ * refused with PWM_ERR_INVALID before anything is launched, since the kernels
+ * add into y with hardware FP64 atomics that such memory does not support. */
 int pwm_mv_device(pwm_matrix* h, const void* x_dev, void* y_dev);
 
 /* Power method on device vectors with an FP64 eigenvalue estimate and a
@@ -222,6 +227,17 @@ int pwm_device_copy(void* dst, const void* src, int64_t bytes);
  * between steps). */
 int pwm_time_mv(pwm_matrix* h, const void* x_dev, void* y_dev, int reps, int64_t flush_bytes,
                 double* ms_per_mv, double* kernel_ms);
+
+/* Timing (addition, used by bench.py): `reps` rounds of one SpMV on each of
+ * `count` loaded handles in turn (hs[k] with xs[k], ys[k]), all launched on
+ * hs[0]'s stream, between HIP events.  When the handles' arrays together
+ * exceed the 256 MB Infinity Cache, each SpMV finds its matrix in HBM: the
+ * back-to-back rate of a matrix that alone would fit that cache.  Writes the
+ * mean milliseconds per SpMV and, if kernel_ms is not NULL, of the dominant
+ * kernel alone (events around every SpMV; NULL: one pair of events around
+ * all of them).  Calls on these handles must not overlap it. */
+int pwm_time_mv_rotating(pwm_matrix* const* hs, const void* const* xs, void* const* ys, int count, int reps,
+                         double* ms_per_mv, double* kernel_ms);
 
 /* Gather-rate probe (addition, used by bench.py): mean milliseconds of one
  * pass over n entries (rounded down to a multiple of 8192) of a stream of
@@ -325,7 +341,12 @@ const char* pwm_server_last_error(const pwm_server* s);
  * launches), "persistent" (as for pwm_batch_set_option), "items_per_thread"
  * (4, 8, 16, default 16; only with nothing resident). */
 int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
-#define PWM_SERVER_BORROW 1 /* keep the caller's arrays (alive until removed) instead of a copy */
+/* PWM_SERVER_BORROW: keep the caller's arrays instead of a copy.  They must
+ * stay alive and unchanged until the matrix is removed; they are checked
+ * again (row_start order, column range) before every upload, and a matrix
+ * that no longer passes makes pwm_server_step fail with PWM_ERR_INVALID
+ * (remove it to go on). */
+#define PWM_SERVER_BORROW 1
 int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* row_start, const int32_t* col_ind,
                           const double* vals, int32_t flags, int32_t* mat);
 /* 2D Poisson matrix, generated on the device when it becomes resident. */

@@ -57,7 +57,7 @@ static void run_all(const char* itype) {
     std::vector<int32_t> r32(tr.begin(), tr.end()), c32(tc.begin(), tc.end());
     orc_triplet_to_crs(R, (int64_t)tr.size(), r32.data(), c32.data(), tv.data(), trs.data(), tci.data(), tdv.data());
 
-    for (int method : {101, 108}) {
+    for (int method : {101, 102, 108, 109, 110}) {
         for (int threads : {1, 3, -1}) {
             std::unique_ptr<pwm::SparseMatrix<double, I>> mat(pwm::selectTypeHip<double, I>(method, threads));
             char name[128];

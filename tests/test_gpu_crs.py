@@ -219,25 +219,6 @@ def test_edge_shapes(cls):
             assert _relerr(y, ref) <= 1e-13
 
 
-def test_stencil9_and_powerlaw_full_config1():
-    """Config-1 size (N = 4 194 304, ~67M nnz) against the oracle's CRS::mv."""
-    A = _powerlaw(4194304)
-    x = pw.gen_uniform(A.noc, 0)
-    ref = oracle.crs_mv(A, x)
-    for cls in FORMATS:
-        m = _load(cls, A)
-        y = np.zeros(A.nor)
-        m.mv(x, y)
-        err = _relerr(y, ref)
-        print(f"{cls.__name__} config-1 rel.err {err:.3e}", flush=True)
-        assert err <= 1e-13
-    # merge split points at the device's tile count, bit-exact
-    m = _load(pw.CRSMergeHip, A)
-    a, b = m.merge_splits()
-    ra, rb = oracle.merge_splits(A, len(a) - 1)
-    assert np.array_equal(a, ra) and np.array_equal(b, rb)
-
-
 def test_device_vectors_and_errors():
     A = oracle.poisson_crs(40, 30)
     m = _load(pw.CRSHip, A)
@@ -551,3 +532,76 @@ def test_host_array_mv_equals_device_mv(values, n):
     xa, ya = x.copy(), np.zeros(n)
     m.powerIteration(xa, ya, 3)
     assert np.all(np.isfinite(xa)) and np.all(np.isfinite(ya))
+
+
+# ---------------------------------------------------------------- device pointers
+
+def _hip_runtime():
+    """The HIP runtime libpwmhip.so runs on (already loaded in this process)."""
+    import ctypes
+    pw.lib()
+    with open("/proc/self/maps") as f:
+        paths = {ln.split()[-1] for ln in f if "libamdhip64.so" in ln}
+    assert paths, "libamdhip64 not loaded"
+    return ctypes.CDLL(sorted(paths)[0])
+
+
+class _Ptr:
+    def __init__(self, p):
+        self.ptr = p
+
+
+def test_device_entry_points_refuse_host_and_short_buffers():
+    """pwm_mv_device / pwm_power_method_device / pwm_time_mv take hipMalloc
+    device memory only: pinned host memory (hipHostMalloc), pageable host
+    memory and a device buffer shorter than the vector are refused with
+    PWM_ERR_INVALID before any kernel runs; y keeps its contents."""
+    import ctypes
+    A = oracle.poisson_crs(512, 256)
+    x = pw.DeviceVector.from_array(np.ones(A.nor))
+    y = pw.DeviceVector.from_array(np.full(A.nor, 7.0))
+    hip = _hip_runtime()
+    pinned = ctypes.c_void_p()
+    assert hip.hipHostMalloc(ctypes.byref(pinned), ctypes.c_size_t(A.nor * 8), ctypes.c_uint(0)) == 0
+    host = np.zeros(A.nor)
+    # the second half of a device buffer of nor doubles: half a vector short
+    buf = pw.DeviceVector(A.nor)
+    short = _Ptr(ctypes.c_void_p(buf.ptr.value + A.nor * 4))
+    try:
+        for cls in (pw.CRSHip, pw.CRSMergeHip, pw.CSBHip, pw.BlockCOHHip):
+            m = _load(cls, A)
+            for bad in (_Ptr(pinned), _Ptr(host.ctypes.data_as(ctypes.c_void_p)), short):
+                for call in (lambda: m.mv_device(x, bad), lambda: m.mv_device(bad, y),
+                             lambda: m.power_method_device(x, bad, 3),
+                             lambda: m.time_mv(x, bad, 1)):
+                    with pytest.raises(pw.PwmError) as e:
+                        call()
+                    assert e.value.code == -1, (cls.__name__, str(e.value))
+            assert np.all(y.download() == 7.0)
+            # device memory still works
+            m.mv_device(x, y)
+            assert np.array_equal(y.download(), oracle.crs_mv(A, np.ones(A.nor)))
+            y.fill(7.0)
+    finally:
+        hip.hipHostFree(pinned)
+
+
+def test_poisson_vector_kernel_long_row_threshold_below_row_length():
+    """A generated Poisson matrix on the vector kernel with the long-row
+    threshold under the row length: every row goes through the long-row
+    path and still matches the oracle."""
+    for m_, n_ in ((7, 5), (64, 48), (300, 200)):
+        A = oracle.poisson_crs(m_, n_)
+        for lr in (1, 3, 4):
+            m = pw.CRSHip(-1, "f64", crs_kernel=1, long_row=lr)
+            m.generatePoissonMatrix(m_, n_, 1)
+            assert m.info()["long_rows"] > 0
+            x = pw.gen_uniform(A.nor, 3)
+            y = np.full(A.nor, np.nan)
+            m.mv(x, y)
+            assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13, (m_, n_, lr)
+            xs, ys = np.ones(A.nor), np.zeros(A.nor)
+            m.powerIteration(xs, ys, 6)
+            xr, yr = oracle.crs_power_iteration(A, np.ones(A.nor), 6)
+            assert np.max(np.abs(xs - xr)) <= 1e-12 and np.max(np.abs(ys - yr)) <= 1e-12
+

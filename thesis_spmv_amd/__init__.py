@@ -69,6 +69,8 @@ _SIGS = {
     "pwm_device_fill": [VP, I64, C.c_int, F64],
     "pwm_device_copy": [VP, VP, I64],
     "pwm_time_mv": [VP, VP, VP, C.c_int, I64, C.POINTER(F64), C.POINTER(F64)],
+    "pwm_time_mv_rotating": [C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.c_int, C.c_int, C.POINTER(F64),
+                             C.POINTER(F64)],
     "pwm_matrix_stats": [VP, VP],
     "pwm_gather_rate": [VP, I64, I64, C.c_int, I64, C.POINTER(F64)],
     "pwm_gen_powerlaw": [I64, F64, F64, I64, C.c_uint64, C.c_uint64, C.POINTER(VP), C.POINTER(VP),
@@ -175,6 +177,20 @@ class DeviceVector:
             self.free()
         except Exception:
             pass
+
+
+def time_mv_rotating(mats, xs, ys, reps, per_step=True):
+    """(ms per SpMV, ms of its dominant kernel or None) of `reps` rounds of one
+    SpMV on each handle in `mats` in turn, back to back on one stream
+    (pwm_time_mv_rotating): with enough independent matrices, none of them is
+    still in the Infinity Cache when its turn comes."""
+    k = len(mats)
+    hs = (VP * k)(*[m.h for m in mats])
+    xp = (VP * k)(*[x.ptr for x in xs])
+    yp = (VP * k)(*[y.ptr for y in ys])
+    ms, kms = F64(), F64()
+    _check(lib().pwm_time_mv_rotating(hs, xp, yp, k, int(reps), C.byref(ms), C.byref(kms) if per_step else None))
+    return ms.value, (kms.value if per_step else None)
 
 
 def gather_rate(x, window, per_xcd, n):
@@ -428,6 +444,7 @@ class Batch:
         self.h = VP()
         _check(lib().pwm_batch_create(VALUES[values], C.byref(self.h)))
         self.values = values
+        self._sizes = []  # rows of each instance, in id order
         for k, v in options.items():
             self.set_option(k, v)
 
@@ -457,12 +474,24 @@ class Batch:
         dv = np.ascontiguousarray(data, np.float64)
         i = I32()
         self._chk(lib().pwm_batch_add_crs(self.h, len(rs) - 1, _ptr(rs), _ptr(ci), _ptr(dv), C.byref(i)))
+        self._sizes.append(len(rs) - 1)
         return i.value
 
     def add_poisson(self, m, n):
         i = I32()
         self._chk(lib().pwm_batch_add_poisson(self.h, int(m), int(n), C.byref(i)))
+        self._sizes.append(int(m) * int(n))
         return i.value
+
+    def _rows(self, i, n):
+        """Rows of instance i; a caller's n that disagrees is an error (the
+        library writes the instance's own number of entries)."""
+        if not 0 <= int(i) < len(self._sizes):
+            raise PwmError(-1, f"batch instance {i} does not exist")
+        rows = self._sizes[int(i)]
+        if n is not None and int(n) != rows:
+            raise PwmError(-1, f"batch instance {i} has {rows} rows, not {n}")
+        return rows
 
     def size(self):
         a, b, c = I32(), I64(), I64()
@@ -478,13 +507,13 @@ class Batch:
                                         C.byref(ms)))
         return lam, its, ms.value
 
-    def start_vector(self, i, n, seed=0):
-        out = np.zeros(int(n))
+    def start_vector(self, i, n=None, seed=0):
+        out = np.zeros(self._rows(i, n))
         self._chk(lib().pwm_batch_start_vector(self.h, int(i), int(seed), _ptr(out)))
         return out
 
-    def vector(self, i, n):
-        out = np.zeros(int(n))
+    def vector(self, i, n=None):
+        out = np.zeros(self._rows(i, n))
         yn = F64()
         self._chk(lib().pwm_batch_vector(self.h, int(i), _ptr(out), C.byref(yn)))
         return out, yn.value
@@ -503,6 +532,7 @@ class Server:
         self.values = values
         self._sizes = {}
         self._held = {}
+        self._job_rows = {}  # job id -> rows of its matrix
         for k, v in options.items():
             self.set_option(k, v)
 
@@ -551,6 +581,7 @@ class Server:
     def submit(self, mat, max_it, tol=-1.0, seed=0):
         j = I64()
         self._chk(lib().pwm_server_submit(self.h, int(mat), int(max_it), float(tol), int(seed), C.byref(j)))
+        self._job_rows[j.value] = self._sizes[int(mat)]
         return j.value
 
     def step(self):
@@ -570,11 +601,17 @@ class Server:
         return buf[:cnt.value].tolist()
 
     def result(self, job, n=None):
-        """(lambda, ynorm, iterations, vector or None); raises while unfinished."""
+        """(lambda, ynorm, iterations, vector or None); raises while unfinished.
+        The vector is returned when n is given; n must be the rows of the job's
+        matrix (the library writes that many entries)."""
         lam, yn, its = F64(), F64(), I64()
+        rows = self._job_rows.get(int(job))
+        if n is not None and rows is not None and int(n) != rows:
+            raise PwmError(-1, f"job {job} has {rows} rows, not {n}")
         vec = None if n is None else np.zeros(int(n))
         self._chk(lib().pwm_server_result(self.h, int(job), C.byref(lam), C.byref(yn), C.byref(its),
                                           None if vec is None else _ptr(vec)))
+        self._job_rows.pop(int(job), None)
         return lam.value, yn.value, its.value, vec
 
     def timing(self):

@@ -372,9 +372,20 @@ static int plan_crs_vector(pwm_matrix* h, int G) {
     h->lr.threshold = (int32_t)std::min<int64_t>(thr, INT32_MAX);
     std::vector<pwmk::LongChunk> chunks;
     std::vector<int32_t> ids, begin;
-    if (!h->h_rs.empty()) {
+    // the row offsets: the host copy when the load kept one, else read back
+    // from the device (every row longer than the threshold must be planned,
+    // since the vector kernel skips them)
+    std::vector<int32_t> rs_dev;
+    const int32_t* rsh = h->h_rs.empty() ? nullptr : h->h_rs.data();
+    if (!rsh && h->nor > 0) {
+        rs_dev.resize(h->nor + 1);
+        hipError_t e = hipMemcpy(rs_dev.data(), h->rs.p, (h->nor + 1) * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(h, e, "long-row plan: row offsets");
+        rsh = rs_dev.data();
+    }
+    if (rsh) {
         for (int64_t r = 0; r < h->nor; ++r) {
-            const int64_t s = h->h_rs[r], e = h->h_rs[r + 1];
+            const int64_t s = rsh[r], e = rsh[r + 1];
             if (e - s <= thr) continue;
             begin.push_back((int32_t)chunks.size());
             for (int64_t k = s; k < e; k += h->chunk) {
@@ -398,9 +409,10 @@ static int plan_crs_vector(pwm_matrix* h, int G) {
             (e = h->lr_begin.alloc(begin.size() * 4)) != hipSuccess ||
             (e = h->lr_partial.alloc(chunks.size() * 8)) != hipSuccess)
             return hip_fail(h, e, "long-row plan allocation");
-        (void)hipMemcpy(h->lr_chunks.p, chunks.data(), chunks.size() * sizeof(pwmk::LongChunk), hipMemcpyHostToDevice);
-        (void)hipMemcpy(h->lr_rows.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice);
-        if ((e = hipMemcpy(h->lr_begin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        if ((e = hipMemcpy(h->lr_chunks.p, chunks.data(), chunks.size() * sizeof(pwmk::LongChunk),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->lr_rows.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(h->lr_begin.p, begin.data(), begin.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
             return hip_fail(h, e, "long-row plan upload");
         h->lr.chunks = h->lr_chunks.as<pwmk::LongChunk>();
         h->lr.row_ids = h->lr_rows.as<int32_t>();
@@ -911,6 +923,43 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
 
 // ---------------------------------------------------------------- SpMV
 
+// A vector handed to a _device entry point must be coarse-grained device
+// memory (hipMalloc) that holds `bytes` from p.  Host memory, pinned
+// (hipHostMalloc) or not, managed memory and fine-grained device allocations
+// are refused: the kernels add into y with hardware FP64 atomics (the library
+// is built with unsafe FP atomics), which are not performed on fine-grained
+// or host memory.
+static int check_device_vec(pwm_matrix* h, const void* p, size_t bytes, const char* who, const char* name) {
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // a refused pointer must not show up as a later launch error
+        return fail(h, PWM_ERR_INVALID, std::string(who) + ": " + name + " is not device memory");
+    }
+    if (a.type != hipMemoryTypeDevice || a.isManaged)
+        return fail(h, PWM_ERR_INVALID, std::string(who) + ": " + name + " is host or managed memory, not hipMalloc device memory");
+    if (a.allocationFlags & hipDeviceMallocFinegrained)
+        return fail(h, PWM_ERR_INVALID, std::string(who) + ": " + name + " is fine-grained device memory");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(h, PWM_ERR_INVALID, std::string(who) + ": " + name + " has no device allocation");
+    }
+    if ((const char*)p + bytes > (const char*)base + size)
+        return fail(h, PWM_ERR_INVALID, std::string(who) + ": " + name + " is shorter than the vector");
+    return PWM_OK;
+}
+
+static int check_device_xy(pwm_matrix* h, const void* x, size_t x_elems, const void* y, size_t y_elems,
+                           const char* who) {
+    const size_t vs = vector_size(h->vtype);
+    int rc = check_device_vec(h, x, x_elems * vs, who, "x");
+    if (rc == PWM_OK) rc = check_device_vec(h, y, y_elems * vs, who, "y");
+    return rc;
+}
+
 extern "C" int pwm_mv_device(pwm_matrix* h, const void* x, void* y) {
     if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_mv_device: NULL handle");
     if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_mv_device: no matrix loaded");
@@ -920,6 +969,7 @@ extern "C" int pwm_mv_device(pwm_matrix* h, const void* x, void* y) {
     const char* yb = (const char*)y;
     if (xb < yb + h->nor * vs && yb < xb + h->noc * vs)
         return fail(h, PWM_ERR_INVALID, "pwm_mv_device: x and y overlap");
+    if (int rc0 = check_device_xy(h, x, h->noc, y, h->nor, "pwm_mv_device")) return rc0;
     int rc = spmv_launch(h, x, y, nullptr, nullptr);
     if (rc != PWM_OK) return rc;
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -1072,6 +1122,7 @@ extern "C" int pwm_power_method_device(pwm_matrix* h, void* x, void* y, int64_t 
     const size_t vs = vector_size(h->vtype);
     if ((const char*)x < (const char*)y + h->nor * vs && (const char*)y < (const char*)x + h->nor * vs)
         return fail(h, PWM_ERR_INVALID, "pwm_power_method_device: x and y overlap");
+    if (int rc0 = check_device_xy(h, x, h->nor, y, h->nor, "pwm_power_method_device")) return rc0;
     int64_t steps = 0;
     int rc = run_power(h, x, y, max_it, tol, &steps, lambda, ynorm);
     if (iterations) *iterations = steps;
@@ -1182,10 +1233,49 @@ extern "C" int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window,
     return PWM_OK;
 }
 
+// One SpMV of h on h->stream with `mid` recorded after its dominant kernel
+// (the tiles / vector / merge / CSB slab kernel; the whole SpMV for BlockCOH).
+static int launch_marked(pwm_matrix* h, const void* x, void* y, hipEvent_t mid) {
+    const CrsDev A = crs_of(h);
+    hipError_t e = hipSuccess;
+    switch (h->format) {
+        case PWM_FORMAT_CRS:
+            if (h->crs_used == 2) {
+                e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, nullptr, h->stream);
+                (void)hipEventRecord(mid, h->stream);
+                if (e == hipSuccess && h->tiles_carry) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+                break;
+            }
+            e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, nullptr, h->stream);
+            (void)hipEventRecord(mid, h->stream);
+            if (e == hipSuccess && h->lr.n_rows > 0)
+                e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, nullptr, h->stream);
+            break;
+        case PWM_FORMAT_CRS_MERGE:
+            e = pwmk::launch_merge_spmv(A, h->mp, x, y, nullptr, h->stream);
+            (void)hipEventRecord(mid, h->stream);
+            if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
+            break;
+        case PWM_FORMAT_CSB:
+            e = csb_launch(h, x, y, nullptr, 1);
+            (void)hipEventRecord(mid, h->stream);
+            if (e == hipSuccess) e = csb_launch(h, x, y, nullptr, 2);
+            break;
+        default: {
+            const int rc = spmv_launch(h, x, y, nullptr, nullptr);
+            (void)hipEventRecord(mid, h->stream);
+            return rc;
+        }
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "timed SpMV");
+    return PWM_OK;
+}
+
 extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int64_t flush_bytes, double* ms_per_mv,
                            double* kernel_ms) {
     if (!h || !x || !y || reps <= 0) return fail(h, PWM_ERR_INVALID, "pwm_time_mv: bad argument");
     if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_time_mv: no matrix loaded");
+    if (int rc0 = check_device_xy(h, x, h->noc, y, h->nor, "pwm_time_mv")) return rc0;
     hipError_t e;
     if (flush_bytes > 0 && (int64_t)h->flush.bytes < flush_bytes && (e = h->flush.alloc(flush_bytes)) != hipSuccess)
         return hip_fail(h, e, "flush buffer");
@@ -1213,40 +1303,10 @@ extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int6
     std::vector<hipEvent_t> ev(3 * reps);
     for (auto& v : ev)
         if ((e = hipEventCreate(&v)) != hipSuccess) return hip_fail(h, e, "event create");
-    const CrsDev A = crs_of(h);
     for (int r = 0; r < reps && rc == PWM_OK; ++r) {
         if (flush_bytes > 0) (void)pwmk::launch_flush(h->flush.p, flush_bytes, h->stream);
         (void)hipEventRecord(ev[3 * r], h->stream);
-        // dominant kernel first, then the rest of the sequence
-        switch (h->format) {
-            case PWM_FORMAT_CRS:
-                if (h->crs_used == 2) {
-                    e = pwmk::launch_tiles_spmv(A, h->tiles.as<pwmk::ETile>(), h->mp, x, y, nullptr, h->stream);
-                    (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                    if (e == hipSuccess && h->tiles_carry) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
-                    break;
-                }
-                e = pwmk::launch_crs_vector(A, h->group, h->vec_blocks, h->lr, x, y, nullptr, h->stream);
-                (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                if (e == hipSuccess && h->lr.n_rows > 0)
-                    e = pwmk::launch_crs_finalize(A, h->vec_blocks, h->lr, y, nullptr, h->stream);
-                break;
-            case PWM_FORMAT_CRS_MERGE:
-                e = pwmk::launch_merge_spmv(A, h->mp, x, y, nullptr, h->stream);
-                (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                if (e == hipSuccess) e = pwmk::launch_merge_fixup(A, h->mp, y, nullptr, h->stream);
-                break;
-            case PWM_FORMAT_CSB:
-                e = csb_launch(h, x, y, nullptr, 1);
-                (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                if (e == hipSuccess) e = csb_launch(h, x, y, nullptr, 2);
-                break;
-            default:
-                rc = spmv_launch(h, x, y, nullptr, nullptr);
-                (void)hipEventRecord(ev[3 * r + 1], h->stream);
-                e = hipSuccess;
-        }
-        if (e != hipSuccess) rc = hip_fail(h, e, "timed SpMV");
+        rc = launch_marked(h, x, y, ev[3 * r + 1]);
         (void)hipEventRecord(ev[3 * r + 2], h->stream);
     }
     e = hipStreamSynchronize(h->stream);
@@ -1264,4 +1324,69 @@ extern "C" int pwm_time_mv(pwm_matrix* h, const void* x, void* y, int reps, int6
     if (ms_per_mv) *ms_per_mv = tot / reps;
     if (kernel_ms) *kernel_ms = ker / reps;
     return PWM_OK;
+}
+
+// SpMVs of `count` handles in turn, reps rounds, all on the first handle's
+// stream: with count independent matrices whose arrays together exceed the
+// 256 MB Infinity Cache, every SpMV reads its matrix from HBM, back to back.
+extern "C" int pwm_time_mv_rotating(pwm_matrix* const* hs, const void* const* xs, void* const* ys, int count,
+                                    int reps, double* ms_per_mv, double* kernel_ms) {
+    if (!hs || !xs || !ys || count <= 0 || reps <= 0)
+        return fail(nullptr, PWM_ERR_INVALID, "pwm_time_mv_rotating: bad argument");
+    for (int k = 0; k < count; ++k) {
+        pwm_matrix* h = hs[k];
+        if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_time_mv_rotating: NULL handle");
+        if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_time_mv_rotating: no matrix loaded");
+        if (!xs[k] || !ys[k]) return fail(h, PWM_ERR_INVALID, "pwm_time_mv_rotating: NULL vector");
+        if (int rc0 = check_device_xy(h, xs[k], h->noc, ys[k], h->nor, "pwm_time_mv_rotating")) return rc0;
+    }
+    pwm_matrix* h0 = hs[0];
+    const hipStream_t s = h0->stream;
+    std::vector<hipStream_t> saved(count);
+    for (int k = 0; k < count; ++k) {
+        saved[k] = hs[k]->stream;
+        hs[k]->stream = s;
+    }
+    const int n = count * reps;
+    std::vector<hipEvent_t> ev(kernel_ms ? 3 * n : 2, nullptr);
+    int rc = PWM_OK;
+    hipError_t e = hipSuccess;
+    for (auto& v : ev)
+        if (e == hipSuccess) e = hipEventCreate(&v);
+    if (e != hipSuccess) rc = hip_fail(h0, e, "event create");
+    if (rc == PWM_OK && !kernel_ms) {
+        (void)hipEventRecord(ev[0], s);
+        for (int i = 0; i < n && rc == PWM_OK; ++i) rc = spmv_launch(hs[i % count], xs[i % count], ys[i % count], nullptr, nullptr);
+        (void)hipEventRecord(ev[1], s);
+    } else if (rc == PWM_OK) {
+        for (int i = 0; i < n && rc == PWM_OK; ++i) {
+            (void)hipEventRecord(ev[3 * i], s);
+            rc = launch_marked(hs[i % count], xs[i % count], ys[i % count], ev[3 * i + 1]);
+            (void)hipEventRecord(ev[3 * i + 2], s);
+        }
+    }
+    e = hipStreamSynchronize(s);
+    if (rc == PWM_OK && e != hipSuccess) rc = hip_fail(h0, e, "timed SpMV");
+    for (int k = count - 1; k >= 0; --k) hs[k]->stream = saved[k];  // reverse order: a handle may repeat
+    if (rc == PWM_OK) {
+        double tot = 0, ker = 0;
+        if (!kernel_ms) {
+            float t = 0;
+            (void)hipEventElapsedTime(&t, ev[0], ev[1]);
+            tot = t;
+        } else {
+            for (int i = 0; i < n; ++i) {
+                float a = 0, b = 0;
+                (void)hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 2]);
+                (void)hipEventElapsedTime(&b, ev[3 * i], ev[3 * i + 1]);
+                tot += a;
+                ker += b;
+            }
+            *kernel_ms = ker / n;
+        }
+        if (ms_per_mv) *ms_per_mv = tot / n;
+    }
+    for (auto& v : ev)
+        if (v) (void)hipEventDestroy(v);
+    return rc;
 }

@@ -131,6 +131,7 @@ extern "C" int pwm_batch_add_crs(pwm_batch* b, int64_t n, const int32_t* rs, con
     I.ci.assign(ci, ci + nnz);
     I.val.assign(vals, vals + nnz);
     b->inst.push_back(std::move(I));
+    b->where.clear();  // results of an earlier solve do not cover the new instance
     b->rows += n;
     b->nnz += nnz;
     b->uploaded = false;
@@ -150,6 +151,7 @@ extern "C" int pwm_batch_add_poisson(pwm_batch* b, int64_t m, int64_t n, int32_t
     b->rows += I.n;
     b->nnz += I.nnz;
     b->inst.push_back(std::move(I));
+    b->where.clear();  // results of an earlier solve do not cover the new instance
     b->uploaded = false;
     return PWM_OK;
 }
@@ -388,7 +390,8 @@ extern "C" int pwm_batch_start_vector(pwm_batch* b, int32_t id, uint64_t seed, d
 // Normalised final iterate of instance `id` after pwm_batch_solve.
 extern "C" int pwm_batch_vector(pwm_batch* b, int32_t id, double* out, double* ynorm) {
     if (!b || !out || id < 0 || id >= (int32_t)b->inst.size()) return bfail(b, PWM_ERR_INVALID, "pwm_batch_vector");
-    if (b->where.empty()) return bfail(b, PWM_ERR_STATE, "pwm_batch_vector: solve first");
+    if (b->where.empty() || id >= (int32_t)b->where.size())
+        return bfail(b, PWM_ERR_STATE, "pwm_batch_vector: solve first");
     const BatchInst& I = b->inst[id];
     const size_t xs = pwm_impl::vector_size(b->vtype);
     const char* src = (const char*)b->arena.at(I.o_vec) + (b->where[id] ? I.n * xs : 0);

@@ -197,6 +197,38 @@ static int add_matrix_common(pwm_server* s, SrvMatrix&& M, int32_t* mat) {
     return PWM_OK;
 }
 
+// Checks a host CRS matrix of n rows with `nnz` entries before it is
+// uploaded: non-decreasing row_start from 0 to nnz, columns in [0, n), and
+// for BF16 storage values exact in BF16.  Runs on the host thread pool.
+static int check_crs(pwm_server* s, int64_t n, int64_t nnz, const int32_t* rs, const int32_t* ci, const double* vals,
+                     const char* who) {
+    if (rs[0] != 0 || rs[n] != nnz) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": bad row_start");
+    std::atomic<int> bad{0};  // 1 row_start, 2 column, 3 BF16 value
+    pwm_impl::parallel_for(n, 1 << 18, [&](int64_t b, int64_t e, int) {
+        for (int64_t r = b; r < e; ++r)
+            if (rs[r + 1] < rs[r]) bad = 1;
+    });
+    if (bad == 0)
+        pwm_impl::parallel_for(nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
+            for (int64_t k = b; k < e; ++k) {
+                if (ci[k] < 0 || ci[k] >= n) {
+                    bad = 2;
+                    return;
+                }
+                if (s->vtype == PWM_VALUES_BF16) {
+                    const float f = (float)vals[k];
+                    uint32_t bits;
+                    memcpy(&bits, &f, 4);
+                    if ((double)f != vals[k] || (bits & 0xffffu)) bad = 3;
+                }
+            }
+        });
+    if (bad == 1) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": row_start must be non-decreasing");
+    if (bad == 2) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": column index out of range");
+    if (bad == 3) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": BF16 storage needs values exact in BF16");
+    return PWM_OK;
+}
+
 // flags bit 0 (PWM_SERVER_BORROW): keep the caller's arrays instead of a copy.
 extern "C" int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* rs, const int32_t* ci, const double* vals,
                                      int32_t flags, int32_t* mat) {
@@ -204,18 +236,8 @@ extern "C" int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* rs
     const int64_t nnz = rs[n];
     if (rs[0] != 0 || nnz < 0 || (nnz > 0 && (!ci || !vals)) || n + nnz > INT32_MAX)
         return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: bad row_start or NULL arrays");
-    for (int64_t r = 0; r < n; ++r)
-        if (rs[r + 1] < rs[r]) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: row_start must be non-decreasing");
-    for (int64_t k = 0; k < nnz; ++k) {
-        if (ci[k] < 0 || ci[k] >= n) return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: column index out of range");
-        if (s->vtype == PWM_VALUES_BF16) {
-            const float f = (float)vals[k];
-            uint32_t bits;
-            memcpy(&bits, &f, 4);
-            if ((double)f != vals[k] || (bits & 0xffffu))
-                return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: BF16 storage needs values exact in BF16");
-        }
-    }
+    int rc = check_crs(s, n, nnz, rs, ci, vals, "pwm_server_add_matrix");
+    if (rc != PWM_OK) return rc;
     SrvMatrix M;
     M.n = n;
     M.nnz = nnz;
@@ -299,6 +321,12 @@ static int make_resident(pwm_server* s, int mi) {
     if (e == hipSuccess && M.kind == 1) {
         e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, s->vtype, s->stream);
     } else if (e == hipSuccess) {
+        if (M.borrowed) {
+            // borrowed arrays may have changed since registration: check them
+            // again before the kernels index x with them
+            const int rc = check_crs(s, M.n, M.nnz, M.rs_p, M.ci_p, M.val_p, "server: borrowed matrix");
+            if (rc != PWM_OK) return rc;
+        }
         e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->stream);
         if (e == hipSuccess) e = pwm_impl::upload_staged(base + M.o_ci, M.ci_p, M.nnz * 4, s->stream);
         if (e == hipSuccess && M.nnz) {

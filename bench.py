@@ -1,35 +1,46 @@
-"""Benchmark: SpMV GB/s on one MI355X (default workload: power-law CRS,
-N = 4 194 304, ~67M nnz, FP64 values, int32 indices).
+"""Benchmark: SpMV and power-method throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload powerlaw|poisson|stencil9|batch|serve]
+                    [--workload all|powerlaw|poisson|stencil9|batch|serve|coo]
 
-batch: 512 uneven SPD power methods stepped together (solves/s against the
-plain FP64 per-matrix GPU loop, eigenvalue error against FP64); serve:
-Poisson-arrival requests on the continuous-batching server (p50/p99 latency
-and solves/s against the FP64 per-request path).  Otherwise:
+The default (`all`) prints ONE JSON line whose top level is the power-law
+SpMV (N = 4 194 304, ~67M nnz, FP64 values, int32 indices) and whose
+"configs" key carries the other workloads measured in the same run:
 
-A step is one SpMV y = A x over the whole matrix with A, x, y resident in HBM.
-W untimed steps, then K timed steps launched back to back (no event or
-marker between them) between two device synchronisations; `value` =
-algorithmic bytes per SpMV / wall time per step.  The algorithmic
-bytes of an FP64/int32 CRS SpMV are 12 Z + 4 (N+1) + 16 N (values and
-column indices once, row offsets once, x and y once).
+- config0_poisson: 2D 5-point Poisson 1024 x 1024 (84 MB per SpMV, small
+  enough for the 256 MB Infinity Cache).  Its roofline figure comes from
+  SpMVs rotating over several independent copies of the matrix and vectors
+  whose arrays together exceed that cache, so every SpMV reads HBM; the rate
+  of repeated SpMVs on one copy is reported beside it as `warm`.
+- config2_stencil9: 9-point stencil 2896^2 + 2 random columns per row
+  (~92M nnz), FP64 and FP32 storage, FP32 error against FP64.
+- config3_batch: 512 uneven SPD power methods stepped together (solves/s
+  against the plain FP64 per-matrix GPU loop, eigenvalue error against FP64).
+- config4_serve: Poisson-arrival requests on the continuous-batching server
+  (p50/p99 latency and solves/s against the FP64 per-request GPU path, the
+  rate the server sustains, and each served result's error against FP64).
 
-All four GPU formats are measured (CRS vector kernel, merge path, CSB,
-BlockCOH); the fastest gives `value`, its dominant kernel's HIP-event time
-gives `roofline.achieved`, and `roofline.traffic` comes from the committed
-rocprof PMC summary for that workload and format.  Also reported: the
-device power method per iteration, and (--values f32, default for stencil9)
-FP32-value runs with their error against FP64.  The CPU baseline is the
-oracle's restatements of the reference's CRSOMP::mv and CRS_Merge::mv timed
-on the host's cores.
+A step of an SpMV workload is one SpMV y = A x over the whole matrix with A,
+x, y resident in HBM: W untimed steps, then K timed steps launched back to
+back (no event or marker between them) between two device synchronisations;
+`value` = algorithmic bytes per SpMV / wall time per step.  The algorithmic
+bytes of an FP64/int32 CRS SpMV are 12 Z + 4 (N+1) + 16 N (values and column
+indices once, row offsets once, x and y once).
+
+All four GPU formats are measured (CRS, merge path, CSB, BlockCOH); the
+fastest gives `value`, its dominant kernel's HIP-event time gives
+`roofline.achieved`, and `roofline.traffic` comes from the committed rocprof
+PMC summary for that workload and format.  The CPU baseline is the oracle's
+restatements of the reference's CRSOMP::mv and CRS_Merge::mv (built at -O3
+like the reference) timed on the host's cores.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -37,7 +48,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MALL_BYTES = 256 << 20  # MI355X Infinity Cache
 METRIC = "SpMV GB/s (% of HBM peak); batched power-method solves/s + p99 latency"
+# host threads of the CPU baseline: the CPU share of one GPU on the GPU box
+CPU_THREADS = 16
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(n, z, value_bytes=8, vector_bytes=8):
@@ -81,47 +99,312 @@ def make_workload(name):
     return n, rs, ci, dv, desc, time.time() - t0
 
 
-def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=6.0, y_gpu=None):
+def _host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=5.0, power_budget_s=6.0, y_gpu=None):
     """The oracle's restatements of CRSOMP::mv (Env_Implementations/CRSOMP.hpp:110-123,
     OpenMP dynamic,512) and CRS_Merge::mv (SOA_Implementations/CRS_Merge.hpp:95-141,
-    P = threads) on `threads` host cores, each bounded to about `budget_s`
-    seconds of full SpMVs of the same matrix, plus the power iteration
-    (CRSOMP.hpp:135-157) per iteration.  `value` is the faster SpMV.  As the
-    checker, the CRSOMP result (per row the sequential CRS::mv order) also
-    gives the relative max-norm error of the GPU's y (`y_gpu`)."""
+    P = threads), built at -O3 like the reference, on `threads` host cores
+    with OMP_PROC_BIND=close.  Protocol of the reference's SpMV driver
+    (driver_spmv_input.cpp:94-107): 50 warm-up calls, then up to 1000 timed
+    calls, median, each leg bounded to about `budget_s` seconds (warm-ups
+    included).  The power iteration (CRSOMP.hpp:135-157) is timed as calls of
+    100 iterations: 3 warm-ups and 10 timed calls as in the reference's
+    Poisson timings, as many as fit `power_budget_s`.  `value` is the faster
+    SpMV.  As the checker, the sequential oracle's CRS::mv result gives the
+    relative max-norm error of the GPU's y (`y_gpu`)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     A = oracle.CRSMatrix(n, n, rs, ci, dv)
+    B = oracle.baseline_lib()
+    xs = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(n)
 
-    def timed(fn):
-        fn()  # warm-up
-        times, t_start = [], time.time()
-        while len(times) < 100 and time.time() - t_start < budget_s:
+    def timed(fn, warm, most, budget):
+        t_start = time.time()
+        t0 = time.perf_counter()
+        fn()
+        one = time.perf_counter() - t0
+        # warm-ups and timed calls within the budget (at least one of each)
+        warm = max(1, min(warm, int(0.2 * budget / max(one, 1e-9))))
+        for _ in range(warm - 1):
+            fn()
+        times = []
+        while len(times) < most and (not times or time.time() - t_start < budget):
             t0 = time.perf_counter()
             fn()
             times.append(time.perf_counter() - t0)
-        return float(np.median(times)), len(times)
+        return float(np.median(times)), warm, len(times)
 
-    t_omp, k_omp = timed(lambda: oracle.crsomp_mv(A, x, threads))
-    t_mrg, k_mrg = timed(lambda: oracle.merge_mv(A, x, threads, threads))
-    # power iteration: calls of `it` iterations (100 for small matrices)
-    it = 100 if n <= (1 << 21) else 10
-    px, py = np.array(x, np.float64), np.zeros(n)
-    L = oracle.lib()
-    t_pow, k_pow = timed(lambda: L.orc_crsomp_power_iteration(threads, n, A.row_start, A.col_ind, A.data, px, py, it))
+    t_omp, w_omp, k_omp = timed(lambda: B.orc_crsomp_mv(threads, n, A.row_start, A.col_ind, A.data, xs, y),
+                                50, 1000, budget_s)
+    t_mrg, w_mrg, k_mrg = timed(lambda: B.orc_merge_mv(threads, threads, n, A.nnz, A.row_start, A.col_ind, A.data,
+                                                        xs, y), 50, 1000, budget_s)
+    px, py = np.array(xs), np.zeros(n)
+    t_pow, w_pow, k_pow = timed(lambda: B.orc_crsomp_power_iteration(threads, n, A.row_start, A.col_ind, A.data,
+                                                                     px, py, 100), 3, 10, power_budget_s)
     best = min(t_omp, t_mrg)
     err = None
     if y_gpu is not None:
-        y_ref = oracle.crsomp_mv(A, x, threads)
+        y_ref = oracle.crs_mv(A, xs)
         err = float(np.max(np.abs(y_gpu - y_ref)) / max(float(np.max(np.abs(y_ref))), 1e-300))
     return {"value": round(bytes_per_mv / best / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
+            "host_cores_visible": _host_cores(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "gpu_y_rel_err_vs_oracle": err,
             "sample": f"median of {k_omp} CRSOMP::mv and {k_mrg} CRS_Merge::mv (P={threads}) full SpMVs of the same "
-                      f"matrix: {t_omp * 1e3:.2f} ms and {t_mrg * 1e3:.2f} ms (oracle restatements, g++ -O2 -fopenmp); "
-                      f"power iteration {t_pow / it * 1e3:.2f} ms/iteration ({k_pow} calls of {it})",
-            "crsomp_ms": round(t_omp * 1e3, 3), "crs_merge_ms": round(t_mrg * 1e3, 3),
-            "power_ms_per_iteration": round(t_pow / it * 1e3, 3)}
+                      f"matrix after {w_omp} / {w_mrg} warm-ups: {t_omp * 1e3:.3f} ms and {t_mrg * 1e3:.3f} ms "
+                      f"(oracle restatements, gcc -O3 -fopenmp, {threads} threads, OMP_PROC_BIND=close); "
+                      f"power iteration {t_pow / 100 * 1e3:.3f} ms/iteration (median of {k_pow} calls of 100 "
+                      f"after {w_pow} warm-ups)",
+            "crsomp_ms": round(t_omp * 1e3, 4), "crs_merge_ms": round(t_mrg * 1e3, 4),
+            "power_ms_per_iteration": round(t_pow / 100 * 1e3, 4)}
 
+
+# ---------------------------------------------------------------- SpMV configs
+
+def bench_spmv(args, workload, cpu=True):
+    """One SpMV workload (configs 0-2): every format, the best one's roofline,
+    the gather probe, the device power method and the CPU baseline."""
+    import thesis_spmv_amd as pw
+    values = args.values or ("f64,f32" if workload == "stencil9" else "f64")
+    n, rs, ci, dv, desc, gen_s = make_workload(workload)
+    z = int(rs[-1])
+    x_host = pw.gen_uniform(n, 0)
+    bytes_mv = algorithmic_bytes(n, z)
+    log(f"# workload: {desc}; nnz={z}; generated in {gen_s:.1f}s; {bytes_mv / 1e6:.1f} MB per SpMV")
+    # a matrix that fits the Infinity Cache is also timed rotating over
+    # enough copies (at least 4) that they overflow it twice over
+    copies = max(4, math.ceil(2 * MALL_BYTES / bytes_mv)) if bytes_mv < MALL_BYTES else 0
+
+    x = pw.DeviceVector.from_array(x_host)
+    y = pw.DeviceVector(n)
+    results = {}
+    wanted = set(args.formats.split(","))
+    csb_opts = {"block_bits": 14} if workload == "stencil9" else {}
+    csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
+    if args.csb_nt >= 0:
+        csb_opts["csb_nt"] = args.csb_nt
+    crs_opts = {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}
+    if args.crs_nt >= 0:
+        crs_opts["crs_nt"] = args.crs_nt
+    formats = (("crs", pw.CRSHip, crs_opts),
+               ("merge_path", pw.CRSMergeHip, {}), ("csb", pw.CSBHip, csb_opts), ("blockcoh", pw.BlockCOHHip, {}))
+    if workload == "stencil9":
+        # the workload names beta = 2^14; the reference's own rule gives 2^15
+        formats += (("csb_ref_beta", pw.CSBHip, {k: v for k, v in csb_opts.items() if k != "block_bits"}),)
+    for name, cls, opts in formats:
+        if name not in wanted:
+            continue
+        t_load = time.time()
+        m = cls(-1, "f64", **opts)
+        m.loadCRS(n, n, rs, ci, dv)
+        t_load = time.time() - t_load
+        for _ in range(args.warmup):
+            m.time_mv(x, y, 1)
+        # per-step events: the SpMV's device time and its dominant kernel
+        ms_ev, kms = m.time_mv(x, y, args.steps)
+        # wall: the same SpMVs back to back, no markers between the steps
+        pw.synchronize()
+        t0 = time.perf_counter()
+        m.time_mv(x, y, args.steps, per_step=False)
+        pw.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
+        rot = None
+        if copies:
+            # the same SpMV rotating over `copies` independent matrices and
+            # vectors: each finds its arrays in HBM, not the Infinity Cache
+            mats, xs, ys = [m], [x], [y]
+            for _ in range(copies - 1):
+                mk = cls(-1, "f64", **opts)
+                mk.loadCRS(n, n, rs, ci, dv)
+                mats.append(mk)
+                xs.append(pw.DeviceVector.from_array(x_host))
+                ys.append(pw.DeviceVector(n))
+            pw.time_mv_rotating(mats, xs, ys, 2)  # warm-up
+            r_ev, r_k = pw.time_mv_rotating(mats, xs, ys, args.steps)
+            pw.synchronize()
+            t0 = time.perf_counter()
+            pw.time_mv_rotating(mats, xs, ys, args.steps, per_step=False)
+            pw.synchronize()
+            r_wall = (time.perf_counter() - t0) * 1e3 / (args.steps * copies)
+            rot = {"copies": copies, "wall_ms": r_wall, "event_ms": r_ev, "kernel_ms": r_k}
+            del mats, xs, ys
+        # the drop-in host API: host x in, host y out (PCIe included; never `value`)
+        yh = np.empty(n)
+        host_ms = []
+        for _ in range(6):
+            th = time.perf_counter()
+            m.mv(x_host, yh)
+            host_ms.append((time.perf_counter() - th) * 1e3)
+        host_ms = float(np.median(host_ms[1:]))
+        inf = m.info()
+        mstats = m.stats()
+        yv = y.download()
+        results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
+                         "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush, "rot": rot,
+                         "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf, "load_s": t_load,
+                         "y_checksum": float(np.sum(np.abs(yv))), "y": yv, "host_ms": host_ms}
+        if rot:
+            results[name]["GBs_hbm"] = bytes_mv / (rot["wall_ms"] * 1e-3) / 1e9
+        log(f"# {name}: wall {wall_ms * 1e3:.1f} us/step, events {ms_ev * 1e3:.1f} us, dominant kernel "
+            f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us, "
+            f"host-array mv {host_ms * 1e3:.0f} us" +
+            (f"; rotating over {copies} copies: wall {rot['wall_ms'] * 1e3:.1f} us, kernel "
+             f"{rot['kernel_ms'] * 1e3:.1f} us ({bytes_mv / (rot['kernel_ms'] * 1e-3) / 1e9:.0f} GB/s)" if rot else ""))
+        del m
+
+    # the format that gives `value`: the highest HBM-resident wall rate (for
+    # a matrix that fits the Infinity Cache, the rotating rate)
+    rate_key = "GBs_hbm" if copies else "GBs"
+    best = max(results, key=lambda k: results[k][rate_key])
+    r = results[best]
+
+    # FP32 values and vectors, FP32 row sums: rate and error against FP64
+    f32 = {}
+    if "f32" in values.split(","):
+        x32 = pw.DeviceVector.from_array(x_host, np.float32)
+        y32 = pw.DeviceVector(n, np.float32)
+        bytes32 = algorithmic_bytes(n, z, 4, 4)
+        for name, cls, opts in formats:
+            if name not in results:
+                continue
+            m = cls(-1, "f32", **opts)
+            m.loadCRS(n, n, rs, ci, dv)
+            for _ in range(args.warmup):
+                m.time_mv(x32, y32, 1)
+            ms_ev, kms = m.time_mv(x32, y32, args.steps)
+            pw.synchronize()
+            t0 = time.perf_counter()
+            m.time_mv(x32, y32, args.steps, per_step=False)
+            pw.synchronize()
+            wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            y64 = results[name]["y"]
+            err = float(np.max(np.abs(y32.download().astype(np.float64) - y64)) / np.max(np.abs(y64)))
+            f32[name] = {"GBs_wall": round(bytes32 / (wall_ms * 1e-3) / 1e9, 1), "us_per_step": round(wall_ms * 1e3, 2),
+                         "dominant_kernel_us": round(kms * 1e3, 2),
+                         "kernel_GBs": round(bytes32 / (kms * 1e-3) / 1e9, 1), "y_rel_err_vs_f64": err}
+            log(f"# {name} f32: wall {wall_ms * 1e3:.1f} us/step, kernel {kms * 1e3:.1f} us "
+                f"({bytes32 / (kms * 1e-3) / 1e9:.0f} GB/s of {bytes32 / 1e6:.1f} MB), y rel.err vs FP64 {err:.2e}")
+            del m
+        del x32, y32
+
+    # gather-rate probe on this workload's x: a 12-byte stream per entry plus
+    # one random x gather each, over all of x, over an eighth of x per XCD (a
+    # CSB column group's window) and over 128 KiB per XCD (L2-resident)
+    gather = {}
+    if n >= 8 and z >= 8192:
+        m_eff = z // 8192 * 8192
+        for label, window, per_xcd in (("stream_only", 0, False), ("all_x", n, False), ("eighth_per_xcd", n // 8, True),
+                                       ("128KiB_per_xcd", min(16384, n // 8), True)):
+            ms = pw.gather_rate(x, window, per_xcd, z)
+            rate = m_eff / (ms * 1e-3)
+            gather[label] = {"window_doubles": int(window), "Ggathers_per_s": round(rate / 1e9, 1),
+                             "us_for_nnz": round(z / rate * 1e6, 1)}
+            if window == 0:
+                gather[label]["stream_GBps"] = round(12 * rate / 1e9, 1)
+                log(f"# gather probe {label}: 12 B/entry stream at {12 * rate / 1e9:.0f} GB/s, "
+                    f"{z / rate * 1e6:.1f} us for nnz entries")
+                continue
+            log(f"# gather probe {label} ({window} doubles{' per XCD' if per_xcd else ''}): "
+                f"{rate / 1e9:.1f} G gathers/s with the 12 B/entry stream, {z / rate * 1e6:.1f} us for nnz entries")
+
+    # power method on the device with the best format: ms per iteration
+    cls, opts = [(c, o) for nm, c, o in formats if nm == best][0]
+    m = cls(-1, "f64", **opts)
+    m.loadCRS(n, n, rs, ci, dv)
+    xp = pw.DeviceVector.from_array(x_host)
+    m.power_method_device(xp, y, 3, -1.0)
+    xp.upload(x_host)
+    pw.synchronize()
+    t0 = time.perf_counter()
+    lam, _, _ = m.power_method_device(xp, y, 100, -1.0)
+    pw.synchronize()
+    power_ms = (time.perf_counter() - t0) * 1e3 / 100
+    log(f"# power method ({best}, FP64, device vectors): {power_ms * 1e3:.1f} us/iteration")
+    del m, xp
+    # the dominant kernel's time, or for a CSB SpMV with column groups the
+    # slab kernel and the group reduction together (HIP events around each
+    # SpMV, the dominant kernel's end marked between them); for a matrix
+    # that fits the Infinity Cache, the rotating (HBM-resident) times
+    two_kernels = best in ("csb", "csb_ref_beta") and r["info"]["group_size"] > 1
+    src = r["rot"] if copies else {"event_ms": r["event_ms"], "kernel_ms": r["kernel_ms"]}
+    t_roof = src["event_ms"] if two_kernels else src["kernel_ms"]
+    achieved = bytes_mv / (t_roof * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{workload}_{best}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    what = "csb slab kernel + column-group reduction" if two_kernels else f"{best} dominant kernel"
+    if copies:
+        what += f", rotating over {copies} copies of A, x, y ({copies * bytes_mv / 1e6:.0f} MB > 256 MB Infinity Cache)"
+    wall_ms = r["rot"]["wall_ms"] if copies else r["ms_per_step"]
+    out = {
+        "metric": METRIC,
+        "value": round(bytes_mv / (wall_ms * 1e-3) / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "format": best, "nnz": z, "rows": n, "index": "int32",
+                   "algorithmic_bytes_per_spmv": bytes_mv,
+                   "pct_of_hbm_peak": round(100 * bytes_mv / (wall_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 1)},
+        "gflops": round(2.0 * z / (wall_ms * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "timed": f"{what}, {round(t_roof * 1e3, 2)} us"},
+    }
+    if copies:
+        out["warm"] = {"note": "the same SpMV repeated on one copy, served from the Infinity Cache",
+                       "GBs_wall": round(r["GBs"], 1), "us_per_step": round(r["ms_per_step"] * 1e3, 2),
+                       "dominant_kernel_us": round(r["kernel_ms"] * 1e3, 2),
+                       "frac_of_hbm_peak": round(bytes_mv / (r["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    fm = {}
+    for k, v in results.items():
+        e = {"GBs_wall": round(v["GBs"], 1), "us_per_step": round(v["ms_per_step"] * 1e3, 2),
+             "device_us": round(v["event_ms"] * 1e3, 2), "dominant_kernel_us": round(v["kernel_ms"] * 1e3, 2),
+             "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
+                                            else round(v["kernel_ms_flushed"] * 1e3, 2)),
+             "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
+             "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
+             "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"],
+             "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"],
+             "host_array_mv_us": round(v["host_ms"] * 1e3, 1)}
+        if k == "crs":
+            e["crs_kernel"] = "tiles" if v["info"]["group_size"] == 0 else "vector"
+        if v["rot"]:
+            e["hbm_rotating"] = {"copies": v["rot"]["copies"], "GBs_wall": round(v["GBs_hbm"], 1),
+                                 "us_per_step": round(v["rot"]["wall_ms"] * 1e3, 2),
+                                 "device_us": round(v["rot"]["event_ms"] * 1e3, 2),
+                                 "dominant_kernel_us": round(v["rot"]["kernel_ms"] * 1e3, 2)}
+        fm[k] = e
+    out["formats"] = fm
+    out["power_method_us_per_iteration"] = round(power_ms * 1e3, 2)
+    out["power_method_lambda_after_100"] = lam
+    if gather:
+        out["gather_probe"] = gather
+    out["matrix_stats"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in mstats.items()}
+    if f32:
+        out["f32"] = f32
+    if cpu and not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, rs, ci, dv, x_host, bytes_mv, CPU_THREADS, y_gpu=r["y"])
+        log(f"# cpu baseline: {out['cpu_baseline']['sample']}")
+    return out
+
+
+# ---------------------------------------------------------------- batch
 
 def bench_batch(args):
     """Batched mixed-precision power method: 512 SPD instances of uneven size
@@ -147,8 +430,7 @@ def bench_batch(args):
     b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk, **opts)
     ids = [b.add_poisson(int(s), int(s)) for s in sides] + [b.add_crs(*m) for m in spd]
     n_inst, rows, nnz = b.size()
-    print(f"# batch: {n_inst} instances, {rows} rows, {nnz} nnz, built in {time.time() - t0:.1f}s",
-          file=sys.stderr, flush=True)
+    log(f"# batch: {n_inst} instances, {rows} rows, {nnz} nnz, built in {time.time() - t0:.1f}s")
     b.solve(5, -1.0, seed=0)  # warm-up (uploads the matrices)
     pw.synchronize()
     t1 = time.perf_counter()
@@ -157,8 +439,8 @@ def bench_batch(args):
     wall = time.perf_counter() - t1
     solves = n_inst / wall
     label = "f32, bf16 where exact" if mixed else args.batch_values
-    print(f"# batched {label}: {wall * 1e3:.1f} ms wall ({ms:.1f} ms device), {solves:.1f} solves/s, "
-          f"iterations min/median/max {its.min()}/{int(np.median(its))}/{its.max()}", file=sys.stderr, flush=True)
+    log(f"# batched {label}: {wall * 1e3:.1f} ms wall ({ms:.1f} ms device), {solves:.1f} solves/s, "
+        f"iterations min/median/max {its.min()}/{int(np.median(its))}/{its.max()}")
     out = {"metric": METRIC, "value": round(solves, 2), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
            "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32+bf16" if mixed else args.batch_values, "data": "synthetic",
@@ -180,8 +462,7 @@ def bench_batch(args):
             [(len(m[0]) - 1, int(m[0][-1]), 2 if mixed and exact16(m[2]) else vb0) for m in spd]
     traffic = sum(int(k) * ((vb + 4) * z + 4 * (n + 1) + 2 * xb * n) for k, (n, z, vb) in zip(its, sizes))
     out["achieved_GBps"] = round(traffic / wall / 1e9, 1)
-    print(f"# algorithmic traffic {traffic / 1e12:.2f} TB over all steps -> {traffic / wall / 1e9:.0f} GB/s",
-          file=sys.stderr, flush=True)
+    log(f"# algorithmic traffic {traffic / 1e12:.2f} TB over all steps -> {traffic / wall / 1e9:.0f} GB/s")
     # one step with every instance active: 128 steps without the stop test
     # (after a first such solve that captures their graphs)
     full = sum((vb + 4) * z + 4 * (n + 1) + 2 * xb * n for n, z, vb in sizes)
@@ -192,24 +473,31 @@ def bench_batch(args):
     pw.synchronize()
     step_ms = (time.perf_counter() - t2) * 1e3 / 128
     out["full_step"] = {"ms": round(step_ms, 3), "GB": round(full / 1e9, 2), "GBps": round(full / step_ms / 1e6, 1)}
-    print(f"# one step with all {n_inst} instances active: {step_ms:.3f} ms for {full / 1e9:.2f} GB "
-          f"({full / step_ms / 1e6:.0f} GB/s)", file=sys.stderr, flush=True)
+    out["roofline"] = {"bound": "hbm", "achieved": round(full / step_ms / 1e6, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(full / step_ms / 1e6 / HBM_PEAK_GBS, 4), "traffic": None,
+                       "timed": "one engine step with all 512 instances active (tile + finish launches), "
+                                f"{step_ms * 1e3:.1f} us"}
+    log(f"# one step with all {n_inst} instances active: {step_ms:.3f} ms for {full / 1e9:.2f} GB "
+        f"({full / step_ms / 1e6:.0f} GB/s)")
     if not args.no_baseline:
         # plain FP64 per-matrix GPU loop, same start vectors
         lam64 = np.zeros(n_inst)
         its64 = np.zeros(n_inst, np.int64)
         t_loop = 0.0
-        for i in range(n_inst):
+
+        def fp64_handle(i):
             m = pw.CRSHip(-1, "f64")
             if i < n_each:
                 m.generatePoissonMatrix(int(sides[i]), int(sides[i]), 1)
-                n = int(sides[i]) ** 2
             else:
                 rs, ci, dv = spd[i - n_each]
-                n = len(rs) - 1
-                m.loadCRS(n, n, rs, ci, dv)
-            x = pw.DeviceVector.from_array(b.start_vector(ids[i], n, seed=0))
-            y = pw.DeviceVector(n)
+                m.loadCRS(len(rs) - 1, len(rs) - 1, rs, ci, dv)
+            return m
+
+        for i in range(n_inst):
+            m = fp64_handle(i)
+            x = pw.DeviceVector.from_array(b.start_vector(ids[i], seed=0))
+            y = pw.DeviceVector(x.n)
             pw.synchronize()
             ts = time.perf_counter()
             lam64[i], _, its64[i] = m.power_method_device(x, y, args.max_it, args.tol)
@@ -221,16 +509,9 @@ def bench_batch(args):
         lam_same = lam64.copy()
         redo = np.nonzero(its64 != its)[0]
         for i in redo:
-            m = pw.CRSHip(-1, "f64")
-            if i < n_each:
-                m.generatePoissonMatrix(int(sides[i]), int(sides[i]), 1)
-                n = int(sides[i]) ** 2
-            else:
-                rs, ci, dv = spd[i - n_each]
-                n = len(rs) - 1
-                m.loadCRS(n, n, rs, ci, dv)
-            x = pw.DeviceVector.from_array(b.start_vector(ids[i], n, seed=0))
-            y = pw.DeviceVector(n)
+            m = fp64_handle(i)
+            x = pw.DeviceVector.from_array(b.start_vector(ids[i], seed=0))
+            y = pw.DeviceVector(x.n)
             lam_same[i], _, _ = m.power_method_device(x, y, int(its[i]), -1.0)
             del m, x, y
         rel = np.abs(lam - lam_same) / np.abs(lam_same)
@@ -239,16 +520,13 @@ def bench_batch(args):
         out["eigenvalue_rel_err_vs_fp64_same_steps"] = {"max": float(rel.max()), "median": float(np.median(rel))}
         out["eigenvalue_rel_err_vs_fp64_loop"] = {"max": float(rel_loop.max()), "median": float(np.median(rel_loop)),
                                                   "instances_stopping_at_other_step": int(len(redo))}
-        print(f"# FP64 per-matrix loop: {t_loop:.2f} s ({n_inst / t_loop:.1f} solves/s); batched speedup "
-              f"{out['speedup_vs_fp64_loop']}x; eigenvalue rel.err at equal steps max {rel.max():.2e}, "
-              f"vs the loop's own stop max {rel_loop.max():.2e} ({len(redo)} stop elsewhere)",
-              file=sys.stderr, flush=True)
-        for i in np.argsort(-rel_loop)[:5]:
-            kind = "poisson" if i < n_each else "spd-powerlaw"
-            print(f"#   worst vs loop: {i} {kind} steps {its[i]} vs {its64[i]} lam {lam[i]:.12g} vs {lam64[i]:.12g} "
-                  f"(equal-step err {rel[i]:.2e})", file=sys.stderr, flush=True)
-    print(json.dumps(out), flush=True)
+        log(f"# FP64 per-matrix loop: {t_loop:.2f} s ({n_inst / t_loop:.1f} solves/s); batched speedup "
+            f"{out['speedup_vs_fp64_loop']}x; eigenvalue rel.err at equal steps max {rel.max():.2e}, "
+            f"vs the loop's own stop max {rel_loop.max():.2e} ({len(redo)} stop elsewhere)")
+    return out
 
+
+# ---------------------------------------------------------------- serve
 
 def _serve_requests(rng, n_req, rate, n_pool):
     """Poisson arrivals at `rate` req/s; each request is an SpMV chain
@@ -270,6 +548,187 @@ def _percentiles(lat):
     return {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
 
+def bench_serve(args):
+    """Served stream: requests arrive as a Poisson process and are served by
+    the continuous-batching server (FP32 storage, FP64 reductions, 64 GB
+    admission budget) and, for comparison, by the plain FP64 per-request GPU
+    path (one CRS handle per matrix, requests one after another).  The
+    capacity the server sustains is its completion rate at the highest
+    arrival rates; below it latency stays bounded.  Each served result of one
+    rate's requests is also compared with the FP64 path at the same number
+    of steps (results do not depend on what else is in flight, so they come
+    from an untimed replay)."""
+    import thesis_spmv_amd as pw
+    rng = np.random.default_rng(5)
+    n_pool = args.serve_pool
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(args.serve_max_n), n_pool)).astype(np.int64)
+    kinds = ["poisson", "powerlaw", "stencil9"]
+    pool = []
+    t0 = time.time()
+    for i, n in enumerate(ns):
+        kind = kinds[i % 3]
+        if kind == "poisson":
+            side = max(2, int(np.sqrt(n)))
+            pool.append({"kind": kind, "side": side, "n": side * side})
+        elif kind == "powerlaw":
+            rs, ci, dv = pw.gen_powerlaw(int(n), alpha=1.8, xm=7.3333, cap=50000, struct_seed=100 + i,
+                                         value_seed=200 + i)
+            pool.append({"kind": kind, "n": int(n), "crs": (rs, ci, dv)})
+        else:
+            side = max(3, int(np.sqrt(n)))
+            rs, ci, dv = pw.gen_stencil9(side, 2, 300 + i)
+            pool.append({"kind": kind, "n": side * side, "crs": (rs, ci, dv)})
+    log(f"# serve pool: {n_pool} matrices, N {min(p['n'] for p in pool)}..{max(p['n'] for p in pool)}, "
+        f"built in {time.time() - t0:.1f}s")
+
+    def register(srv, p):
+        return srv.add_poisson(p["side"], p["side"]) if p["kind"] == "poisson" else \
+            srv.add_matrix(*p["crs"], borrow=True)
+
+    def run_server(reqs):
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
+        resident_ids = {}  # pool index -> list of server matrix ids
+        lat, jobs, pending, k = [], {}, list(reqs), 0
+        start = time.perf_counter()
+        while k < len(pending) or jobs:
+            now = time.perf_counter() - start
+            while k < len(pending) and pending[k]["t"] <= now:
+                r = pending[k]
+                p = pool[r["mat"]]
+                ids = [m for m in resident_ids.get(r["mat"], []) if srv.resident(m)]
+                if r["repeat"] and ids:
+                    mid = ids[0]
+                else:
+                    mid = register(srv, p)
+                    resident_ids.setdefault(r["mat"], []).append(mid)
+                jobs[srv.submit(mid, r["steps"], r["tol"], seed=r["seed"])] = r
+                k += 1
+            if not jobs:
+                time.sleep(max(0.0, pending[k]["t"] - (time.perf_counter() - start)))
+                continue
+            srv.step()
+            done_at = time.perf_counter() - start
+            for j in srv.finished():
+                r = jobs.pop(j)
+                srv.result(j)
+                lat.append(done_at - r["t"])
+        makespan = time.perf_counter() - start
+        tm = srv.timing()
+        log(f"#   server phases: admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), run {tm['run_s']:.2f}s, "
+            f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s")
+        del srv
+        return lat, makespan
+
+    def fp64_handle(p):
+        m = pw.CRSHip(-1, "f64")
+        if p["kind"] == "poisson":
+            m.generatePoissonMatrix(p["side"], p["side"], 1)
+        else:
+            rs, ci, dv = p["crs"]
+            m.loadCRS(len(rs) - 1, len(rs) - 1, rs, ci, dv)
+        return m
+
+    def run_fp64(reqs):
+        cache, lat = {}, []
+        start = time.perf_counter()
+        for r in reqs:
+            now = time.perf_counter() - start
+            if now < r["t"]:
+                time.sleep(r["t"] - now)
+            p = pool[r["mat"]]
+            m = cache.get(r["mat"]) if r["repeat"] else None
+            if m is None:
+                m = fp64_handle(p)
+                cache[r["mat"]] = m
+            n = p["n"]
+            x = pw.DeviceVector.from_array(pw.server_start_vector(n, r["seed"]))
+            y = pw.DeviceVector(n)
+            m.power_method_device(x, y, r["steps"], r["tol"])
+            lat.append(time.perf_counter() - start - r["t"])
+            del x, y
+        return lat, time.perf_counter() - start
+
+    def result_errors(reqs):
+        """Every request's served result (untimed replay, all submitted at
+        once) against the FP64 path at the same number of steps: eigenvalue
+        error over max(|lambda_64|, |A x|_64) (the scale of the quotient's
+        rounding errors; lambda itself can sit near 0 on the nonsymmetric
+        power-law matrices), relative where lambda_64 is the dominant scale,
+        and the max-norm difference of the normalised final vectors."""
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
+        mids = {}
+        jobs = {}
+        for r in reqs:
+            if r["mat"] not in mids:
+                mids[r["mat"]] = register(srv, pool[r["mat"]])
+            jobs[srv.submit(mids[r["mat"]], r["steps"], r["tol"], seed=r["seed"])] = r
+        res = {}
+        while len(res) < len(jobs):
+            srv.step()
+            for j in srv.finished():
+                res[j] = srv.result(j, pool[jobs[j]["mat"]]["n"])
+        del srv
+        lam_err, vec_err, rel_err = [], [], []
+        cache = {}
+        for j, r in jobs.items():
+            lam, yn, its, vec = res[j]
+            p = pool[r["mat"]]
+            m = cache.get(r["mat"]) or cache.setdefault(r["mat"], fp64_handle(p))
+            x = pw.DeviceVector.from_array(pw.server_start_vector(p["n"], r["seed"]))
+            y = pw.DeviceVector(p["n"])
+            lam64, yn64, its64 = m.power_method_device(x, y, its, -1.0)
+            v64 = (x if its64 % 2 == 0 else y).download()
+            lam_err.append(abs(lam - lam64) / max(abs(lam64), yn64))
+            rel_err.append(abs(lam - lam64) / abs(lam64))
+            vec_err.append(float(np.max(np.abs(vec - v64))))
+            del x, y
+        lam_err, vec_err, rel_err = map(np.asarray, (lam_err, vec_err, rel_err))
+        return {"results": len(lam_err),
+                "lambda_err_scaled": {"max": float(lam_err.max()), "median": float(np.median(lam_err))},
+                "lambda_rel_err": {"max": float(rel_err.max()), "median": float(np.median(rel_err)),
+                                   "p90": float(np.percentile(rel_err, 90))},
+                "vector_maxnorm_err": {"max": float(vec_err.max()), "median": float(np.median(vec_err))}}
+
+    rows = []
+    for rate in args.serve_rates:
+        reqs = _serve_requests(np.random.default_rng(int(rate)), args.serve_requests, rate, n_pool)
+        lat, span = run_server(reqs)
+        row = {"rate": rate, "requests": len(reqs), "server": {"solves_per_s": round(len(lat) / span, 2),
+                                                               **_percentiles(lat)}}
+        log(f"# rate {rate}/s: server {row['server']}")
+        if not args.no_baseline:
+            lat64, span64 = run_fp64(reqs)
+            row["fp64_per_request"] = {"solves_per_s": round(len(lat64) / span64, 2), **_percentiles(lat64)}
+            log(f"#   fp64 per-request: {row['fp64_per_request']}")
+        rows.append(row)
+    capacity = max(r["server"]["solves_per_s"] for r in rows)
+    for row in rows:
+        row["above_capacity"] = bool(row["rate"] > capacity)
+    top = rows[-1]
+    out = {"metric": METRIC, "value": top["server"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+           "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"served stream: Poisson arrivals, 75% SpMV chains (1-64) / 25% power solves "
+                                  f"(tol 1e-7, <=1000 steps), 20% resident repeats, 64 GB budget, pool of {n_pool} "
+                                  f"matrices N log-U[1e4,{args.serve_max_n:.0e}]",
+                      "rates": args.serve_rates, "requests_per_rate": args.serve_requests},
+           "p50_ms": top["server"]["p50_ms"], "p99_ms": top["server"]["p99_ms"],
+           "capacity_solves_per_s": capacity,
+           "capacity_note": "the server's completion rate at saturation (the highest over the swept rates); "
+                            "rates below it keep p99 bounded by the longest job",
+           "rates": rows}
+    if not args.no_baseline:
+        out["speedup_vs_fp64_at_top_rate"] = round(top["server"]["solves_per_s"] /
+                                                   top["fp64_per_request"]["solves_per_s"], 2)
+        err_rate = args.serve_rates[0]
+        reqs = _serve_requests(np.random.default_rng(int(err_rate)), args.serve_requests, err_rate, n_pool)
+        out["error_vs_fp64"] = {"rate": err_rate, **result_errors(reqs)}
+        log(f"# served results vs FP64 at equal steps (rate {err_rate}/s requests): {out['error_vs_fp64']}")
+    return out
+
+
+# ---------------------------------------------------------------- COO -> CRS
+
 def bench_coo(args):
     """COO -> CRS (TripletToCRS, Util/TripletToCRS.hpp:135-172) of the
     power-law matrix (N=4194304, ~67M entries) given as shuffled triplets:
@@ -285,7 +744,7 @@ def bench_coo(args):
     perm = np.random.default_rng(5).permutation(z)
     t = pw.Triplet(n, n, rows[perm], ci[perm], dv[perm])
     del rows, perm
-    print(f"# coo: {desc}; {z} shuffled triplets built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    log(f"# coo: {desc}; {z} shuffled triplets built in {time.time() - t0:.1f}s")
     small = pw.Triplet(1000, 1000, np.arange(1000), np.arange(1000), np.ones(1000))
     pw.CRSMergeHip().loadFromTriplets(small)  # warm-up: loads the kernels
     times = []
@@ -308,8 +767,8 @@ def bench_coo(args):
     ref = oracle.triplet_to_crs(ot)
     cpu_s = time.perf_counter() - t1
     exact = bool(np.array_equal(grs, ref.row_start) and np.array_equal(gci, ref.col_ind) and np.array_equal(gdv, ref.data))
-    print(f"# coo: loadFromTriplets {gpu_s * 1e3:.1f} ms (median of {len(times)}), {z / gpu_s / 1e6:.0f} M entries/s; "
-          f"oracle TripletToCRS {cpu_s * 1e3:.0f} ms; arrays bit-exact: {exact}", file=sys.stderr, flush=True)
+    log(f"# coo: loadFromTriplets {gpu_s * 1e3:.1f} ms (median of {len(times)}), {z / gpu_s / 1e6:.0f} M entries/s; "
+        f"oracle TripletToCRS {cpu_s * 1e3:.0f} ms; arrays bit-exact: {exact}")
     out = {"metric": METRIC, "value": round(z / gpu_s / 1e6, 1), "unit": "M entries/s", "n_gpus": 1,
            "steps": len(times), "warmup": 1, "ms_per_step": round(gpu_s * 1e3, 2), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int32 keys, f64 values", "data": "synthetic",
@@ -318,121 +777,9 @@ def bench_coo(args):
            "cpu_baseline": {"value": round(z / cpu_s / 1e6, 1), "unit": "M entries/s", "cores": 1, "kind": "port",
                             "sample": f"the oracle's TripletToCRS restatement on all {z} triplets, one call, "
                                       f"{cpu_s * 1e3:.0f} ms"}}
-    print(json.dumps(out), flush=True)
     if not exact:
-        sys.exit(1)
-
-
-def bench_serve(args):
-    """Served stream: requests arrive as a Poisson process and are served by
-    the continuous-batching server (FP32 storage, FP64 reductions, 64 GB
-    admission budget) and, for comparison, by the plain FP64 per-request GPU
-    path (one CRS handle per matrix, requests one after another)."""
-    import thesis_spmv_amd as pw
-    rng = np.random.default_rng(5)
-    n_pool = args.serve_pool
-    ns = np.exp(rng.uniform(np.log(1e4), np.log(args.serve_max_n), n_pool)).astype(np.int64)
-    kinds = ["poisson", "powerlaw", "stencil9"]
-    pool = []
-    t0 = time.time()
-    for i, n in enumerate(ns):
-        kind = kinds[i % 3]
-        if kind == "poisson":
-            side = max(2, int(np.sqrt(n)))
-            pool.append({"kind": kind, "side": side, "n": side * side})
-        elif kind == "powerlaw":
-            rs, ci, dv = pw.gen_powerlaw(int(n), alpha=1.8, xm=7.3333, cap=50000, struct_seed=100 + i,
-                                         value_seed=200 + i)
-            pool.append({"kind": kind, "n": int(n), "crs": (rs, ci, dv)})
-        else:
-            side = max(3, int(np.sqrt(n)))
-            rs, ci, dv = pw.gen_stencil9(side, 2, 300 + i)
-            pool.append({"kind": kind, "n": side * side, "crs": (rs, ci, dv)})
-    print(f"# serve pool: {n_pool} matrices, N {min(p['n'] for p in pool)}..{max(p['n'] for p in pool)}, "
-          f"built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-
-    def run_server(reqs):
-        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
-        resident_ids = {}  # pool index -> list of server matrix ids
-        lat, jobs, pending, k = [], {}, list(reqs), 0
-        start = time.perf_counter()
-        while k < len(pending) or jobs:
-            now = time.perf_counter() - start
-            while k < len(pending) and pending[k]["t"] <= now:
-                r = pending[k]
-                p = pool[r["mat"]]
-                ids = [m for m in resident_ids.get(r["mat"], []) if srv.resident(m)]
-                if r["repeat"] and ids:
-                    mid = ids[0]
-                else:
-                    mid = srv.add_poisson(p["side"], p["side"]) if p["kind"] == "poisson" else \
-                        srv.add_matrix(*p["crs"], borrow=True)
-                    resident_ids.setdefault(r["mat"], []).append(mid)
-                jobs[srv.submit(mid, r["steps"], r["tol"], seed=r["seed"])] = r
-                k += 1
-            if not jobs:
-                time.sleep(max(0.0, pending[k]["t"] - (time.perf_counter() - start)))
-                continue
-            srv.step()
-            done_at = time.perf_counter() - start
-            for j in srv.finished():
-                r = jobs.pop(j)
-                srv.result(j)
-                lat.append(done_at - r["t"])
-        makespan = time.perf_counter() - start
-        tm = srv.timing()
-        print(f"#   server phases: admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), run {tm['run_s']:.2f}s, "
-              f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s", file=sys.stderr, flush=True)
-        del srv
-        return lat, makespan
-
-    def run_fp64(reqs):
-        cache, lat = {}, []
-        start = time.perf_counter()
-        for r in reqs:
-            now = time.perf_counter() - start
-            if now < r["t"]:
-                time.sleep(r["t"] - now)
-            p = pool[r["mat"]]
-            m = cache.get(r["mat"]) if r["repeat"] else None
-            if m is None:
-                m = pw.CRSHip(-1, "f64")
-                if p["kind"] == "poisson":
-                    m.generatePoissonMatrix(p["side"], p["side"], 1)
-                else:
-                    rs, ci, dv = p["crs"]
-                    m.loadCRS(len(rs) - 1, len(rs) - 1, rs, ci, dv)
-                cache[r["mat"]] = m
-            n = p["n"]
-            x = pw.DeviceVector.from_array(pw.server_start_vector(n, r["seed"]))
-            y = pw.DeviceVector(n)
-            m.power_method_device(x, y, r["steps"], r["tol"])
-            lat.append(time.perf_counter() - start - r["t"])
-            del x, y
-        return lat, time.perf_counter() - start
-
-    rows = []
-    for rate in args.serve_rates:
-        reqs = _serve_requests(np.random.default_rng(int(rate)), args.serve_requests, rate, n_pool)
-        lat, span = run_server(reqs)
-        row = {"rate": rate, "requests": len(reqs), "server": {"solves_per_s": round(len(lat) / span, 2),
-                                                               **_percentiles(lat)}}
-        print(f"# rate {rate}/s: server {row['server']}", file=sys.stderr, flush=True)
-        if not args.no_baseline:
-            lat64, span64 = run_fp64(reqs)
-            row["fp64_per_request"] = {"solves_per_s": round(len(lat64) / span64, 2), **_percentiles(lat64)}
-            print(f"#   fp64 per-request: {row['fp64_per_request']}", file=sys.stderr, flush=True)
-        rows.append(row)
-    top = rows[-1]
-    out = {"metric": METRIC, "value": top["server"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
-           "steps": len(rows), "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"served stream: Poisson arrivals, 75% SpMV chains (1-64) / 25% power solves "
-                                  f"(tol 1e-7, <=1000 steps), 20% resident repeats, 64 GB budget, pool of {n_pool} "
-                                  f"matrices N log-U[1e4,{args.serve_max_n:.0e}]",
-                      "rates": args.serve_rates, "requests_per_rate": args.serve_requests},
-           "p50_ms": top["server"]["p50_ms"], "p99_ms": top["server"]["p99_ms"], "rates": rows}
-    print(json.dumps(out), flush=True)
+        raise RuntimeError("COO -> CRS arrays differ from the oracle's")
+    return out
 
 
 def main():
@@ -440,7 +787,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="powerlaw")
+    ap.add_argument("--workload", default="all",
+                    help="all (power-law line + configs 0, 2, 3, 4) or one of powerlaw, poisson, stencil9, batch, "
+                         "serve, coo")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
     ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh,csb_ref_beta")
@@ -465,8 +814,8 @@ def main():
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-baseline", action="store_true")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[50.0, 200.0, 1000.0, 2000.0])
-    ap.add_argument("--serve-requests", type=int, default=120, help="requests per arrival rate")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[10.0, 20.0, 50.0, 200.0, 1000.0, 2000.0])
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests per arrival rate")
     ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
     ap.add_argument("--serve-max-n", type=float, default=1e7)
     ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per pwm_server_step")
@@ -475,211 +824,46 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "sharding the rows over several GPUs is designed, not built", file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline's OpenMP threads stay on neighbouring cores; libgomp
+    # reads this when the oracle library is first loaded
+    os.environ.setdefault("OMP_PROC_BIND", "close")
 
     import thesis_spmv_amd as pw
-    if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "serve":
-        return bench_serve(args)
-    if args.workload == "coo":
-        return bench_coo(args)
     if args.calibrate_copy > 0:
         nb = args.calibrate_copy // 16 * 16
         src, dst = pw.DeviceVector(nb // 8), pw.DeviceVector(nb // 8)
         src.fill(1.0)
         pw._check(pw.lib().pwm_device_copy(dst.ptr, src.ptr, nb))
-        print(f"# calibration copy: {nb} bytes read and written once", file=sys.stderr, flush=True)
+        log(f"# calibration copy: {nb} bytes read and written once")
         del src, dst
-    if args.values is None:
-        args.values = "f64,f32" if args.workload == "stencil9" else "f64"
-    n, rs, ci, dv, desc, gen_s = make_workload(args.workload)
-    z = int(rs[-1])
-    x_host = pw.gen_uniform(n, 0)
-    bytes_mv = algorithmic_bytes(n, z)
-    print(f"# workload: {desc}; nnz={z}; generated in {gen_s:.1f}s; {bytes_mv / 1e6:.1f} MB per SpMV",
-          file=sys.stderr, flush=True)
-
-    x = pw.DeviceVector.from_array(x_host)
-    y = pw.DeviceVector(n)
-    results = {}
-    wanted = set(args.formats.split(","))
-    csb_opts = {"block_bits": 14} if args.workload == "stencil9" else {}
-    csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
-    if args.csb_nt >= 0:
-        csb_opts["csb_nt"] = args.csb_nt
-    crs_opts = {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}
-    if args.crs_nt >= 0:
-        crs_opts["crs_nt"] = args.crs_nt
-    formats = (("crs", pw.CRSHip, crs_opts),
-               ("merge_path", pw.CRSMergeHip, {}), ("csb", pw.CSBHip, csb_opts), ("blockcoh", pw.BlockCOHHip, {}))
-    if args.workload == "stencil9":
-        # the workload names beta = 2^14; the reference's own rule gives 2^15
-        formats += (("csb_ref_beta", pw.CSBHip, {k: v for k, v in csb_opts.items() if k != "block_bits"}),)
-    for name, cls, opts in formats:
-        if name not in wanted:
-            continue
-        t_load = time.time()
-        m = cls(-1, "f64", **opts)
-        m.loadCRS(n, n, rs, ci, dv)
-        t_load = time.time() - t_load
-        for _ in range(args.warmup):
-            m.time_mv(x, y, 1)
-        # per-step events: the SpMV's device time and its dominant kernel
-        ms_ev, kms = m.time_mv(x, y, args.steps)
-        # wall: the same SpMVs back to back, no markers between the steps
-        pw.synchronize()
-        t0 = time.perf_counter()
-        m.time_mv(x, y, args.steps, per_step=False)
-        pw.synchronize()
-        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        ms_flush, kms_flush = (float("nan"), float("nan")) if args.no_flush else m.time_mv(x, y, 10, flush_bytes=512 << 20)
-        # the drop-in host API: host x in, host y out (PCIe included; never `value`)
-        yh = np.empty(n)
-        host_ms = []
-        for _ in range(6):
-            th = time.perf_counter()
-            m.mv(x_host, yh)
-            host_ms.append((time.perf_counter() - th) * 1e3)
-        host_ms = float(np.median(host_ms[1:]))
-        inf = m.info()
-        mstats = m.stats()
-        # parity spot check of this run's result against the first format's
-        yv = y.download()
-        results[name] = {"ms_per_step": wall_ms, "event_ms": ms_ev, "kernel_ms": kms,
-                         "kernel_ms_flushed": kms_flush, "event_ms_flushed": ms_flush,
-                         "GBs": bytes_mv / (wall_ms * 1e-3) / 1e9, "info": inf, "load_s": t_load,
-                         "y_checksum": float(np.sum(np.abs(yv))), "y": yv, "host_ms": host_ms}
-        print(f"# {name}: wall {wall_ms * 1e3:.1f} us/step, events {ms_ev * 1e3:.1f} us, dominant kernel "
-              f"{kms * 1e3:.1f} us ({bytes_mv / (kms * 1e-3) / 1e9:.0f} GB/s), flushed {kms_flush * 1e3:.1f} us, "
-              f"host-array mv {host_ms * 1e3:.0f} us",
-              file=sys.stderr, flush=True)
-        del m
-
-    best = max(results, key=lambda k: results[k]["GBs"])
-    r = results[best]
-
-    # FP32 values and vectors, FP32 row sums: rate and error against FP64
-    f32 = {}
-    if "f32" in args.values.split(","):
-        x32 = pw.DeviceVector.from_array(x_host, np.float32)
-        y32 = pw.DeviceVector(n, np.float32)
-        bytes32 = algorithmic_bytes(n, z, 4, 4)
-        for name, cls, opts in formats:
-            if name not in results:
-                continue
-            m = cls(-1, "f32", **opts)
-            m.loadCRS(n, n, rs, ci, dv)
-            for _ in range(args.warmup):
-                m.time_mv(x32, y32, 1)
-            pw.synchronize()
-            t0 = time.perf_counter()
-            ms_ev, kms = m.time_mv(x32, y32, args.steps)
-            pw.synchronize()
-            wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-            y64 = results[name]["y"]
-            err = float(np.max(np.abs(y32.download().astype(np.float64) - y64)) / np.max(np.abs(y64)))
-            f32[name] = {"GBs_wall": round(bytes32 / (wall_ms * 1e-3) / 1e9, 1), "us_per_step": round(wall_ms * 1e3, 2),
-                         "dominant_kernel_us": round(kms * 1e3, 2),
-                         "kernel_GBs": round(bytes32 / (kms * 1e-3) / 1e9, 1), "y_rel_err_vs_f64": err}
-            print(f"# {name} f32: wall {wall_ms * 1e3:.1f} us/step, kernel {kms * 1e3:.1f} us "
-                  f"({bytes32 / (kms * 1e-3) / 1e9:.0f} GB/s of {bytes32 / 1e6:.1f} MB), y rel.err vs FP64 {err:.2e}",
-                  file=sys.stderr, flush=True)
-            del m
-        del x32, y32
-
-    # gather-rate probe on this workload's x: a 12-byte stream per entry plus
-    # one random x gather each, over all of x, over an eighth of x per XCD (a
-    # CSB column group's window) and over 128 KiB per XCD (L2-resident)
-    gather = {}
-    if n >= 8 and z >= 8192:
-        m_eff = z // 8192 * 8192
-        for label, window, per_xcd in (("stream_only", 0, False), ("all_x", n, False), ("eighth_per_xcd", n // 8, True),
-                                       ("128KiB_per_xcd", min(16384, n // 8), True)):
-            ms = pw.gather_rate(x, window, per_xcd, z)
-            rate = m_eff / (ms * 1e-3)
-            gather[label] = {"window_doubles": int(window), "Ggathers_per_s": round(rate / 1e9, 1),
-                             "us_for_nnz": round(z / rate * 1e6, 1)}
-            if window == 0:
-                gather[label]["stream_GBps"] = round(12 * rate / 1e9, 1)
-                print(f"# gather probe {label}: 12 B/entry stream at {12 * rate / 1e9:.0f} GB/s, "
-                      f"{z / rate * 1e6:.1f} us for nnz entries", file=sys.stderr, flush=True)
-                continue
-            print(f"# gather probe {label} ({window} doubles{' per XCD' if per_xcd else ''}): "
-                  f"{rate / 1e9:.1f} G gathers/s with the 12 B/entry stream, {z / rate * 1e6:.1f} us for nnz entries",
-                  file=sys.stderr, flush=True)
-
-    # power method on the device with the best format: ms per iteration
-    cls, opts = [(c, o) for nm, c, o in formats if nm == best][0]
-    m = cls(-1, "f64", **opts)
-    m.loadCRS(n, n, rs, ci, dv)
-    xp = pw.DeviceVector.from_array(x_host)
-    m.power_method_device(xp, y, 3, -1.0)
-    xp.upload(x_host)
-    pw.synchronize()
-    t0 = time.perf_counter()
-    m.power_method_device(xp, y, 100, -1.0)
-    pw.synchronize()
-    power_ms = (time.perf_counter() - t0) * 1e3 / 100
-    print(f"# power method ({best}, FP64, device vectors): {power_ms * 1e3:.1f} us/iteration", file=sys.stderr,
-          flush=True)
-    del m, xp
-    # the dominant kernel's time, or for a CSB SpMV with column groups the
-    # slab kernel and the group reduction together (HIP events around each
-    # SpMV, the dominant kernel's end marked between them)
-    two_kernels = best == "csb" and r["info"]["group_size"] > 1
-    t_roof = r["event_ms"] if two_kernels else r["kernel_ms"]
-    achieved = bytes_mv / (t_roof * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{best}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    out = {
-        "metric": METRIC,
-        "value": round(r["GBs"], 2),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r["ms_per_step"], 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": desc, "format": best, "nnz": z, "rows": n, "index": "int32",
-                   "algorithmic_bytes_per_spmv": bytes_mv, "pct_of_hbm_peak": round(100 * r["GBs"] / HBM_PEAK_GBS, 1)},
-        "gflops": round(2.0 * z / (r["ms_per_step"] * 1e-3) / 1e9, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "timed": (f"csb slab kernel + column-group reduction, {round(t_roof * 1e3, 2)} us" if two_kernels
-                               else f"{best} dominant kernel, {round(t_roof * 1e3, 2)} us")},
-        "formats": {k: {"GBs_wall": round(v["GBs"], 1), "us_per_step": round(v["ms_per_step"] * 1e3, 2),
-                        "device_us": round(v["event_ms"] * 1e3, 2),
-                        "dominant_kernel_us": round(v["kernel_ms"] * 1e3, 2),
-                        "dominant_kernel_us_flushed": (None if v["kernel_ms_flushed"] != v["kernel_ms_flushed"]
-                                                       else round(v["kernel_ms_flushed"] * 1e3, 2)),
-                        "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
-                        **({"crs_kernel": "tiles" if v["info"]["group_size"] == 0 else "vector"} if k == "crs" else {}),
-                        "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
-                        "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"],
-                        "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"],
-                        "host_array_mv_us": round(v["host_ms"] * 1e3, 1)}
-                    for k, v in results.items()},
-        "power_method_us_per_iteration": round(power_ms * 1e3, 2),
-    }
-    if gather:
-        out["gather_probe"] = gather
-    out["matrix_stats"] = {k: (round(v, 6) if isinstance(v, float) else v) for k, v in mstats.items()}
-    if f32:
-        out["f32"] = f32
-    if not args.no_cpu:
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
-        out["cpu_baseline"] = cpu_baseline(n, rs, ci, dv, x_host, bytes_mv, min(16, cores), y_gpu=r["y"])
+    single = {"batch": bench_batch, "serve": bench_serve, "coo": bench_coo}
+    if args.workload in single:
+        out = single[args.workload](args)
+    elif args.workload in ("powerlaw", "poisson", "stencil9"):
+        out = bench_spmv(args, args.workload)
+    elif args.workload == "all":
+        t_all = time.time()
+        out = bench_spmv(args, "powerlaw")
+        extra = {}
+        for key, fn in (("config0_poisson", lambda: bench_spmv(args, "poisson")),
+                        ("config2_stencil9", lambda: bench_spmv(args, "stencil9")),
+                        ("config3_batch", lambda: bench_batch(args)),
+                        ("config4_serve", lambda: bench_serve(args))):
+            t0 = time.time()
+            log(f"## {key}")
+            try:
+                res = fn()
+                for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+                    res.pop(k, None)
+                extra[key] = res
+            except Exception as e:  # one failing extra config must not hide the others
+                traceback.print_exc()
+                extra[key] = {"error": repr(e)}
+            extra[key]["bench_seconds"] = round(time.time() - t0, 1)
+        out["configs"] = extra
+        out["bench_seconds"] = round(time.time() - t_all, 1)
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
     print(json.dumps(out), flush=True)
 
 

@@ -15,7 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_BASELINE_PATH = os.path.join(_HERE, "liboracle_baseline.so")
 _lib = None
+_baseline = None
 
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
@@ -31,60 +33,74 @@ def build():
 
 
 def lib():
+    """The checker (-O2, no FMA)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            build()
-        L = C.CDLL(_LIB_PATH)
-        sig = {
-            "orc_integer_ceil": (I64, [I64, I64]),
-            "orc_binary_search_leftmost": (I64, [i32p, C.c_int32, I64]),
-            "orc_norm2": (C.c_double, [f64p, I64]),
-            "orc_normalize": (None, [f64p, I64]),
-            "orc_poisson_nnz": (I64, [I64, I64]),
-            "orc_fill_poisson": (None, [I64, I64, i32p, i32p, f64p]),
-            "orc_triplet_poisson": (None, [I64, I64, i32p, i32p, f64p]),
-            "orc_load_mm": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64),
-                                      C.POINTER(I64), C.POINTER(C.POINTER(C.c_int32)),
-                                      C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double))]),
-            "orc_load_bin": (C.c_int, [C.c_char_p, I64, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64),
-                                       C.POINTER(I64), C.POINTER(C.POINTER(C.c_int32)),
-                                       C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double))]),
-            "orc_free": (None, [C.c_void_p]),
-            "orc_mt19937_seed": (None, [u32p, C.c_uint32]),
-            "orc_mt19937_next": (C.c_uint32, [u32p]),
-            "orc_uniform_real": (C.c_double, [u32p, C.c_double, C.c_double]),
-            "orc_triplet_to_crs": (None, [I64, I64, i32p, i32p, f64p, i32p, i32p, f64p]),
-            "orc_row_partitions": (None, [I64, C.c_int, i32p, i32p]),
-            "orc_crs_mv": (None, [I64, i32p, i32p, f64p, f64p, f64p]),
-            "orc_crs_power_iteration": (None, [I64, i32p, i32p, f64p, f64p, f64p, I64]),
-            "orc_crsomp_mv": (None, [C.c_int, I64, i32p, i32p, f64p, f64p, f64p]),
-            "orc_crsomp_power_iteration": (None, [C.c_int, I64, i32p, i32p, f64p, f64p, f64p, I64]),
-            "orc_merge_search": (None, [I64, i32p, I64, I64, C.POINTER(I64), C.POINTER(I64)]),
-            "orc_merge_splits": (None, [I64, I64, I64, i32p, i64p, i64p]),
-            "orc_merge_mv": (None, [I64, C.c_int, I64, I64, i32p, i32p, f64p, f64p, f64p]),
-            "orc_merge_power_iteration": (None, [I64, C.c_int, I64, I64, i32p, i32p, f64p, f64p, f64p, I64]),
-            "orc_csb_block_bits": (C.c_int, [I64, I64, C.c_int]),
-            "orc_zmorton_less_equal": (C.c_int, [C.c_uint32] * 4 + [C.c_int]),
-            "orc_csb_build": (C.c_int, [I64, I64, I64, i32p, i32p, f64p, C.c_int, u32p, f64p, i32p]),
-            "orc_csb_mv": (None, [I64, I64, C.c_int, u32p, f64p, i32p, f64p, f64p]),
-            "orc_hilbert": (I64, [I64, I64, I64]),
-            "orc_blockcoh_threads": (C.c_int, [I64, C.c_int]),
-            "orc_blockcoh_bands": (None, [I64, I64, i32p, C.c_int, i32p, i64p]),
-            "orc_blockcoh_beta_bits": (C.c_int, [I64, I64]),
-            "orc_blockcoh_hilbert_size": (I64, [I64, I64]),
-            "orc_blockcoh_build_band": (None, [I64, i32p, i32p, f64p, I64, I64, C.c_void_p]),
-            "orc_blockcoh_mv_band": (None, [C.c_void_p, I64, I64, f64p, f64p]),
-            "orc_matrix_stats": (None, [I64, i32p, i32p, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                        C.POINTER(I64), C.POINTER(C.c_int), C.POINTER(I64),
-                                        C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def baseline_lib():
+    """The same restatements built at -O3 (the reference's optimisation
+    level), for bench.py's CPU baseline timings only."""
+    global _baseline
+    if _baseline is None:
+        _baseline = _load(_BASELINE_PATH)
+    return _baseline
+
+
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    sig = {
+        "orc_integer_ceil": (I64, [I64, I64]),
+        "orc_binary_search_leftmost": (I64, [i32p, C.c_int32, I64]),
+        "orc_norm2": (C.c_double, [f64p, I64]),
+        "orc_normalize": (None, [f64p, I64]),
+        "orc_poisson_nnz": (I64, [I64, I64]),
+        "orc_fill_poisson": (None, [I64, I64, i32p, i32p, f64p]),
+        "orc_triplet_poisson": (None, [I64, I64, i32p, i32p, f64p]),
+        "orc_load_mm": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64),
+                                  C.POINTER(I64), C.POINTER(C.POINTER(C.c_int32)),
+                                  C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double))]),
+        "orc_load_bin": (C.c_int, [C.c_char_p, I64, C.c_int, C.c_int, C.POINTER(I64), C.POINTER(I64),
+                                   C.POINTER(I64), C.POINTER(C.POINTER(C.c_int32)),
+                                   C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_double))]),
+        "orc_free": (None, [C.c_void_p]),
+        "orc_mt19937_seed": (None, [u32p, C.c_uint32]),
+        "orc_mt19937_next": (C.c_uint32, [u32p]),
+        "orc_uniform_real": (C.c_double, [u32p, C.c_double, C.c_double]),
+        "orc_triplet_to_crs": (None, [I64, I64, i32p, i32p, f64p, i32p, i32p, f64p]),
+        "orc_row_partitions": (None, [I64, C.c_int, i32p, i32p]),
+        "orc_crs_mv": (None, [I64, i32p, i32p, f64p, f64p, f64p]),
+        "orc_crs_power_iteration": (None, [I64, i32p, i32p, f64p, f64p, f64p, I64]),
+        "orc_crsomp_mv": (None, [C.c_int, I64, i32p, i32p, f64p, f64p, f64p]),
+        "orc_crsomp_power_iteration": (None, [C.c_int, I64, i32p, i32p, f64p, f64p, f64p, I64]),
+        "orc_merge_search": (None, [I64, i32p, I64, I64, C.POINTER(I64), C.POINTER(I64)]),
+        "orc_merge_splits": (None, [I64, I64, I64, i32p, i64p, i64p]),
+        "orc_merge_mv": (None, [I64, C.c_int, I64, I64, i32p, i32p, f64p, f64p, f64p]),
+        "orc_merge_power_iteration": (None, [I64, C.c_int, I64, I64, i32p, i32p, f64p, f64p, f64p, I64]),
+        "orc_csb_block_bits": (C.c_int, [I64, I64, C.c_int]),
+        "orc_zmorton_less_equal": (C.c_int, [C.c_uint32] * 4 + [C.c_int]),
+        "orc_csb_build": (C.c_int, [I64, I64, I64, i32p, i32p, f64p, C.c_int, u32p, f64p, i32p]),
+        "orc_csb_mv": (None, [I64, I64, C.c_int, u32p, f64p, i32p, f64p, f64p]),
+        "orc_hilbert": (I64, [I64, I64, I64]),
+        "orc_blockcoh_threads": (C.c_int, [I64, C.c_int]),
+        "orc_blockcoh_bands": (None, [I64, I64, i32p, C.c_int, i32p, i64p]),
+        "orc_blockcoh_beta_bits": (C.c_int, [I64, I64]),
+        "orc_blockcoh_hilbert_size": (I64, [I64, I64]),
+        "orc_blockcoh_build_band": (None, [I64, i32p, i32p, f64p, I64, I64, C.c_void_p]),
+        "orc_blockcoh_mv_band": (None, [C.c_void_p, I64, I64, f64p, f64p]),
+        "orc_matrix_stats": (None, [I64, i32p, i32p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                    C.POINTER(I64), C.POINTER(C.c_int), C.POINTER(I64),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
 
 
 # ---------------------------------------------------------------- matrices

@@ -180,6 +180,23 @@ int pwm_mv_device(pwm_matrix* h, const void* x_dev, void* y_dev);
 int pwm_power_method_device(pwm_matrix* h, void* x_dev, void* y_dev, int64_t max_it, double tol,
                             double* lambda, double* ynorm, int64_t* iterations);
 
+/* No counterpart in the reference.  pwm_power_method_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same kernels and results, but
+ * the power method's workspace (step control, one record per step, FP64
+ * partial sums) is a fixed 8 MiB device array of the library, one per GPU,
+ * held until the process ends, instead of buffers the handle allocates on
+ * first use.  No hipMalloc or hipFree is called; the handle itself must be
+ * loaded (loading allocates).  Limits: max_it <= PWM_NOALLOC_MAX_IT and the
+ * handle's partial-sum slots (its tiles / workgroups) <=
+ * PWM_NOALLOC_MAX_SLOTS, else PWM_ERR_INVALID.  Every call on a GPU uses that
+ * GPU's one array: calls that overlap on one GPU (from other threads, on
+ * other handles or streams) race and must not be made. */
+#define PWM_NOALLOC_MAX_IT    65536
+#define PWM_NOALLOC_MAX_SLOTS 262133
+int pwm_power_method_device_noalloc(pwm_matrix* h, void* x_dev, void* y_dev, int64_t max_it, double tol,
+                                   double* lambda, double* ynorm, int64_t* iterations);
+
 /* CRS_Merge split points actually used on the device (P+1 entries each;
  * SOA_Implementations/CRS_Merge.hpp:96-112).  Call with a == NULL to get P. */
 int pwm_merge_splits(pwm_matrix* h, int64_t* P, int64_t* a, int64_t* b);

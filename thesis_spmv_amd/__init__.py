@@ -57,6 +57,7 @@ _SIGS = {
     "pwm_set_option": [VP, C.c_char_p, I64],
     "pwm_mv_device": [VP, VP, VP],
     "pwm_power_method_device": [VP, VP, VP, I64, F64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64)],
+    "pwm_power_method_device_noalloc": [VP, VP, VP, I64, F64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64)],
     "pwm_merge_splits": [VP, C.POINTER(I64), VP, VP],
     "pwm_crs_export": [VP, VP, VP, VP],
     "pwm_csb_layout": [VP, C.POINTER(I32), C.POINTER(I64), VP, VP, VP],
@@ -288,6 +289,14 @@ class SparseMatrixHip:
         lam, nrm, its = F64(), F64(), I64()
         _check(lib().pwm_power_method_device(self.h, x.ptr, y.ptr, int(max_it), float(tol), C.byref(lam),
                                              C.byref(nrm), C.byref(its)), self.h)
+        return lam.value, nrm.value, its.value
+
+    def power_method_device_noalloc(self, x, y, max_it, tol=-1.0):
+        """power_method_device with the library's fixed per-GPU workspace:
+        allocates no device memory (pwm_power_method_device_noalloc)."""
+        lam, nrm, its = F64(), F64(), I64()
+        _check(lib().pwm_power_method_device_noalloc(self.h, x.ptr, y.ptr, int(max_it), float(tol), C.byref(lam),
+                                                     C.byref(nrm), C.byref(its)), self.h)
         return lam.value, nrm.value, its.value
 
     def time_mv(self, x, y, reps, flush_bytes=0, per_step=True):

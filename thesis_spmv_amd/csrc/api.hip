@@ -564,6 +564,8 @@ static int64_t partials_needed(const pwm_matrix* h) {
 
 // ---------------------------------------------------------------- lifetime
 
+static int noalloc_scratch(pwm_matrix* h, char** out);
+
 extern "C" int pwm_create(int format, int threads, int value_type, pwm_matrix** out) {
     if (!out) return fail(nullptr, PWM_ERR_INVALID, "pwm_create: out is NULL");
     *out = nullptr;
@@ -584,6 +586,15 @@ extern "C" int pwm_create(int format, int threads, int value_type, pwm_matrix** 
     if (e != hipSuccess) {
         delete h;
         return hip_fail(nullptr, e, "pwm_create: stream");
+    }
+    // set-up of the no-allocation power method: the per-GPU workspace array
+    // is made resident on this device and its address looked up, once per
+    // device, so that pwm_power_method_device_noalloc itself allocates nothing
+    char* ws = nullptr;
+    if (int rc = noalloc_scratch(nullptr, &ws)) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
     }
     *out = h;
     return PWM_OK;
@@ -1045,23 +1056,37 @@ static int power_workspace(pwm_matrix* h, int64_t max_it) {
     return PWM_OK;
 }
 
-// Runs the device power loop on buf0 (start vector) / buf1; returns steps done.
+// Device workspace of one power method: step control (2 ints), one record
+// of 4 doubles per step, 3 FP64 partial sums per partial slot.
+struct PowerWs {
+    int* ctrl;
+    double* rec;
+    double* partials;
+};
+
+// Runs the device power loop on buf0 (start vector) / buf1 with workspace ws
+// (NULL: the handle's own, allocated on first use); returns steps done.
 static int run_power(pwm_matrix* h, void* buf0, void* buf1, int64_t max_it, double tol, int64_t* steps,
-                     double* lambda, double* ynorm) {
+                     double* lambda, double* ynorm, const PowerWs* ws = nullptr) {
     if (h->nor != h->noc) return fail(h, PWM_ERR_NOT_SQUARE, "power method needs a square matrix");
     if (max_it < 0) return fail(h, PWM_ERR_INVALID, "negative iteration count");
     if (max_it == 0) {
         if (steps) *steps = 0;
         return PWM_OK;
     }
-    int rc = power_workspace(h, max_it);
-    if (rc != PWM_OK) return rc;
+    int rc;
+    PowerWs own;
+    if (!ws) {
+        if ((rc = power_workspace(h, max_it)) != PWM_OK) return rc;
+        own = PowerWs{h->pw_ctrl.as<int>(), h->pw_rec.as<double>(), h->pw_partials.as<double>()};
+        ws = &own;
+    }
     PowerArgs pw;
     pw.buf0 = buf0;
     pw.buf1 = buf1;
-    pw.ctrl = h->pw_ctrl.as<int>();
-    pw.rec = h->pw_rec.as<double>();
-    pw.partials = h->pw_partials.as<double>();
+    pw.ctrl = ws->ctrl;
+    pw.rec = ws->rec;
+    pw.partials = ws->partials;
     pw.max_it = max_it;
     pw.tol = tol;
     hipError_t e = pwmk::launch_power_reset(pw, h->stream);
@@ -1077,20 +1102,21 @@ static int run_power(pwm_matrix* h, void* buf0, void* buf1, int64_t max_it, doub
         }
         issued += n;
         if (tol >= 0) {
-            e = hipMemcpyAsync(ctrl, h->pw_ctrl.p, 8, hipMemcpyDeviceToHost, h->stream);
+            e = hipMemcpyAsync(ctrl, ws->ctrl, 8, hipMemcpyDeviceToHost, h->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
             if (e != hipSuccess) return hip_fail(h, e, "power loop");
             if (ctrl[1]) break;
         }
     }
     e = pwmk::launch_power_finish(pw, h->nor, h->vtype == PWM_VALUES_F64, 0, h->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(ctrl, h->pw_ctrl.p, 8, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctrl, ws->ctrl, 8, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "power finish");
     if (steps) *steps = ctrl[0];
     if ((lambda || ynorm) && ctrl[0] > 0) {
         double rec[4];
-        e = hipMemcpy(rec, h->pw_rec.as<double>() + 4 * (ctrl[0] - 1), 32, hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(rec, ws->rec + 4 * (ctrl[0] - 1), 32, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "power record");
         if (lambda) *lambda = rec[3];
         if (ynorm) *ynorm = sqrt(rec[0]);
@@ -1125,6 +1151,76 @@ extern "C" int pwm_power_method_device(pwm_matrix* h, void* x, void* y, int64_t 
     if (int rc0 = check_device_xy(h, x, h->nor, y, h->nor, "pwm_power_method_device")) return rc0;
     int64_t steps = 0;
     int rc = run_power(h, x, y, max_it, tol, &steps, lambda, ynorm);
+    if (iterations) *iterations = steps;
+    return rc;
+}
+
+// ---------------------------------------------------------------- no-allocation power method
+
+// Workspace of pwm_power_method_device_noalloc: one fixed device array per
+// GPU (each device holds its own copy until the process ends).  Layout:
+// step control at 0, kNoAllocMaxIt step records (32 B) from 256, then the
+// FP64 partial sums (24 B per slot) up to the end.
+constexpr int64_t kNoAllocBytes = (int64_t)8 << 20;
+constexpr int64_t kNoAllocMaxIt = 65536;
+constexpr int64_t kNoAllocRecOff = 256;
+constexpr int64_t kNoAllocPartOff = kNoAllocRecOff + 32 * kNoAllocMaxIt;
+constexpr int64_t kNoAllocMaxSlots = (kNoAllocBytes - kNoAllocPartOff) / 24;
+static_assert(PWM_NOALLOC_MAX_IT == kNoAllocMaxIt, "header and library disagree on the iteration limit");
+static_assert(PWM_NOALLOC_MAX_SLOTS == kNoAllocMaxSlots, "header and library disagree on the slot limit");
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (pwm_create does it for every device it
+// runs on; the lookup loads the array onto the device).  The host-side name
+// `scratch` is only a stand-in: host code passes this address to kernels,
+// never the name.
+static int noalloc_scratch(pwm_matrix* h, char** out) {
+    static std::mutex mu;
+    static std::vector<char*> addr;  // per device ordinal
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_power_method_device_noalloc: hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return hip_fail(h, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                            "pwm_power_method_device_noalloc: workspace address");
+        }
+        addr[dev] = (char*)p;
+    }
+    *out = addr[dev];
+    return PWM_OK;
+}
+
+extern "C" int pwm_power_method_device_noalloc(pwm_matrix* h, void* x, void* y, int64_t max_it, double tol,
+                                               double* lambda, double* ynorm, int64_t* iterations) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, "pwm_power_method_device_noalloc: NULL handle");
+    if (!h->loaded) return fail(h, PWM_ERR_STATE, "pwm_power_method_device_noalloc: no matrix loaded");
+    if (!x || !y) return fail(h, PWM_ERR_INVALID, "pwm_power_method_device_noalloc: NULL vector");
+    const size_t vs = vector_size(h->vtype);
+    if ((const char*)x < (const char*)y + h->nor * vs && (const char*)y < (const char*)x + h->nor * vs)
+        return fail(h, PWM_ERR_INVALID, "pwm_power_method_device_noalloc: x and y overlap");
+    if (max_it > kNoAllocMaxIt)
+        return fail(h, PWM_ERR_INVALID, "pwm_power_method_device_noalloc: max_it above " +
+                                            std::to_string(kNoAllocMaxIt) + ", the workspace's step records");
+    const int64_t slots = std::max<int64_t>(partials_needed(h), 1);
+    if (slots > kNoAllocMaxSlots)
+        return fail(h, PWM_ERR_INVALID, "pwm_power_method_device_noalloc: the matrix's plan needs " +
+                                            std::to_string(slots) + " partial-sum slots, above the workspace's " +
+                                            std::to_string(kNoAllocMaxSlots));
+    if (int rc0 = check_device_xy(h, x, h->nor, y, h->nor, "pwm_power_method_device_noalloc")) return rc0;
+    char* base = nullptr;
+    if (int rc0 = noalloc_scratch(h, &base)) return rc0;
+    // the step control is reset by the power loop's first kernel; records
+    // and partial sums are written before they are read
+    const PowerWs ws{(int*)base, (double*)(base + kNoAllocRecOff), (double*)(base + kNoAllocPartOff)};
+    int64_t steps = 0;
+    int rc = run_power(h, x, y, max_it, tol, &steps, lambda, ynorm, &ws);
     if (iterations) *iterations = steps;
     return rc;
 }

@@ -186,7 +186,9 @@ def bench_spmv(args, workload, cpu=True):
     results = {}
     wanted = set(args.formats.split(","))
     csb_opts = {"block_bits": 14} if workload == "stencil9" else {}
-    csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll)) if v})
+    csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll),
+                                       ("csb_layout", args.csb_layout), ("csb_window_bits", args.csb_window_bits))
+                     if v})
     if args.csb_nt >= 0:
         csb_opts["csb_nt"] = args.csb_nt
     crs_opts = {"crs_kernel": args.crs_kernel} if args.crs_kernel else {}
@@ -341,7 +343,10 @@ def bench_spmv(args, workload, cpu=True):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    csb_layouts = {1: "slab pieces", 2: "slab streams", 3: "wide slab streams", 4: "swept slabs"}
     what = "csb slab kernel + column-group reduction" if two_kernels else f"{best} dominant kernel"
+    if best.startswith("csb"):
+        what += f" ({csb_layouts.get(r['info']['layout'], 'layout ' + str(r['info']['layout']))})"
     if copies:
         what += f", rotating over {copies} copies of A, x, y ({copies * bytes_mv / 1e6:.0f} MB > 256 MB Infinity Cache)"
     wall_ms = r["rot"]["wall_ms"] if copies else r["ms_per_step"]
@@ -379,7 +384,7 @@ def bench_spmv(args, workload, cpu=True):
                                             else round(v["kernel_ms_flushed"] * 1e3, 2)),
              "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
              "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
-             "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"],
+             "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"], "layout": v["info"]["layout"],
              "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"],
              "host_array_mv_us": round(v["host_ms"] * 1e3, 1)}
         if k == "crs":
@@ -797,6 +802,9 @@ def main():
                     help="CRS kernel: 0 = time the candidates at load, 1 = vector, 2 = row-aligned tiles")
     ap.add_argument("--csb-groups", type=int, default=0, help="CSB block-column groups (0 = time 1, 8, 16 at load)")
     ap.add_argument("--csb-unroll", type=int, default=0, help="CSB stream kernel unroll (0 = time 2 and 4 at load)")
+    ap.add_argument("--csb-layout", type=int, default=0,
+                    help="CSB layout: 0 = automatic, 1 slab pieces, 2 slab streams, 3 wide streams, 4 swept slabs")
+    ap.add_argument("--csb-window-bits", type=int, default=0, help="swept slabs: log2 columns per window (0 = 1 MB)")
     ap.add_argument("--csb-nt", type=int, default=-1,
                     help="CSB stream kernel non-temporal entry loads: 1, 0, or -1 = time both at load")
     ap.add_argument("--crs-nt", type=int, default=-1,

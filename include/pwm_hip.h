@@ -77,7 +77,7 @@ typedef struct {
     int32_t balanced;          /* CSB balanced flag (CSB.hpp:549-559) */
     int32_t unroll;            /* CSB slab-stream kernel: groups of 4 entries per thread per step */
     int32_t non_temporal;      /* CSB slab streams / CRS tiles: 1 = non-temporal entry loads */
-    int32_t layout;            /* CSB: 1 slab pieces, 2 slab streams, 3 wide slab streams */
+    int32_t layout;            /* CSB: 1 slab pieces, 2 slab streams, 3 wide slab streams, 4 swept slabs */
 } pwm_info_t;
 
 /* Constructor SparseMatrix(int threads) (Matrix/SparseMatrix.hpp:30) through
@@ -147,8 +147,13 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
  * load and keep the fastest), "csb_layout" (CSB device layout: 1 = slab
  * pieces over the stored block order, 2 = one contiguous entry stream per row
  * slab with (row in slab, column) packed in 32 bits, 3 = the same streams
- * with the column (u32) and the row in slab (u16) stored apart, 0 = 2 when
- * slabs of at least 256 rows pack, else 3), "csb_unroll" (slab-stream kernel groups of 4 entries
+ * with the column (u32) and the row in slab (u16) stored apart, 4 = swept
+ * slabs: one nnz-balanced row slab per CU, its y-slice in LDS, the entries
+ * stored window by window (1 MB of x per window) and every workgroup
+ * walking the windows in order; 0 = automatic: 2 when slabs of at least 256
+ * rows pack, else 3, replaced by 4 when x exceeds 2 MB, the slabs fit and 4
+ * runs faster in the load-time timing), "csb_window_bits" (swept slabs: log2 of the
+ * columns per window, 0 = 1 MB of x), "csb_unroll" (slab-stream kernel groups of 4 entries
  * per thread per step, 2 or 4; 0 = time both with each group count),
  * "csb_nt" (slab-stream kernel: 1 = non-temporal entry loads and partial-slab
  * stores, 0 = plain, -1 = time both at load, the default),

@@ -17,10 +17,13 @@ FORMATS = [pw.CSBHip, pw.BlockCOHHip]
 # CSB device layouts: 1 = slab pieces over the stored block order, 2 = one
 # contiguous stream per slab (the automatic choice when its index packs),
 # 3 = the same streams with column and row in slab stored apart (the
-# automatic choice past 2^24 columns)
+# automatic choice past 2^24 columns), 4 = swept slabs (persistent
+# nnz-balanced slabs walking x window by window)
 # layout options as stored: slab pieces built on the device and on the host,
-# packed and wide slab streams
-CSB_VARIANTS = [{"csb_layout": 1}, {"csb_layout": 1, "csb_build": 1}, {"csb_layout": 2}, {"csb_layout": 3}]
+# packed and wide slab streams, swept slabs with 1 MB windows and with
+# 1024-column windows
+CSB_VARIANTS = [{"csb_layout": 1}, {"csb_layout": 1, "csb_build": 1}, {"csb_layout": 2}, {"csb_layout": 3},
+                {"csb_layout": 4}, {"csb_layout": 4, "csb_window_bits": 10}]
 
 
 def _relerr(a, b):
@@ -365,3 +368,55 @@ def test_csb_wide_streams_past_2_24_columns():
         assert _relerr(y, ref) <= 1e-13, opt
         gb, gind, gdata, gblk = m.csb_layout()
         assert gb == rb and np.array_equal(gind, rind) and np.array_equal(gblk, rblk) and np.array_equal(gdata, rdata)
+
+
+# ---------------------------------------------------------------- swept slabs
+
+@pytest.mark.parametrize("values,tol", [("f64", 1e-13), ("f32", 2e-5), ("bf16", 2e-5)])
+@pytest.mark.parametrize("opt", [{}, {"csb_window_bits": 12}, {"csb_nt": 0}],
+                         ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()) or "default")
+def test_csb_swept_slabs(values, tol, opt):
+    """Swept slabs on power-law (FP64/FP32) and Poisson (BF16, exact values)
+    matrices against the oracle: SpMV, and 9 power iterations from ones."""
+    A = oracle.poisson_crs(300, 170) if values == "bf16" else _powerlaw(150000, seed=9, cap=20000)
+    m = _load(pw.CSBHip, A, values=values, csb_layout=4, **opt)
+    info = m.info()
+    assert info["layout"] == 4 and info["threads"] >= 1
+    x = np.random.default_rng(3).uniform(-1, 1, A.nor)
+    y = np.full(A.nor, np.nan)
+    m.mv(x, y)
+    err = _relerr(y, oracle.crs_mv(A, x))
+    print(f"swept slabs {values} {opt}: {info['threads']} slabs, rel.err {err:.2e}", flush=True)
+    assert err <= tol
+    if values == "f64":
+        xs, ys = np.ones(A.nor), np.zeros(A.nor)
+        m.powerIteration(xs, ys, 9)
+        xr, yr = oracle.crs_power_iteration(A, np.ones(A.nor), 9)
+        assert np.max(np.abs(xs - xr)) <= 1e-12 and np.max(np.abs(ys - yr)) <= 1e-12
+
+
+def test_csb_swept_slabs_edge_shapes():
+    """Empty rows, a row longer than a slab's share, and rectangular matrices
+    whose column count is not a multiple of the window."""
+    rng = np.random.default_rng(11)
+    cases = []
+    n = 5000
+    lens = rng.integers(0, 6, n)
+    lens[:100] = 0
+    lens[2500] = 4000
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(n, int(L), replace=False)) for L in lens]).astype(np.int32)
+    cases.append(oracle.CRSMatrix(n, n, rs, ci, rng.standard_normal(int(rs[-1]))))
+    for nor, noc in ((700, 5000), (5000, 1300)):
+        lens = rng.integers(0, 9, nor)
+        rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        ci = np.concatenate([np.sort(rng.choice(noc, int(L), replace=False)) for L in lens]).astype(np.int32)
+        cases.append(oracle.CRSMatrix(nor, noc, rs, ci, rng.standard_normal(int(rs[-1]))))
+    for A in cases:
+        for wbits in (0, 10, 11):
+            m = _load(pw.CSBHip, A, csb_layout=4, csb_window_bits=wbits)
+            x = rng.standard_normal(A.noc)
+            y = np.full(A.nor, np.nan)
+            m.mv(x, y)
+            assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13, (A.nor, A.noc, wbits)
+

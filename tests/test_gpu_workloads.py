@@ -43,6 +43,7 @@ FORMATS = [
     ("merge_path", pw.CRSMergeHip, {}),
     ("csb_auto", pw.CSBHip, {}),
     ("csb_beta14_8groups", pw.CSBHip, {"block_bits": 14, "csb_groups": 8, "csb_nt": 1}),
+    ("csb_swept", pw.CSBHip, {"csb_layout": 4}),
     ("blockcoh", pw.BlockCOHHip, {}),
 ]
 
@@ -100,13 +101,20 @@ def test_powerlaw_full_size(powerlaw, name, cls, opt):
 def test_stencil9_full_size(stencil9, name, cls, opt):
     A, x, ref = stencil9
     print(f"stencil9 N={A.nor} nnz={A.nnz} format {name}", flush=True)
-    m = _load(cls, A, **opt)
-    y = np.zeros(A.nor)
-    m.mv(x, y)
-    err = _relerr(y, ref)
-    print(f"  {name} FP64: rel.err {err:.3e}", flush=True)
-    assert err <= 1e-13
-    del m
+    if name == "csb_swept":
+        # 8.4M FP64 rows (67 MB) exceed the LDS of all the chip's CUs together
+        # (256 x 160 KB): the swept layout refuses them; FP32 rows fit
+        with pytest.raises(pw.PwmError) as e:
+            _load(cls, A, **opt)
+        assert e.value.code == -8
+    else:
+        m = _load(cls, A, **opt)
+        y = np.zeros(A.nor)
+        m.mv(x, y)
+        err = _relerr(y, ref)
+        print(f"  {name} FP64: rel.err {err:.3e}", flush=True)
+        assert err <= 1e-13
+        del m
     # FP32 values, vectors and row sums: each row within the first-order
     # error bound of an FP32 dot product of its length over FP32-rounded
     # operands, (len + 4) u sum_j |a_ij x_j|

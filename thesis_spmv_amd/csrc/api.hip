@@ -637,10 +637,15 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
         if (value < 0 || value > 64) return fail(h, PWM_ERR_INVALID, "csb_groups must be 0..64");
         h->csb_groups = (int32_t)value;
     } else if (k == "csb_layout") {
-        if (value < 0 || value > 3)
+        if (value < 0 || value > 4)
             return fail(h, PWM_ERR_INVALID,
-                        "csb_layout must be 0 (automatic), 1 (slab pieces), 2 (slab streams) or 3 (wide slab streams)");
+                        "csb_layout must be 0 (automatic), 1 (slab pieces), 2 (slab streams), 3 (wide slab streams) "
+                        "or 4 (swept slabs)");
         h->csb_layout = (int32_t)value;
+    } else if (k == "csb_window_bits") {
+        if (value != 0 && (value < 10 || value > 24))
+            return fail(h, PWM_ERR_INVALID, "csb_window_bits must be 0 (1 MB of x) or in [10, 24]");
+        h->csb_window_bits = (int32_t)value;
     } else if (k == "csb_unroll") {
         if (value != 0 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_unroll must be 0, 2 or 4");
         h->csb_unroll = (int32_t)value;
@@ -914,14 +919,14 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
     out->block_bits = h->format == PWM_FORMAT_CSB ? h->csb.bits : 0;
     out->balanced = h->format == PWM_FORMAT_CSB ? h->csb.balanced : 0;
     out->unroll = h->format == PWM_FORMAT_CSB && h->csb.stream ? h->csb.unroll : 0;
-    out->layout = h->format != PWM_FORMAT_CSB ? 0 : !h->csb.stream ? 1 : h->csb.wide ? 3 : 2;
+    out->layout = h->format != PWM_FORMAT_CSB ? 0 : h->csb.sweep ? 4 : !h->csb.stream ? 1 : h->csb.wide ? 3 : 2;
     out->non_temporal = h->format == PWM_FORMAT_CSB                         ? h->csb.nt
                         : (h->format == PWM_FORMAT_CRS && h->crs_used == 2) ? h->crs_nt_used
                                                                            : 0;
     const pwm_impl::DevBuf* bufs[] = {&h->rs, &h->ci, &h->val, &h->lr_chunks, &h->lr_rows, &h->lr_begin,
                                       &h->lr_partial, &h->split_a, &h->split_b, &h->carry_row, &h->carry_val, &h->tiles,
                                       &h->csb.ind, &h->csb.data, &h->csb.blk_ptr, &h->csb.work, &h->csb.ypart,
-                                      &h->csb.soff, &h->csb.srows,
+                                      &h->csb.soff, &h->csb.srows, &h->csb.sweep_start,
                                       &h->coh.row_jump, &h->coh.col_jump, &h->coh.data, &h->coh.row_jump_block,
                                       &h->coh.col_jump_block, &h->coh.block_nnz, &h->coh.block_coords,
                                       &h->coh.band_table, &h->coh.tasks, &h->pw_ctrl, &h->pw_rec, &h->pw_partials,

@@ -50,6 +50,10 @@ struct DevBuf {
     ~DevBuf() { release(); }
     hipError_t alloc(size_t n);
     void release();
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
     template <typename T> T* as() const { return (T*)p; }
 };
 
@@ -81,6 +85,16 @@ struct CsbDev {
     int32_t nt = 0;              // stream kernel: non-temporal entry loads / partial stores
     int32_t wide = 0;            // wide streams: ind = column, srows = row in slab (u16)
     DevBuf soff, srows;
+    // Swept slabs (sweep = 1): T nnz-balanced row slabs [sstart[t],
+    // sstart[t+1]), each one persistent workgroup; a slab's entries are
+    // stored window by window (W windows of 2^sweep_wb columns, soff[t*W + p]
+    // the first entry of window p), packed (row in slab << sweep_wb) |
+    // column in window.
+    int32_t sweep = 0;
+    int32_t sweep_wb = 0;
+    int64_t sweep_W = 0;
+    int32_t sweep_rows = 0;      // LDS rows of a slab (the largest slab)
+    DevBuf sweep_start;
     void clear() {
         bits = 0;
         H = V = 0;
@@ -93,6 +107,11 @@ struct CsbDev {
         ypart.release();
         soff.release();
         srows.release();
+        sweep_start.release();
+        sweep = 0;
+        sweep_wb = 0;
+        sweep_W = 0;
+        sweep_rows = 0;
         wide = 0;
         stream = 0;
         col_bits = 0;
@@ -171,7 +190,8 @@ struct pwm_matrix {
     // blocked formats
     int32_t csb_bits_override = 0;
     int32_t csb_groups = 0;  // 0 = autotune
-    int32_t csb_layout = 0;  // 0 = streams when the packed index fits, 1 = pieces, 2 = streams
+    int32_t csb_layout = 0;  // 0 = automatic, 1 = pieces, 2 = streams, 3 = wide streams, 4 = swept slabs
+    int32_t csb_window_bits = 0;  // swept slabs: log2 of the columns per window (0 = 1 MB of x)
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
     int32_t csb_build = 0;   // CSB slab pieces: 0 = built on the device, 1 = on the host
@@ -205,6 +225,12 @@ int csb_stream_device(pwm_matrix* h, int rb, int cb, int64_t S, bool wide);
 // CSB slab-piece layout (the stored order, blk_ptr and the pieces of slabs of
 // beta >> k rows) built on the device the same way; h->csb.bits/H/V set.
 int csb_pieces_device(pwm_matrix* h, int k);
+// CSB swept-slab layout built on the device: slabs cut from the host
+// row_start `rs` (nnz-balanced towards `target_slabs`, at most `max_rows`
+// rows each), windows of 2^wb columns.  Returns PWM_ERR_UNSUPPORTED, with
+// nothing built, when the cut needs more than `max_slabs` slabs.
+int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int64_t max_rows, int64_t max_slabs,
+                     int wb);
 // BlockCOH layout built on the device from the device CRS arrays
 // (coh_build.hip); h->coh.bands holds the band partition (start, rows, nnz,
 // beta_bits, H, V, off_nz) on entry and the per-band sizes and offsets on

@@ -52,7 +52,21 @@ struct CsbArgs {
     // row in the slab (u16), for column indices too wide to pack.
     int32_t wide = 0;
     const uint16_t* srows = nullptr;
+    // Swept slabs (sweep = 1): n_tasks slabs [sstart[t], sstart[t+1]), one
+    // persistent workgroup each with max_rows LDS rows; entries of slab t in
+    // column window p are ind/val[soff[t*W + p] .. soff[t*W + p + 1]),
+    // packed (row in slab << wb) | column in window.
+    int32_t sweep = 0;
+    int32_t wb = 0;
+    int64_t W = 0;
+    int32_t max_rows = 0;
+    const int32_t* sstart = nullptr;
 };
+
+// Swept slabs: workgroups of the sweep kernel one CU holds at once with
+// `rows` LDS rows of the value type's accumulator (0 if none fit).
+int csb_sweep_blocks_per_cu(int vtype, int rows);
+
 
 // One BlockCOH block (offsets are relative to its band's arrays).
 struct CohBlock {

@@ -65,6 +65,46 @@ hipError_t upload(DevBuf& d, const void* src, size_t bytes) {
 
 // Values of the handle's storage type: uploaded as double, converted on the
 // device (same path as the CRS formats).
+// A swept-slab layout set aside while the other layouts are built and timed.
+struct SweptSave {
+    DevBuf ind, data, soff, start;
+    int32_t wb = 0, rows = 0, nt = 1;
+    int64_t W = 0, T = 0;
+    void take(CsbDev& c) {
+        ind.swap(c.ind);
+        data.swap(c.data);
+        soff.swap(c.soff);
+        start.swap(c.sweep_start);
+        wb = c.sweep_wb;
+        W = c.sweep_W;
+        rows = c.sweep_rows;
+        T = c.n_tasks;
+        nt = c.nt;
+        c.sweep = 0;
+        c.sweep_wb = 0;
+        c.sweep_W = 0;
+        c.sweep_rows = 0;
+    }
+    void put(CsbDev& c) {
+        c.ind.swap(ind);
+        c.data.swap(data);
+        c.soff.swap(soff);
+        c.sweep_start.swap(start);
+        c.sweep = 1;
+        c.sweep_wb = wb;
+        c.sweep_W = W;
+        c.sweep_rows = rows;
+        c.n_tasks = T;
+        c.nt = nt;
+        c.stream = 0;
+        c.wide = 0;
+        c.groups = 1;
+        c.unroll = 2;
+        c.slabs = 1;
+        c.slab_rows = rows;
+    }
+};
+
 static int upload_blocked_values(pwm_matrix* h, DevBuf& dst, const std::vector<double>& v) {
     hipError_t e = dst.alloc(std::max<size_t>(v.size(), 1) * value_size(h->vtype));
     if (e != hipSuccess) return hip_fail(h, e, "blocked values");
@@ -111,6 +151,61 @@ static int download_crs(pwm_matrix* h, std::vector<int32_t>& ci, std::vector<dou
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(vals.data(), tmp.p, nnz * 8, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(h, e, "CRS download");
+    return PWM_OK;
+}
+
+// Mean milliseconds of 5 SpMVs (after 2 warm-up ones) of the handle's CSB
+// layout as it stands, on x = 1.
+static int time_csb(pwm_matrix* h, float* ms) {
+    DevBuf tx, ty;
+    const size_t vs = vector_size(h->vtype);
+    hipError_t e;
+    if ((e = tx.alloc(std::max<int64_t>(h->noc, 1) * vs)) != hipSuccess ||
+        (e = ty.alloc(std::max<int64_t>(h->nor, 1) * vs)) != hipSuccess)
+        return hip_fail(h, e, "CSB autotune vectors");
+    e = pwmk::launch_fill(tx.p, h->noc, h->vtype == PWM_VALUES_F64, 1.0, h->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = csb_launch(h, tx.p, ty.p, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = csb_launch(h, tx.p, ty.p, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(h, e, "CSB autotune");
+    *ms = t / 5.0f;
+    return PWM_OK;
+}
+
+// Swept-slab layout: as many nnz-balanced slabs as the chip has CUs (one
+// persistent workgroup of 1024 threads each, with all of the CU's LDS for its
+// y-slice), windows of 1 MB of x unless csb_window_bits says otherwise.
+// PWM_ERR_UNSUPPORTED when the slabs' rows cannot fit.
+static int build_swept(pwm_matrix* h, const int32_t* rs) {
+    CsbDev& c = h->csb;
+    int cus = 0, lds_cu = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 0);
+    if (e != hipSuccess) return hip_fail(h, e, "CSB swept slabs: device attributes");
+    const int acc = h->vtype == PWM_VALUES_F64 ? 8 : 4;
+    int wb = h->csb_window_bits;
+    if (wb == 0) {
+        wb = 10;
+        while (((int64_t)acc << (wb + 1)) <= ((int64_t)1 << 20)) ++wb;
+        // the packed index keeps 32 - wb bits for the row in the slab: narrow
+        // the window while the slabs' expected rows (a fifth of slack) need more
+        const int64_t rows_needed = (h->nor + cus - 1) / std::max(1, cus * 24 / 25) * 6 / 5;
+        while (wb > 12 && (((int64_t)1 << (32 - wb)) - 1) < rows_needed) --wb;
+    }
+    const int64_t rows_cap = std::min<int64_t>(((lds_cu - 1024) / acc) & ~(int64_t)63, ((int64_t)1 << (32 - wb)) - 1);
+    if (rows_cap <= 0 || blocked::csb_sweep_blocks_per_cu(h->vtype, (int)rows_cap) < 1) return PWM_ERR_UNSUPPORTED;
+    const int rc = csb_sweep_device(h, rs, std::max(1, cus * 24 / 25), rows_cap, cus, wb);
+    if (rc != PWM_OK) return rc;
+    c.nt = h->csb_nt >= 0 ? h->csb_nt : 1;
     return PWM_OK;
 }
 
@@ -167,6 +262,37 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
     const bool use_stream = h->csb_layout != 1;
     const int rb = wide ? rb_fit : rb_packed;
     const int64_t S_stream = beta >> rb;
+    // Swept slabs (one persistent workgroup per CU, nnz-balanced row slabs,
+    // x walked in 1 MB windows): built when asked (csb_layout = 4) or, by
+    // default, when x outgrows one XCD's L2 (2 MB), every slab's rows fit
+    // the LDS of one workgroup per CU, and the caller fixed no slab-stream
+    // kernel parameter (csb_groups, csb_unroll).  By default they are then
+    // timed against the slab-stream candidates below and the faster is kept.
+    SweptSave swept;
+    float swept_ms = -1.0f;
+    const bool try_sweep = h->csb_layout == 4 ||
+                           (h->csb_layout == 0 && h->csb_groups == 0 && h->csb_unroll == 0 && nnz > 0 &&
+                            noc * (int64_t)vector_size(h->vtype) > ((int64_t)2 << 20));
+    if (try_sweep) {
+        rc = build_swept(h, rs);
+        if (rc == PWM_OK) {
+            if (h->csb_layout == 4) {
+                // the CRS arrays are not needed by the CSB kernels
+                h->rs.release();
+                h->ci.release();
+                h->val.release();
+                return PWM_OK;
+            }
+            if ((rc = time_csb(h, &swept_ms)) != PWM_OK) return rc;
+            swept.take(c);
+        } else if (rc != PWM_ERR_UNSUPPORTED) {
+            return rc;
+        } else if (h->csb_layout == 4) {
+            return fail(h, PWM_ERR_UNSUPPORTED,
+                        "CSB swept slabs: the rows do not fit the LDS of one workgroup per CU");
+        }
+        rc = PWM_OK;
+    }
     if (use_stream && csb_key_bits(V * S_stream * H, bits) <= 64) {
         // the slab streams, sorted on the device from the device CRS arrays
         rc = csb_stream_device(h, rb, cb, S_stream, wide);
@@ -361,6 +487,19 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
         c.nt = best_nt;
         c.n_tasks = units * best_g;
         if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(h, e, "CSB autotune");
+    }
+    if (swept_ms >= 0.0f) {
+        // swept slabs against the tuned slab streams / pieces: keep the faster
+        float ms = 0.0f;
+        if ((rc = time_csb(h, &ms)) != PWM_OK) return rc;
+        if (swept_ms < ms) {
+            c.ypart.release();
+            c.soff.release();
+            c.srows.release();
+            c.work.release();
+            c.blk_ptr.release();
+            swept.put(c);
+        }
     }
     // the CRS arrays are not needed by the CSB kernels
     h->rs.release();
@@ -660,7 +799,7 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
 }
 
 int64_t csb_partials(const pwm_matrix* h) {
-    return h->csb.groups > 1 ? blocked::csb_reduce_blocks(h->nor) : h->csb.n_tasks;
+    return h->csb.groups > 1 && !h->csb.sweep ? blocked::csb_reduce_blocks(h->nor) : h->csb.n_tasks;
 }
 int64_t coh_partials(const pwm_matrix* h) { return h->coh.n_tasks + h->coh.n_atomic; }
 
@@ -688,6 +827,11 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.nt = h->csb.nt;
     a.wide = h->csb.wide;
     a.srows = h->csb.srows.as<uint16_t>();
+    a.sweep = h->csb.sweep;
+    a.wb = h->csb.sweep_wb;
+    a.W = h->csb.sweep_W;
+    a.max_rows = h->csb.sweep_rows;
+    a.sstart = h->csb.sweep_start.as<int32_t>();
     hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream, part);
     if (e == hipSuccess && pw && part != 1)
         e = blocked::launch_blocked_finalize(h->vtype, (int)csb_partials(h), *pw, h->stream);
@@ -773,6 +917,68 @@ static int csb_stream_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t
     return PWM_OK;
 }
 
+// The stored CSB arrays of a swept-slab handle: every entry is decoded to
+// (row, column), bucketed by block (stably, so a duplicate pair keeps its
+// CRS order: both copies lie in one (slab, window) run) and each block put in
+// Z-Morton order; blk_ptr counts the blocks in block-row-major order.
+static int csb_sweep_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t* blk_ptr) {
+    const CsbDev& c = h->csb;
+    const int64_t T = c.n_tasks, W = c.sweep_W, nnz = h->nnz, H = c.H, V = c.V;
+    const int wb = c.sweep_wb, bits = c.bits;
+    const int64_t beta = (int64_t)1 << bits;
+    const uint32_t wmask = (uint32_t)(((uint64_t)1 << wb) - 1);
+    std::vector<int32_t> sstart(T + 1), woff(T * W + 1);
+    std::vector<uint32_t> sind(std::max<int64_t>(nnz, 1));
+    std::vector<double> sdat(std::max<int64_t>(nnz, 1));
+    hipError_t e = hipMemcpy(sstart.data(), c.sweep_start.p, (T + 1) * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(woff.data(), c.soff.p, woff.size() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nnz) e = hipMemcpy(sind.data(), c.ind.p, nnz * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nnz) {
+        DevBuf tmp;
+        e = tmp.alloc(nnz * 8);
+        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(c.data.p, h->vtype, tmp.as<double>(), nnz, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(sdat.data(), tmp.p, nnz * 8, hipMemcpyDeviceToHost);
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_csb_layout");
+    // decode in storage order, then count per block
+    std::vector<int64_t> blk(std::max<int64_t>(nnz, 1));
+    std::vector<uint32_t> lr(std::max<int64_t>(nnz, 1)), lc(std::max<int64_t>(nnz, 1));
+    parallel_for(T, 1, [&](int64_t t0, int64_t t1, int) {
+        for (int64_t t = t0; t < t1; ++t)
+            for (int64_t p = 0; p < W; ++p)
+                for (int64_t q = woff[t * W + p]; q < woff[t * W + p + 1]; ++q) {
+                    const int64_t row = sstart[t] + (int64_t)(sind[q] >> wb);
+                    const int64_t col = (p << wb) + (int64_t)(sind[q] & wmask);
+                    blk[q] = (row / beta) * H + col / beta;
+                    lr[q] = (uint32_t)(row % beta);
+                    lc[q] = (uint32_t)(col % beta);
+                }
+    });
+    std::vector<int64_t> cnt(H * V + 1, 0);
+    for (int64_t q = 0; q < nnz; ++q) cnt[blk[q] + 1]++;
+    for (int64_t b = 0; b < H * V; ++b) cnt[b + 1] += cnt[b];
+    for (int64_t b = 0; b <= H * V; ++b) blk_ptr[b] = (int32_t)cnt[b];
+    std::vector<int64_t> order(std::max<int64_t>(nnz, 1));
+    {
+        std::vector<int64_t> next(cnt.begin(), cnt.end() - 1);
+        for (int64_t q = 0; q < nnz; ++q) order[next[blk[q]]++] = q;
+    }
+    parallel_for(H * V, 64, [&](int64_t b0, int64_t b1, int) {
+        for (int64_t b = b0; b < b1; ++b) {
+            std::stable_sort(order.begin() + cnt[b], order.begin() + cnt[b + 1], [&](int64_t u, int64_t v) {
+                return zkey(lr[u], lc[u]) < zkey(lr[v], lc[v]);
+            });
+            for (int64_t i = cnt[b]; i < cnt[b + 1]; ++i) {
+                const int64_t q = order[i];
+                ind[i] = (lr[q] << 16) | lc[q];
+                data[i] = sdat[q];
+            }
+        }
+    });
+    return PWM_OK;
+}
+
 extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nblocks, uint32_t* ind, double* data,
                               int32_t* blk_ptr) {
     if (!h || !block_bits || !nblocks) return fail(h, PWM_ERR_INVALID, "pwm_csb_layout: NULL argument");
@@ -780,6 +986,7 @@ extern "C" int pwm_csb_layout(pwm_matrix* h, int32_t* block_bits, int64_t* nbloc
     *block_bits = h->csb.bits;
     *nblocks = h->csb.H * h->csb.V;
     if (!ind || !data || !blk_ptr) return PWM_OK;
+    if (h->csb.sweep) return csb_sweep_layout(h, ind, data, blk_ptr);
     if (h->csb.stream) return csb_stream_layout(h, ind, data, blk_ptr);
     hipError_t e = hipMemcpy(blk_ptr, h->csb.blk_ptr.p, (*nblocks + 1) * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && h->nnz) e = hipMemcpy(ind, h->csb.ind.p, h->nnz * 4, hipMemcpyDeviceToHost);

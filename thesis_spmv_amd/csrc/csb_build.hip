@@ -111,6 +111,52 @@ __global__ void k_csb_soff(const uint64_t* __restrict__ keys, int64_t nnz, int64
     }
 }
 
+// Swept slabs: entry k of row r, column c gets the key t * W + (c >> wb),
+// t the slab holding row r; a stable sort keeps CRS order (rows, then
+// columns, ascending) inside every (slab, window) run.
+struct SweepKeyArgs {
+    int64_t nor, nnz, T, W;
+    int wb;
+};
+
+__device__ __forceinline__ int64_t last_at_or_below(const int32_t* __restrict__ a, int64_t n, int64_t v) {
+    int64_t lo = 0, hi = n - 1;  // the last i in [0, n) with a[i] <= v (a[0] <= v)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_sweep_keys(SweepKeyArgs a, const int32_t* __restrict__ rs, const int32_t* __restrict__ ci,
+                             const int32_t* __restrict__ sstart, uint64_t* __restrict__ keys,
+                             uint32_t* __restrict__ pos) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = last_at_or_below(rs, a.nor, k);
+        const int64_t t = last_at_or_below(sstart, a.T, row);
+        keys[k] = (uint64_t)t * (uint64_t)a.W + (uint64_t)((uint32_t)ci[k] >> a.wb);
+        pos[k] = (uint32_t)k;
+    }
+}
+
+template <typename E>
+__global__ void k_sweep_gather(SweepKeyArgs a, const int32_t* __restrict__ rs, const int32_t* __restrict__ ci,
+                               const int32_t* __restrict__ sstart, const uint64_t* __restrict__ keys,
+                               const uint32_t* __restrict__ pos, const E* __restrict__ val,
+                               uint32_t* __restrict__ sind, E* __restrict__ sdat) {
+    const uint32_t wmask = (uint32_t)(((uint64_t)1 << a.wb) - 1);
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = pos[p];
+        const int64_t t = (int64_t)(keys[p] / (uint64_t)a.W);
+        const int64_t row = last_at_or_below(rs, a.nor, k);
+        sind[p] = ((uint32_t)(row - sstart[t]) << a.wb) | ((uint32_t)ci[k] & wmask);
+        sdat[p] = val[k];
+    }
+}
+
 static unsigned grid_of(int64_t n) {
     const int64_t b = (n + 255) / 256;
     return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
@@ -291,6 +337,121 @@ int csb_pieces_device(pwm_matrix* h, int k) {
     c.slabs = (int32_t)S;
     c.slab_rows = ((int64_t)1 << bits) >> k;
     c.pieces_per_block = (int32_t)np;
+    return PWM_OK;
+}
+
+int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int64_t max_rows, int64_t max_slabs,
+                     int wb) {
+    CsbDev& c = h->csb;
+    const int64_t nnz = h->nnz, nor = h->nor, noc = h->noc;
+    // nnz-balanced row slabs: a slab closes before the row that would take it
+    // past `target` entries (a longer row stands alone), or at max_rows rows.
+    // target starts at nnz / target_slabs (at least 4096 entries: a smaller
+    // slab would leave its workgroup mostly idle) and grows by a quarter
+    // while the cut needs more than max_slabs slabs
+    int64_t target = std::max<int64_t>(4096, (nnz + target_slabs - 1) / std::max<int64_t>(target_slabs, 1));
+    std::vector<int32_t> sstart;
+    for (;;) {
+        sstart.assign(1, 0);
+        bool over = false;
+        for (int64_t r = 0; r < nor && !over;) {
+            const int64_t base = rs[r];
+            int64_t lo = r + 1, hi = std::min(nor, r + max_rows);  // the last end in [r+1, hi] within target
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if ((int64_t)rs[mid] - base <= target)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            r = lo;
+            sstart.push_back((int32_t)r);
+            over = (int64_t)sstart.size() - 1 > max_slabs;
+        }
+        if (!over) break;
+        if (target >= nnz) return PWM_ERR_UNSUPPORTED;  // the row cap alone needs too many slabs
+        target += target / 4 + 1;
+    }
+    const int64_t T = (int64_t)sstart.size() - 1;
+    int32_t rows_max = 0;
+    for (int64_t t = 0; t < T; ++t) rows_max = std::max(rows_max, sstart[t + 1] - sstart[t]);
+    const int64_t W = std::max<int64_t>(1, (noc + ((int64_t)1 << wb) - 1) >> wb);
+    const int64_t cells = T * W;
+    const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
+    const size_t vs = value_size(h->vtype);
+    hipError_t e;
+    if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess ||
+        (e = c.soff.alloc((cells + 1) * 4)) != hipSuccess ||
+        (e = c.sweep_start.alloc(sstart.size() * 4)) != hipSuccess)
+        return hip_fail(h, e, "CSB sweep allocation");
+    if ((e = hipMemsetAsync(c.ind.p, 0, zpad * 4, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(c.sweep_start.p, sstart.data(), sstart.size() * 4, hipMemcpyHostToDevice, h->stream)) !=
+            hipSuccess)
+        return hip_fail(h, e, "CSB sweep allocation");
+    c.srows.release();
+    c.blk_ptr.release();
+    c.work.release();
+    int32_t last = (int32_t)nnz;
+    if (nnz > 0) {
+        const pwmk::SweepKeyArgs a{nor, nnz, T, W, wb};
+        DevBuf keys, keys_out, pos, pos_out, tmp;
+        if ((e = keys.alloc(nnz * 8)) != hipSuccess || (e = keys_out.alloc(nnz * 8)) != hipSuccess ||
+            (e = pos.alloc(nnz * 4)) != hipSuccess || (e = pos_out.alloc(nnz * 4)) != hipSuccess)
+            return hip_fail(h, e, "CSB sweep build allocation");
+        const dim3 g(pwmk::grid_of(nnz));
+        hipLaunchKernelGGL(pwmk::k_sweep_keys, g, dim3(256), 0, h->stream, a, h->rs.as<int32_t>(),
+                           h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys.as<uint64_t>(), pos.as<uint32_t>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep keys");
+        unsigned end_bit = 1;
+        while (end_bit < 64 && ((uint64_t)1 << end_bit) < (uint64_t)cells) ++end_bit;
+        size_t tb = 0;
+        e = rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
+                                      pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
+        if (e == hipSuccess) e = tmp.alloc(std::max<size_t>(tb, 4));
+        if (e == hipSuccess)
+            e = rocprim::radix_sort_pairs(tmp.p, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(),
+                                          pos.as<uint32_t>(), pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit,
+                                          h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "CSB sweep sort");
+        if (vs == 8)
+            hipLaunchKernelGGL(pwmk::k_sweep_gather<uint64_t>, g, dim3(256), 0, h->stream, a, h->rs.as<int32_t>(),
+                               h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint64_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint64_t>());
+        else if (vs == 4)
+            hipLaunchKernelGGL(pwmk::k_sweep_gather<uint32_t>, g, dim3(256), 0, h->stream, a, h->rs.as<int32_t>(),
+                               h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint32_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint32_t>());
+        else
+            hipLaunchKernelGGL(pwmk::k_sweep_gather<uint16_t>, g, dim3(256), 0, h->stream, a, h->rs.as<int32_t>(),
+                               h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys_out.as<uint64_t>(),
+                               pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(),
+                               c.data.as<uint16_t>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep gather");
+        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
+                           keys_out.as<uint64_t>(), nnz, cells, 0, c.soff.as<int32_t>());
+        if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(h->stream);  // before the temporaries go
+        if (e != hipSuccess) return hip_fail(h, e, "CSB sweep build");
+    } else {
+        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream, (const uint64_t*)nullptr,
+                           (int64_t)0, cells, 0, c.soff.as<int32_t>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep build");
+    }
+    if ((e = hipMemcpy((char*)c.soff.p + cells * 4, &last, 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(h, e, "CSB sweep build");
+    c.stream = 0;
+    c.wide = 0;
+    c.sweep = 1;
+    c.sweep_wb = wb;
+    c.sweep_W = W;
+    c.sweep_rows = std::max<int32_t>(rows_max, 1);
+    c.slabs = 1;
+    c.slab_rows = c.sweep_rows;
+    c.n_tasks = T;
+    c.groups = 1;
+    c.unroll = 2;
     return PWM_OK;
 }
 

@@ -396,6 +396,171 @@ int csb_reduce_blocks(int64_t nor) {
     return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);
 }
 
+// ---------------------------------------------------------------- CSB swept slabs
+
+// Swept slabs: one persistent workgroup of kSwBS threads per CU, each
+// owning an nnz-balanced row slab [sstart[t], sstart[t+1]) whose y-slice
+// stays in LDS for the whole kernel.  The slab's entries are stored window
+// by window: window p holds the columns [p << wb, (p+1) << wb) (1 MB of x by
+// default), and every workgroup walks the windows in the same order, adding
+// val * x[col] into its LDS rows (ds_add).  In window p, wave w takes the
+// chunks w, w + 16, ... of 64 x kSU groups of 4 entries of the slab's run;
+// the first chunk of the next window's entries is loaded while the current
+// one is gathered, and a workgroup barrier ends every window.  The barrier
+// keeps a workgroup's 16 waves on one window, and the nnz-balanced
+// workgroups, all started together, stay close enough that each XCD's L2
+// holds the windows its workgroups gather from; no partial sums are written
+// and read back.
+constexpr int kSwBS = 1024;
+
+template <typename VT, int kSU, bool NT>
+struct SweepChunk {
+    pwmk::vi4 w[kSU];
+    pwmk::Quad<VT> v[kSU];
+    // groups qb + lane + 64 u; a group past the run's last one (or of an
+    // empty run) reloads a valid group and its entries are masked
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ ind, const VT* __restrict__ val, int32_t qb,
+                                         int32_t q1, int lane) {
+#pragma unroll
+        for (int u = 0; u < kSU; ++u) {
+            const int32_t q = qb + lane + u * 64;
+            const int64_t qc = q < q1 ? q : (q1 > 0 ? q1 - 1 : 0);
+            if constexpr (NT) {
+                w[u] = __builtin_nontemporal_load(pwmk::G((const pwmk::vi4*)(ind + 4 * qc)));
+                v[u].load_nt(val + 4 * qc);
+            } else {
+                w[u] = *pwmk::G((const pwmk::vi4*)(ind + 4 * qc));
+                v[u].load(val + 4 * qc);
+            }
+        }
+    }
+};
+
+template <typename VT, int kSU, bool NT, typename X, typename Acc>
+__device__ __forceinline__ void sweep_add(const SweepChunk<VT, kSU, NT>& c, int32_t qb, int lane, int32_t e0, int32_t e1,
+                                          const X* __restrict__ xw, uint32_t wmask, int wb, Acc* ylds) {
+    // an entry outside [e0, e1) belongs to a neighbouring run (its column is
+    // in another window): it gathers the window's first column, which
+    // exists, and adds nothing
+    bool ok[kSU][4];
+    X xv[kSU][4];
+#pragma unroll
+    for (int u = 0; u < kSU; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t k = 4 * (int64_t)(qb + lane + u * 64) + j;
+            ok[u][j] = k >= e0 && k < e1;
+            xv[u][j] = xw[ok[u][j] ? ((uint32_t)c.w[u][j] & wmask) : 0u];
+        }
+#pragma unroll
+    for (int u = 0; u < kSU; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (ok[u][j]) atomicAdd(&ylds[(uint32_t)c.w[u][j] >> wb], (Acc)c.v[u].get(j) * (Acc)xv[u][j]);
+}
+
+template <typename VT, bool POWER, bool NT>
+__global__ __launch_bounds__(kSwBS) void k_csb_sweep(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
+                                                     typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
+    using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
+    constexpr int kSU = 2;
+    constexpr int kWv = kSwBS / 64;
+    constexpr int kChunk = 64 * kSU;  // groups of 4 entries per wave per chunk
+    extern __shared__ __align__(16) unsigned char csb_smem[];
+    Acc* ylds = (Acc*)csb_smem;
+    __shared__ double red[48];
+
+    const X* __restrict__ x = x_arg;
+    X* __restrict__ y = y_arg;
+    double scale = 1.0;
+    if constexpr (POWER) {
+        if (pw.ctrl[1]) return;
+        const int k = pw.ctrl[0];
+        x = (const X*)((k & 1) ? pw.buf1 : pw.buf0);
+        y = (X*)((k & 1) ? pw.buf0 : pw.buf1);
+        scale = pwmk::step_scale(pw, k);
+    }
+    const VT* __restrict__ val = (const VT*)a.val;
+    const uint32_t* __restrict__ ind = a.ind;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t t = blockIdx.x;
+    const int32_t r0 = a.sstart[t];
+    const int nrows = a.sstart[t + 1] - r0;
+    for (int i = tid; i < nrows; i += kSwBS) ylds[i] = 0;
+    const int wb = a.wb;
+    const uint32_t wmask = (uint32_t)(((uint64_t)1 << wb) - 1);
+    const int32_t* __restrict__ woff = a.soff + t * a.W;
+    SweepChunk<VT, kSU, NT> cur, nxt;
+    int32_t e0 = woff[0], e1 = woff[1];
+    int32_t q0 = e0 >> 2, q1 = (int32_t)(((int64_t)e1 + 3) >> 2);
+    cur.load(ind, val, q0 + wv * kChunk, q1, lane);
+    __syncthreads();
+    for (int64_t p = 0; p < a.W; ++p) {
+        const X* __restrict__ xw = x + (p << wb);
+        const int32_t pe0 = e0, pe1 = e1, pq0 = q0, pq1 = q1;
+        if (p + 1 < a.W) {  // the next window's run and its first chunk, loaded ahead
+            e0 = woff[p + 1];
+            e1 = woff[p + 2];
+            q0 = e0 >> 2;
+            q1 = (int32_t)(((int64_t)e1 + 3) >> 2);
+            nxt.load(ind, val, q0 + wv * kChunk, q1, lane);
+        }
+        sweep_add<VT, kSU, NT>(cur, pq0 + wv * kChunk, lane, pe0, pe1, xw, wmask, wb, ylds);
+        // the wave's further chunks of this window, if its share is larger
+        for (int32_t qb = pq0 + (wv + kWv) * kChunk; qb < pq1; qb += kWv * kChunk) {
+            SweepChunk<VT, kSU, NT> more;
+            more.load(ind, val, qb, pq1, lane);
+            sweep_add<VT, kSU, NT>(more, qb, lane, pe0, pe1, xw, wmask, wb, ylds);
+        }
+        __syncthreads();  // window p done by every wave
+        cur = nxt;
+    }
+    double yy = 0.0, xy = 0.0, xx = 0.0;
+    for (int i = tid; i < nrows; i += kSwBS) {
+        const int64_t row = (int64_t)r0 + i;
+        if constexpr (POWER) {
+            const X out = (X)((double)ylds[i] * scale);
+            y[row] = out;
+            const double yd = (double)out, xd = (double)x[row] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        } else {
+            y[row] = (X)ylds[i];
+        }
+    }
+    if constexpr (POWER) {
+        pwmk::block_sum3<kSwBS>(yy, xy, xx, red);
+        if (tid == 0) {
+            pw.partials[3 * t + 0] = yy;
+            pw.partials[3 * t + 1] = xy;
+            pw.partials[3 * t + 2] = xx;
+        }
+    }
+}
+
+template <typename VT>
+static int sweep_blocks_t(int rows) {
+    using Acc = typename Arith<VT>::A;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_csb_sweep<VT, false, true>, kSwBS,
+                                                     (size_t)rows * sizeof(Acc)) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+
+int csb_sweep_blocks_per_cu(int vtype, int rows) {
+    switch (vtype) {
+        case 0: return sweep_blocks_t<double>(rows);
+        case 1: return sweep_blocks_t<float>(rows);
+        case 2: return sweep_blocks_t<bf16_t>(rows);
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------- BlockCOH
 
 // One workgroup per task (CohTask): chunks of kCohChunk consecutive entries
@@ -648,6 +813,18 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
     if (a.n_tasks <= 0) return hipSuccess;
+    if (a.sweep) {
+        if (part == 2) return hipSuccess;  // no column-group reduction
+        const size_t lds = (size_t)a.max_rows * sizeof(Acc);
+        auto kp = a.nt ? k_csb_sweep<VT, true, true> : k_csb_sweep<VT, true, false>;
+        auto kf = a.nt ? k_csb_sweep<VT, false, true> : k_csb_sweep<VT, false, false>;
+        if (pw)
+            hipLaunchKernelGGL(kp, dim3((unsigned)a.n_tasks), dim3(kSwBS), lds, s, a, (const X*)nullptr, (X*)nullptr,
+                               *pw);
+        else
+            hipLaunchKernelGGL(kf, dim3((unsigned)a.n_tasks), dim3(kSwBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
+        return hipGetLastError();
+    }
     if (part == 2) {
         // the column-group reduction only
     } else if (a.stream) {

@@ -187,7 +187,8 @@ def bench_spmv(args, workload, cpu=True):
     wanted = set(args.formats.split(","))
     csb_opts = {"block_bits": 14} if workload == "stencil9" else {}
     csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll),
-                                       ("csb_layout", args.csb_layout), ("csb_window_bits", args.csb_window_bits))
+                                       ("csb_layout", args.csb_layout), ("csb_window_bits", args.csb_window_bits),
+                                       ("csb_sync_every", args.csb_sync_every))
                      if v})
     if args.csb_nt >= 0:
         csb_opts["csb_nt"] = args.csb_nt
@@ -732,6 +733,41 @@ def bench_serve(args):
     return out
 
 
+# ---------------------------------------------------------------- gather probe
+
+GATHER_KINDS = ((0, "plain global load"), (1, "buffer load"), (2, "buffer load sc0"), (3, "buffer load nt"),
+                (4, "buffer load sc0 nt"), (17, "buffer load sc1"), (18, "buffer load sc0 sc1"),
+                (19, "buffer load sc1 nt"), (20, "buffer load sc0 sc1 nt"))
+
+
+def bench_gather(args):
+    """The gather-rate probe (pwm_gather_rate) on the power-law workload's x
+    (N = 4 194 304 doubles) with nnz = 67 081 149 entries, for every way of
+    loading the gathered x values: plain global loads and buffer loads with
+    each combination of the sc0 / sc1 / nt cache-policy bits; windows of all
+    of x (a row-ordered CRS sweep) and of an eighth of x per XCD."""
+    import ctypes as C
+    import thesis_spmv_amd as pw
+    n, z = 4194304, 67081149
+    x = pw.DeviceVector.from_array(pw.gen_uniform(n, 0))
+    m_eff = z // 8192 * 8192
+    rows = []
+    for kind, name in GATHER_KINDS:
+        for per_xcd, window in ((0, n), (1, n // 8)):
+            ms = C.c_double()
+            pw._check(pw.lib().pwm_gather_rate(x.ptr, n, window, per_xcd | (kind << 8), z, C.byref(ms)))
+            rate = m_eff / (ms.value * 1e-3)
+            rows.append({"kind": kind, "loads": name, "window": "eighth of x per XCD" if per_xcd else "all of x",
+                         "us_for_nnz": round(z / rate * 1e6, 1), "Ggathers_per_s": round(rate / 1e9, 1)})
+            log(f"# {name:24s} {rows[-1]['window']:20s}: {rows[-1]['us_for_nnz']:8.1f} us for nnz, "
+                f"{rows[-1]['Ggathers_per_s']:6.1f} G gathers/s")
+    return {"metric": METRIC, "value": max(r["Ggathers_per_s"] for r in rows), "unit": "G gathers/s", "n_gpus": 1,
+            "steps": 10, "warmup": 2, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "gather-rate probe on the power-law x, every gather load kind", "entries": z},
+            "kinds": rows}
+
+
 # ---------------------------------------------------------------- COO -> CRS
 
 def bench_coo(args):
@@ -794,7 +830,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="all",
                     help="all (power-law line + configs 0, 2, 3, 4) or one of powerlaw, poisson, stencil9, batch, "
-                         "serve, coo")
+                         "serve, coo, gather")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true", help="skip the Infinity-Cache-flushed timing")
     ap.add_argument("--formats", default="crs,merge_path,csb,blockcoh,csb_ref_beta")
@@ -805,6 +841,7 @@ def main():
     ap.add_argument("--csb-layout", type=int, default=0,
                     help="CSB layout: 0 = automatic, 1 slab pieces, 2 slab streams, 3 wide streams, 4 swept slabs")
     ap.add_argument("--csb-window-bits", type=int, default=0, help="swept slabs: log2 columns per window (0 = 1 MB)")
+    ap.add_argument("--csb-sync-every", type=int, default=0, help="swept slabs: windows between workgroup barriers")
     ap.add_argument("--csb-nt", type=int, default=-1,
                     help="CSB stream kernel non-temporal entry loads: 1, 0, or -1 = time both at load")
     ap.add_argument("--crs-nt", type=int, default=-1,
@@ -844,7 +881,7 @@ def main():
         pw._check(pw.lib().pwm_device_copy(dst.ptr, src.ptr, nb))
         log(f"# calibration copy: {nb} bytes read and written once")
         del src, dst
-    single = {"batch": bench_batch, "serve": bench_serve, "coo": bench_coo}
+    single = {"batch": bench_batch, "serve": bench_serve, "coo": bench_coo, "gather": bench_gather}
     if args.workload in single:
         out = single[args.workload](args)
     elif args.workload in ("powerlaw", "poisson", "stencil9"):

@@ -206,6 +206,22 @@ int pwm_power_method_device_noalloc(pwm_matrix* h, void* x_dev, void* y_dev, int
  * SOA_Implementations/CRS_Merge.hpp:96-112).  Call with a == NULL to get P. */
 int pwm_merge_splits(pwm_matrix* h, int64_t* P, int64_t* a, int64_t* b);
 
+/* TripletToMultipleCRS (Util/TripletToCRS.hpp:309-385) of a loaded CRS or
+ * CRS_MERGE handle: the matrix as `partitions` CRS matrices of consecutive
+ * row ranges, the static row plan of the reference's partitioned classes
+ * and the row shards a multi-GPU power method would place one per GPU.
+ * Partition p holds rows [first_rows[p], first_rows[p] + thread_rows[p]):
+ * nor / partitions rows each (integer division), the remainder in the last
+ * (:320-330).  pwm_crs_partitions writes the plan (arrays of `partitions`
+ * entries, any may be NULL) and each partition's nonzero count;
+ * pwm_crs_partition_export writes partition `part`'s local CRS arrays
+ * (row_start[thread_rows+1] starting at 0, col_ind/values[part_nnz], values
+ * widened to double).  A partition without rows or entries is empty (the
+ * reference writes into its zero-length arrays). */
+int pwm_crs_partitions(pwm_matrix* h, int partitions, int32_t* first_rows, int32_t* thread_rows, int64_t* part_nnz);
+int pwm_crs_partition_export(pwm_matrix* h, int partitions, int part, int32_t* row_start, int32_t* col_ind,
+                             double* values);
+
 /* CRS arrays as held on the device (row_start[nor+1], col_ind[nnz], values
  * widened to double), for bit-exact checks of the device-side builders
  * (generatePoissonMatrix: Util/Poisson.hpp:32-83; loadFromTriplets:
@@ -268,7 +284,11 @@ int pwm_time_mv_rotating(pwm_matrix* const* hs, const void* const* xs, void* con
  * per_xcd from the window [j*window, (j+1)*window) of the XCD j that runs the
  * workgroup.  Indices are hashed, uniform in the window.  The bound a sparse
  * SpMV's gathers meet: n nonzeros cannot take less than n / this rate.
- * window = 0: the stream alone (x_dev may be NULL), its streaming ceiling. */
+ * window = 0: the stream alone (x_dev may be NULL), its streaming ceiling.
+ * per_xcd bits 8..15 choose how the gathers load x: 0 = plain global loads;
+ * 1 + c = bounds-checked buffer loads with cache-policy bits c (1 sc0, 2 nt,
+ * 16 sc1; c in 0..3 or 16..19), to measure how each policy fetches a line
+ * that misses the L2. */
 int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n, double* ms);
 
 /* ---- generators for the benchmark workloads (host side, deterministic) -

@@ -73,6 +73,8 @@ def _load(path):
         "orc_uniform_real": (C.c_double, [u32p, C.c_double, C.c_double]),
         "orc_triplet_to_crs": (None, [I64, I64, i32p, i32p, f64p, i32p, i32p, f64p]),
         "orc_row_partitions": (None, [I64, C.c_int, i32p, i32p]),
+        "orc_triplet_to_multiple_crs": (None, [I64, I64, i32p, i32p, f64p, C.c_int, i32p, i32p, i64p, i32p, i32p,
+                                               f64p]),
         "orc_crs_mv": (None, [I64, i32p, i32p, f64p, f64p, f64p]),
         "orc_crs_power_iteration": (None, [I64, i32p, i32p, f64p, f64p, f64p, I64]),
         "orc_crsomp_mv": (None, [C.c_int, I64, i32p, i32p, f64p, f64p, f64p]),
@@ -197,6 +199,23 @@ def triplet_to_crs(t):
 
 
 # ---------------------------------------------------------------- operators
+def triplet_to_multiple_crs(t, partitions):
+    """Util/TripletToCRS.hpp:309-385: a list of (first_row, CRSMatrix) per
+    partition, each CRS local to its rows."""
+    P = int(partitions)
+    fr, tr = np.zeros(P, np.int32), np.zeros(P, np.int32)
+    off = np.zeros(P + 1, np.int64)
+    rs = np.zeros(t.nor + P, np.int32)
+    ci = np.zeros(max(t.nnz, 1), np.int32)
+    dv = np.zeros(max(t.nnz, 1), np.float64)
+    lib().orc_triplet_to_multiple_crs(t.nor, t.nnz, t.rows, t.cols, t.vals, P, fr, tr, off, rs, ci, dv)
+    out = []
+    for p in range(P):
+        f, n = int(fr[p]), int(tr[p])
+        out.append((f, CRSMatrix(n, t.noc, rs[f + p:f + p + n + 1], ci[off[p]:off[p + 1]], dv[off[p]:off[p + 1]])))
+    return out
+
+
 def crs_mv(A, x):
     """Matrix/CRS.hpp:99-109."""
     y = np.zeros(A.nor, np.float64)

@@ -309,6 +309,31 @@ void orc_row_partitions(int64_t nor, int partitions, int32_t* first_rows, int32_
     }
 }
 
+/* Util/TripletToCRS.hpp:309-385.  The reference sorts all triplets by row,
+ * then for partition i (rows [first_rows[i], first_rows[i] + thread_rows[i]))
+ * collects its entries into local arrays: row_start[0] = 0, local row
+ * offsets counted from the partition's first entry, trailing empty rows
+ * filled with the partition's entry count, and each row sorted by column.
+ * That is the whole matrix's CRS cut at the partition boundaries with
+ * row_start rebased, which is how it is computed here (duplicates in input
+ * order, as orc_triplet_to_crs keeps them).  A partition without entries is
+ * empty here; the reference writes one entry into its zero-length arrays
+ * (:350-353), so it has no defined result there. */
+void orc_triplet_to_multiple_crs(int64_t nor, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                                 const double* vals, int partitions, int32_t* first_rows, int32_t* thread_rows,
+                                 int64_t* part_off, int32_t* row_start, int32_t* col_ind, double* data) {
+    int32_t* rs = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nor + 1));
+    orc_triplet_to_crs(nor, nnz, rows, cols, vals, rs, col_ind, data);
+    orc_row_partitions(nor, partitions, first_rows, thread_rows);
+    for (int p = 0; p < partitions; ++p) {
+        const int64_t f = first_rows[p], n = thread_rows[p];
+        part_off[p] = rs[f];
+        for (int64_t j = 0; j <= n; ++j) row_start[f + p + j] = rs[f + j] - rs[f];
+    }
+    part_off[partitions] = nnz;
+    free(rs);
+}
+
 /* ======================= CRS operators ================================= */
 
 /* Matrix/CRS.hpp:99-109: y = 0, then y[i] = y[i] + data[k]*x[col_ind[k]]

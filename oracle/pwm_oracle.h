@@ -68,6 +68,15 @@ void orc_triplet_to_crs(int64_t nor, int64_t nnz, const int32_t* rows, const int
  * first_rows[p], thread_rows[p] for p < partitions. */
 void orc_row_partitions(int64_t nor, int partitions, int32_t* first_rows, int32_t* thread_rows);
 
+/* TripletToMultipleCRS (TripletToCRS.hpp:309-385): the COO matrix as
+ * `partitions` CRS matrices of consecutive row ranges (orc_row_partitions).
+ * Partition p's arrays are row_start[first_rows[p] + p ..] (thread_rows[p]+1
+ * entries, starting at 0) and col_ind/data[part_off[p] .. part_off[p+1]).
+ * Input is not modified. */
+void orc_triplet_to_multiple_crs(int64_t nor, int64_t nnz, const int32_t* rows, const int32_t* cols,
+                                 const double* vals, int partitions, int32_t* first_rows, int32_t* thread_rows,
+                                 int64_t* part_off, int32_t* row_start, int32_t* col_ind, double* data);
+
 /* ---- L3 operators: CRS (the oracle) ----------------------------------- */
 void orc_crs_mv(int64_t nor, const int32_t* row_start, const int32_t* col_ind,
                 const double* data, const double* x, double* y);      /* CRS.hpp:99-109 */

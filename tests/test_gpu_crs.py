@@ -605,3 +605,35 @@ def test_poisson_vector_kernel_long_row_threshold_below_row_length():
             xr, yr = oracle.crs_power_iteration(A, np.ones(A.nor), 6)
             assert np.max(np.abs(xs - xr)) <= 1e-12 and np.max(np.abs(ys - yr)) <= 1e-12
 
+
+
+# ---------------------------------------------------------------- row partitions
+
+@pytest.mark.parametrize("cls", FORMATS, ids=lambda c: c.__name__)
+def test_triplet_to_multiple_crs_partitions(cls):
+    """TripletToMultipleCRS (Util/TripletToCRS.hpp:309-385) of a loaded
+    handle: the floor-rule row plan and every partition's local CRS arrays
+    bit-exact with the oracle's, for triplets with duplicates and empty rows,
+    and more partitions than rows."""
+    rng = np.random.default_rng(12)
+    nor, noc, nnz = 3001, 2500, 40000
+    r = rng.integers(0, nor, nnz).astype(np.int32)
+    r[(r % 11) == 5] = 17
+    c = rng.integers(0, noc, nnz).astype(np.int32)
+    r[:40], c[:40] = 9, 11
+    v = rng.standard_normal(nnz)
+    t = oracle.Triplets(nor, noc, r, c, v)
+    m = cls()
+    m.loadFromTriplets(t)
+    for P in (1, 2, 3, 16, 255, nor + 5):
+        ref = oracle.triplet_to_multiple_crs(t, P)
+        f, rows, z = m.partitions(P)
+        fr = np.zeros(P, np.int32)
+        tr = np.zeros(P, np.int32)
+        oracle.lib().orc_row_partitions(nor, P, fr, tr)
+        assert np.array_equal(f, fr) and np.array_equal(rows, tr)
+        assert z.tolist() == [A.nnz for _, A in ref]
+        for p in sorted({0, 1 % P, P // 2, P - 1}):
+            rs, ci, dv = m.partition_crs(P, p)
+            A = ref[p][1]
+            assert np.array_equal(rs, A.row_start) and np.array_equal(ci, A.col_ind) and np.array_equal(dv, A.data), (P, p)

@@ -240,6 +240,36 @@ def test_row_partitions():
     assert f.tolist() == [0, 2, 4, 6] and t.tolist() == [2, 2, 2, 4]
 
 
+@pytest.mark.parametrize("P", [1, 2, 3, 7, 64, 130])
+def test_triplet_to_multiple_crs(P):
+    """TripletToMultipleCRS (Util/TripletToCRS.hpp:309-385) against a brute
+    force: each partition's triplets filtered by row range and ordered by a
+    stable numpy lexsort on (row, column), row_start counted per local row;
+    partition sizes by the floor rule; P = 1 is the whole CRS."""
+    rng = np.random.default_rng(P)
+    nor, noc, nnz = 100, 80, 700
+    r = rng.integers(0, nor, nnz).astype(np.int32)
+    r[r % 9 == 4] = 0  # empty rows
+    c = rng.integers(0, noc, nnz).astype(np.int32)
+    r[:20], c[:20] = 7, 3  # duplicates
+    v = rng.standard_normal(nnz)
+    t = oracle.Triplets(nor, noc, r, c, v)
+    parts = oracle.triplet_to_multiple_crs(t, P)
+    am = nor // P
+    assert [f for f, _ in parts] == [min(p * am, nor) if p < P else 0 for p in range(P)]
+    assert [A.nor for _, A in parts] == [am] * (P - 1) + [nor - (P - 1) * am]
+    order = np.lexsort((c, r))  # stable: duplicates keep input order
+    for f, A in parts:
+        sel = order[(r[order] >= f) & (r[order] < f + A.nor)]
+        assert np.array_equal(A.col_ind, c[sel]) and np.array_equal(A.data, v[sel])
+        counts = np.bincount(r[sel] - f, minlength=A.nor) if A.nor else np.zeros(0, int)
+        assert np.array_equal(A.row_start, np.concatenate([[0], np.cumsum(counts)]))
+    if P == 1:
+        W = oracle.triplet_to_crs(t)
+        A = parts[0][1]
+        assert np.array_equal(A.row_start, W.row_start) and np.array_equal(A.col_ind, W.col_ind)
+
+
 # ------------------------------------------------------------------ merge path
 
 

@@ -60,6 +60,8 @@ _SIGS = {
     "pwm_power_method_device_noalloc": [VP, VP, VP, I64, F64, C.POINTER(F64), C.POINTER(F64), C.POINTER(I64)],
     "pwm_merge_splits": [VP, C.POINTER(I64), VP, VP],
     "pwm_crs_export": [VP, VP, VP, VP],
+    "pwm_crs_partitions": [VP, C.c_int, VP, VP, VP],
+    "pwm_crs_partition_export": [VP, C.c_int, C.c_int, VP, VP, VP],
     "pwm_csb_layout": [VP, C.POINTER(I32), C.POINTER(I64), VP, VP, VP],
     "pwm_blockcoh_band": [VP, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP],
     "pwm_device_alloc": [C.c_size_t, C.POINTER(VP)],
@@ -322,6 +324,23 @@ class SparseMatrixHip:
         st = Stats()
         _check(lib().pwm_matrix_stats(self.h, C.byref(st)), self.h)
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def partitions(self, partitions):
+        """CRS and CRS_Merge handles: TripletToMultipleCRS's row plan (Util/TripletToCRS.hpp:320-330):
+        (first_rows, thread_rows, nonzeros) per partition."""
+        P = int(partitions)
+        f, r, z = np.zeros(P, np.int32), np.zeros(P, np.int32), np.zeros(P, np.int64)
+        _check(lib().pwm_crs_partitions(self.h, P, _ptr(f), _ptr(r), _ptr(z)), self.h)
+        return f, r, z
+
+    def partition_crs(self, partitions, part):
+        """Partition `part`'s local CRS arrays (row_start, col_ind, values)."""
+        f, r, z = self.partitions(partitions)
+        rs = np.zeros(int(r[part]) + 1, np.int32)
+        ci = np.zeros(max(int(z[part]), 1), np.int32)
+        dv = np.zeros(max(int(z[part]), 1), np.float64)
+        _check(lib().pwm_crs_partition_export(self.h, int(partitions), int(part), _ptr(rs), _ptr(ci), _ptr(dv)), self.h)
+        return rs, ci[:int(z[part])], dv[:int(z[part])]
 
     def info(self):
         i = Info()

@@ -646,6 +646,10 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
         if (value != 0 && (value < 10 || value > 24))
             return fail(h, PWM_ERR_INVALID, "csb_window_bits must be 0 (1 MB of x) or in [10, 24]");
         h->csb_window_bits = (int32_t)value;
+    } else if (k == "csb_sync_every") {
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(h, PWM_ERR_INVALID, "csb_sync_every must be 1, 2, 4 or 8");
+        h->csb_sync_every = (int32_t)value;
     } else if (k == "csb_unroll") {
         if (value != 0 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_unroll must be 0, 2 or 4");
         h->csb_unroll = (int32_t)value;
@@ -1160,6 +1164,78 @@ extern "C" int pwm_power_method_device(pwm_matrix* h, void* x, void* y, int64_t 
     return rc;
 }
 
+// ---------------------------------------------------------------- row partitions
+
+// TripletToMultipleCRS (Util/TripletToCRS.hpp:309-385) over the device CRS:
+// partition p holds the rows [first_rows[p], first_rows[p] + thread_rows[p])
+// with am_rows = nor / partitions rows each (integer division, :320) and the
+// remainder in the last partition (:325-326).
+static void row_partitions(int64_t nor, int partitions, std::vector<int64_t>& first, std::vector<int64_t>& rows) {
+    first.assign(partitions, 0);
+    rows.assign(partitions, 0);
+    const int64_t am_rows = nor / partitions;
+    int64_t last = 0;
+    for (int p = 0; p < partitions; ++p) {
+        first[p] = last;
+        last = p == partitions - 1 ? nor : first[p] + am_rows;
+        rows[p] = last - first[p];
+    }
+}
+
+static int partition_args(pwm_matrix* h, int partitions, const char* who) {
+    if (!h) return fail(nullptr, PWM_ERR_INVALID, std::string(who) + ": NULL handle");
+    if (!h->loaded || !h->rs.p) return fail(h, PWM_ERR_STATE, std::string(who) + ": needs a loaded CRS or CRS_MERGE handle");
+    if (partitions < 1) return fail(h, PWM_ERR_INVALID, std::string(who) + ": partitions must be at least 1");
+    return PWM_OK;
+}
+
+extern "C" int pwm_crs_partitions(pwm_matrix* h, int partitions, int32_t* first_rows, int32_t* thread_rows,
+                                  int64_t* part_nnz) {
+    if (int rc = partition_args(h, partitions, "pwm_crs_partitions")) return rc;
+    std::vector<int64_t> first, rows;
+    row_partitions(h->nor, partitions, first, rows);
+    for (int p = 0; p < partitions; ++p) {
+        if (first_rows) first_rows[p] = (int32_t)first[p];
+        if (thread_rows) thread_rows[p] = (int32_t)rows[p];
+        if (part_nnz) {
+            int32_t b[2] = {0, 0};
+            hipError_t e = hipMemcpy(&b[0], h->rs.as<int32_t>() + first[p], 4, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) e = hipMemcpy(&b[1], h->rs.as<int32_t>() + first[p] + rows[p], 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return hip_fail(h, e, "pwm_crs_partitions");
+            part_nnz[p] = (int64_t)b[1] - b[0];
+        }
+    }
+    return PWM_OK;
+}
+
+extern "C" int pwm_crs_partition_export(pwm_matrix* h, int partitions, int part, int32_t* row_start, int32_t* col_ind,
+                                        double* values) {
+    if (int rc = partition_args(h, partitions, "pwm_crs_partition_export")) return rc;
+    if (part < 0 || part >= partitions || !row_start)
+        return fail(h, PWM_ERR_INVALID, "pwm_crs_partition_export: bad partition or NULL row_start");
+    std::vector<int64_t> first, rows;
+    row_partitions(h->nor, partitions, first, rows);
+    const int64_t f = first[part], n = rows[part];
+    hipError_t e = hipMemcpy(row_start, h->rs.as<int32_t>() + f, (n + 1) * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_crs_partition_export");
+    const int32_t base = row_start[0];
+    for (int64_t j = 0; j <= n; ++j) row_start[j] -= base;
+    const int64_t nz = row_start[n];
+    if (nz > 0 && (!col_ind || !values)) return fail(h, PWM_ERR_INVALID, "pwm_crs_partition_export: NULL arrays");
+    if (nz > 0) {
+        e = hipMemcpy(col_ind, h->ci.as<int32_t>() + base, nz * 4, hipMemcpyDeviceToHost);
+        DevBuf tmp;
+        if (e == hipSuccess) e = tmp.alloc(nz * 8);
+        if (e == hipSuccess)
+            e = pwmk::launch_copy_values_f64((const char*)h->val.p + (size_t)base * value_size(h->vtype), h->vtype,
+                                             tmp.as<double>(), nz, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e == hipSuccess) e = hipMemcpy(values, tmp.p, nz * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(h, e, "pwm_crs_partition_export");
+    }
+    return PWM_OK;
+}
+
 // ---------------------------------------------------------------- no-allocation power method
 
 // Workspace of pwm_power_method_device_noalloc: one fixed device array per
@@ -1304,7 +1380,11 @@ extern "C" int pwm_device_copy(void* dst, const void* src, int64_t bytes) {
 
 extern "C" int pwm_gather_rate(const void* x_dev, int64_t x_len, int64_t window, int per_xcd, int64_t n,
                                double* ms) {
-    if (!ms || window < 0 || window > UINT32_MAX || (window > 0 && (!x_dev || (per_xcd ? 8 * window : window) > x_len)))
+    const int kind = (per_xcd >> 8) & 0xff;
+    if (kind != 0 && (kind < 1 || (kind > 4 && kind < 17) || kind > 20 || window * 8 > INT32_MAX))
+        return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: unknown gather load kind (or window over 2 GB)");
+    if (!ms || window < 0 || window > UINT32_MAX ||
+        (window > 0 && (!x_dev || ((per_xcd & 1) ? 8 * window : window) > x_len)))
         return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: window must fit x (8 windows when per_xcd)");
     const int64_t m = pwmk::gather_probe_entries(n);
     if (m <= 0) return fail(nullptr, PWM_ERR_INVALID, "pwm_gather_rate: n must be at least 8192");

@@ -192,6 +192,7 @@ struct pwm_matrix {
     int32_t csb_groups = 0;  // 0 = autotune
     int32_t csb_layout = 0;  // 0 = automatic, 1 = pieces, 2 = streams, 3 = wide streams, 4 = swept slabs
     int32_t csb_window_bits = 0;  // swept slabs: log2 of the columns per window (0 = 1 MB of x)
+    int32_t csb_sync_every = 2;   // swept slabs: windows between workgroup barriers
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
     int32_t csb_build = 0;   // CSB slab pieces: 0 = built on the device, 1 = on the host

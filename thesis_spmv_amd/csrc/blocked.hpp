@@ -60,6 +60,7 @@ struct CsbArgs {
     int32_t wb = 0;
     int64_t W = 0;
     int32_t max_rows = 0;
+    int32_t sync_every = 1;  // windows between workgroup barriers (a power of two)
     const int32_t* sstart = nullptr;
 };
 

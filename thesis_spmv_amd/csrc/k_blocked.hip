@@ -513,9 +513,10 @@ __global__ __launch_bounds__(kSwBS) void k_csb_sweep(CsbArgs a, const typename A
             more.load(ind, val, qb, pq1, lane);
             sweep_add<VT, kSU, NT>(more, qb, lane, pe0, pe1, xw, wmask, wb, ylds);
         }
-        __syncthreads();  // window p done by every wave
+        if (((p + 1) & (a.sync_every - 1)) == 0) __syncthreads();  // windows up to p done by every wave
         cur = nxt;
     }
+    __syncthreads();
     double yy = 0.0, xy = 0.0, xx = 0.0;
     for (int i = tid; i < nrows; i += kSwBS) {
         const int64_t row = (int64_t)r0 + i;

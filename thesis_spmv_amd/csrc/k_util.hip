@@ -1,5 +1,7 @@
 // Utility kernels: device Poisson generator, value conversion, vector fills
 // and copies, the power method's final normalisation, cache flush.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace pwmk {
@@ -196,11 +198,16 @@ __global__ void k_gather_fill(uint32_t* __restrict__ idx, double* __restrict__ v
 constexpr int kProbeU = 8;
 constexpr int kProbePerBlock = 8192;  // entries per workgroup, a multiple of 256 * kProbeU
 
-template <bool PER_XCD, bool GATHER>
+// AUX < 0: the gathers are plain global loads; else buffer loads (bounds
+// checked against the x window) with cache-policy bits AUX: 1 sc0, 2 nt,
+// 16 sc1.
+template <bool PER_XCD, bool GATHER, int AUX = -1>
 __global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict__ idx, const double* __restrict__ val,
                                                       const double* __restrict__ x, double* __restrict__ out,
                                                       int64_t window) {
     const double* xw = PER_XCD ? x + (int64_t)(blockIdx.x % 8) * window : x;
+    __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)xw, (short)0, (int)std::min<int64_t>(window * 8, INT32_MAX), 0x00020000);
     const int64_t base = (int64_t)blockIdx.x * kProbePerBlock;
     double acc = 0.0;
     for (int i = threadIdx.x; i < kProbePerBlock; i += kProbeU * 256) {
@@ -213,7 +220,16 @@ __global__ __launch_bounds__(256) void k_gather_probe(const uint32_t* __restrict
         }
         double xv[kProbeU];
 #pragma unroll
-        for (int u = 0; u < kProbeU; ++u) xv[u] = GATHER ? xw[j[u]] : (double)j[u];
+        for (int u = 0; u < kProbeU; ++u) {
+            if constexpr (!GATHER) {
+                xv[u] = (double)j[u];
+            } else if constexpr (AUX < 0) {
+                xv[u] = xw[j[u]];
+            } else {
+                const auto w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)(j[u] * 8u), 0, AUX);
+                __builtin_memcpy(&xv[u], &w, 8);
+            }
+        }
 #pragma unroll
         for (int u = 0; u < kProbeU; ++u) acc += v[u] * xv[u];
     }
@@ -227,16 +243,36 @@ hipError_t launch_gather_fill(uint32_t* idx, double* val, int64_t n, uint32_t wi
     return hipGetLastError();
 }
 
+template <int AUX>
+static void launch_probe_aux(unsigned blocks, const uint32_t* idx, const double* val, const double* x, double* out,
+                             int64_t window, bool per_xcd, hipStream_t s) {
+    if (per_xcd)
+        hipLaunchKernelGGL((k_gather_probe<true, true, AUX>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+    else
+        hipLaunchKernelGGL((k_gather_probe<false, true, AUX>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+}
+
 hipError_t launch_gather_probe(const uint32_t* idx, const double* val, const double* x, double* out, int64_t n,
-                               int64_t window, int per_xcd, hipStream_t s) {
+                               int64_t window, int flags, hipStream_t s) {
     const unsigned blocks = (unsigned)(n / kProbePerBlock);
     if (blocks == 0) return hipSuccess;
-    if (window == 0)
+    const bool per_xcd = flags & 1;
+    if (window == 0) {
         hipLaunchKernelGGL((k_gather_probe<false, false>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
-    else if (per_xcd)
-        hipLaunchKernelGGL((k_gather_probe<true, true>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
-    else
-        hipLaunchKernelGGL((k_gather_probe<false, true>), dim3(blocks), dim3(256), 0, s, idx, val, x, out, window);
+        return hipGetLastError();
+    }
+    switch ((flags >> 8) & 0xff) {  // gather load kind
+        case 0: launch_probe_aux<-1>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 1: launch_probe_aux<0>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 2: launch_probe_aux<1>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 3: launch_probe_aux<2>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 4: launch_probe_aux<3>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 17: launch_probe_aux<16>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 18: launch_probe_aux<17>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 19: launch_probe_aux<18>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        case 20: launch_probe_aux<19>(blocks, idx, val, x, out, window, per_xcd, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

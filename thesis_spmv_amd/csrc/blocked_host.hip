@@ -198,12 +198,12 @@ static int build_swept(pwm_matrix* h, const int32_t* rs) {
         while (((int64_t)acc << (wb + 1)) <= ((int64_t)1 << 20)) ++wb;
         // the packed index keeps 32 - wb bits for the row in the slab: narrow
         // the window while the slabs' expected rows (a fifth of slack) need more
-        const int64_t rows_needed = (h->nor + cus - 1) / std::max(1, cus * 24 / 25) * 6 / 5;
+        const int64_t rows_needed = (h->nor + cus - 1) / std::max(1, cus) * 6 / 5;
         while (wb > 12 && (((int64_t)1 << (32 - wb)) - 1) < rows_needed) --wb;
     }
     const int64_t rows_cap = std::min<int64_t>(((lds_cu - 1024) / acc) & ~(int64_t)63, ((int64_t)1 << (32 - wb)) - 1);
     if (rows_cap <= 0 || blocked::csb_sweep_blocks_per_cu(h->vtype, (int)rows_cap) < 1) return PWM_ERR_UNSUPPORTED;
-    const int rc = csb_sweep_device(h, rs, std::max(1, cus * 24 / 25), rows_cap, cus, wb);
+    const int rc = csb_sweep_device(h, rs, std::max(1, cus), rows_cap, cus, wb);
     if (rc != PWM_OK) return rc;
     c.nt = h->csb_nt >= 0 ? h->csb_nt : 1;
     return PWM_OK;

@@ -347,8 +347,8 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
     // nnz-balanced row slabs: a slab closes before the row that would take it
     // past `target` entries (a longer row stands alone), or at max_rows rows.
     // target starts at nnz / target_slabs (at least 4096 entries: a smaller
-    // slab would leave its workgroup mostly idle) and grows by a quarter
-    // while the cut needs more than max_slabs slabs
+    // slab would leave its workgroup mostly idle) and grows by 1/64 while the
+    // cut needs more than max_slabs slabs
     int64_t target = std::max<int64_t>(4096, (nnz + target_slabs - 1) / std::max<int64_t>(target_slabs, 1));
     std::vector<int32_t> sstart;
     for (;;) {
@@ -370,7 +370,7 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
         }
         if (!over) break;
         if (target >= nnz) return PWM_ERR_UNSUPPORTED;  // the row cap alone needs too many slabs
-        target += target / 4 + 1;
+        target += target / 64 + 1;
     }
     const int64_t T = (int64_t)sstart.size() - 1;
     int32_t rows_max = 0;

@@ -433,12 +433,20 @@ def bench_batch(args):
     spd = [pw.gen_spd_powerlaw(int(n), seed=1000 + i) for i, n in enumerate(ns)]
     mixed = args.batch_values == "f32" and args.batch_bf16_exact
     opts = {"bf16_when_exact": 1} if mixed else {}
+    if args.batch_autotune:
+        opts["autotune"] = 1
     b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk, **opts)
     ids = [b.add_poisson(int(s), int(s)) for s in sides] + [b.add_crs(*m) for m in spd]
     n_inst, rows, nnz = b.size()
     log(f"# batch: {n_inst} instances, {rows} rows, {nnz} nnz, built in {time.time() - t0:.1f}s")
-    b.solve(5, -1.0, seed=0)  # warm-up (uploads the matrices)
+    b.solve(5, -1.0, seed=0)  # warm-up (uploads the matrices, chooses the tile schedule)
     pw.synchronize()
+    sched = b.schedule()
+    log(f"# batch tile schedule: 256 x {sched['items_per_thread']} items, "
+        f"{'persistent' if sched['persistent'] else 'plain'} grid"
+        + (" (autotuned: " + ", ".join(f"{k[0]}{'p' if k[1] else ''} {v:.3f} ms"
+                                        for k, v in sched["candidate_ms"].items()) + " per step)"
+           if sched["tuned"] else ""))
     t1 = time.perf_counter()
     lam, its, ms = b.solve(args.max_it, args.tol, seed=0)
     pw.synchronize()
@@ -454,7 +462,11 @@ def bench_batch(args):
                                   "SPD N log-U[1e3,2e5])", "instances": n_inst, "rows": rows, "nnz": nnz,
                       "max_it": args.max_it, "tol": args.tol},
            "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
-           "engine": b.stats()}
+           "engine": b.stats(),
+           "schedule": {"items_per_thread": sched["items_per_thread"], "persistent": sched["persistent"],
+                        "tuned": sched["tuned"],
+                        "candidate_ms_per_step": {f"{k[0]}{'p' if k[1] else ''}": round(v, 4)
+                                                  for k, v in sched["candidate_ms"].items()}}}
     # algorithmic bytes of every step actually taken (values, indices, row
     # offsets, x and y once per step)
     vb0 = {"f64": 8, "f32": 4, "bf16": 2}[args.batch_values]
@@ -856,6 +868,9 @@ def main():
     ap.add_argument("--batch-bf16-exact", type=int, default=0,
                     help="FP32 batch: store instances whose values are exact in BF16 as BF16 (1) or not (0)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="batch steps per graph replay")
+    ap.add_argument("--batch-autotune", type=int, default=1,
+                    help="1: the warm-up solve times tile sizes 256x8 / 256x16 with plain and persistent grids and "
+                         "keeps the fastest (overrides --batch-ipt)")
     ap.add_argument("--max-it", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-baseline", action="store_true")

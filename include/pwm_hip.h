@@ -358,10 +358,21 @@ int pwm_batch_size(const pwm_batch* b, int32_t* instances, int64_t* rows, int64_
  * "persistent" (0: one workgroup per tile instead of resident workgroups
  * that prefetch their next tile), "bf16_when_exact" (FP32 batches: 1 stores
  * each instance whose values are all exact in BF16 as BF16, the others as
- * FP32; vectors stay FP32). */
+ * FP32; vectors stay FP32), "autotune" (1: the first solve after the
+ * instances are uploaded first times 6 steps of the whole batch for
+ * items_per_thread 8 and 16, each with plain and persistent tile grids
+ * (plain only with bf16_when_exact), and keeps the fastest; it overrides
+ * "items_per_thread" and "persistent").  Results are bit-identical for a
+ * given schedule; schedules differ in the order of FP additions only. */
 int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value);
 /* Kernel launches, graph instantiations and work-list builds so far. */
 int pwm_batch_stats(const pwm_batch* b, int64_t* launches, int64_t* graphs, int64_t* lists);
+/* The tile schedule of the next solve; *tuned = 1 when the autotuner chose
+ * it, and then candidate_ms[4] (may be NULL) holds the ms per step it timed
+ * for (8, plain), (8, persistent), (16, plain), (16, persistent), -1 for a
+ * candidate not timed. */
+int pwm_batch_schedule(const pwm_batch* b, int32_t* items_per_thread, int32_t* persistent, int32_t* tuned,
+                       float* candidate_ms);
 
 /* ---- request server (addition) ----------------------------------------
  * Continuous batching of power-method jobs (stop at relative eigenvalue

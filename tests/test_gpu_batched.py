@@ -177,6 +177,42 @@ def test_tile_size_variants(ipt):
         assert abs(lam[i] - ref) <= 1e-10 * abs(ref), (ipt, i)
 
 
+@pytest.mark.parametrize("values,opt", [("f64", {}), ("f32", {}), ("f32", {"bf16_when_exact": 1})])
+def test_autotune_keeps_a_timed_schedule(values, opt):
+    """The first solve after the upload times the candidate tile schedules and
+    keeps the fastest; the results are those of a batch fixed to that
+    schedule, bit for bit, and a second solve does not tune again."""
+    rng = np.random.default_rng(12)
+    mats = _instances(rng)
+    b = _mixed_batch(values, mats, autotune=1, **opt)
+    assert not b.schedule()["tuned"]
+    lam, its, _ = b.solve(200, 1e-9, seed=5)
+    s = b.schedule()
+    print(f"{values} {opt}: schedule {s}", flush=True)
+    assert s["tuned"]
+    ms = s["candidate_ms"]
+    expect = {(8, False), (16, False)} if opt else {(8, False), (8, True), (16, False), (16, True)}
+    assert set(ms) == expect and all(v > 0 for v in ms.values())
+    assert ms[(s["items_per_thread"], s["persistent"])] == min(ms.values())
+    fixed = _mixed_batch(values, mats, items_per_thread=s["items_per_thread"], persistent=int(s["persistent"]), **opt)
+    lam2, its2, _ = fixed.solve(200, 1e-9, seed=5)
+    assert np.array_equal(lam, lam2) and np.array_equal(its, its2)
+    for i, A in enumerate(mats):
+        assert np.array_equal(b.vector(i, A.nor)[0], fixed.vector(i, A.nor)[0])
+    lam3, its3, _ = b.solve(200, 1e-9, seed=5)
+    assert b.schedule() == s
+    assert np.array_equal(lam, lam3) and np.array_equal(its, its3)
+    # against the oracle: K plain steps
+    K = 40
+    lam4, _, _ = b.solve(K, -1.0, seed=2)
+    tol = 1e-10 if values == "f64" else 1e-5  # FP32 row sums of 9000-entry rows
+    for i, A in enumerate(mats):
+        x0 = b.start_vector(i, A.nor, seed=2)
+        xr, yr = oracle.crs_power_iteration(A, x0, K - 1)
+        ref = _rq(A, xr if (K - 1) % 2 == 0 else yr)
+        assert abs(lam4[i] - ref) <= tol * abs(ref), (i, lam4[i], ref)
+
+
 def test_errors():
     b = pw.Batch("bf16")
     with pytest.raises(pw.PwmError):

@@ -97,6 +97,7 @@ _SIGS = {
     "pwm_batch_size": [VP, C.POINTER(I32), C.POINTER(I64), C.POINTER(I64)],
     "pwm_batch_set_option": [VP, C.c_char_p, I64],
     "pwm_batch_stats": [VP, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)],
+    "pwm_batch_schedule": [VP, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), VP],
     "pwm_server_create": [C.c_int, I64, I32, C.POINTER(VP)],
     "pwm_server_destroy": [VP],
     "pwm_server_last_error": [VP],
@@ -466,7 +467,9 @@ class Batch:
 
     Options: items_per_thread (4, 8, 16), chunk (steps per graph replay),
     graph (False: plain launches), persistent, bf16_when_exact (FP32 batches:
-    BF16 storage for the instances whose values are exact in BF16)."""
+    BF16 storage for the instances whose values are exact in BF16), autotune
+    (the first solve after an upload times the tile schedules and keeps the
+    fastest; schedule() reports the choice)."""
 
     def __init__(self, values="f32", **options):
         self.h = VP()
@@ -483,6 +486,19 @@ class Batch:
         a, b, c = I64(), I64(), I64()
         self._chk(lib().pwm_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
         return {"launches": a.value, "graphs": b.value, "list_builds": c.value}
+
+    def schedule(self):
+        """Tile schedule of the next solve; candidate_ms maps (ipt, persistent)
+        to the ms per step the autotuner timed (empty when it did not run)."""
+        ipt, per, tuned = I32(), I32(), I32()
+        ms = (C.c_float * 4)()
+        self._chk(lib().pwm_batch_schedule(self.h, C.byref(ipt), C.byref(per), C.byref(tuned), ms))
+        cands = {}
+        for k, key in enumerate([(8, False), (8, True), (16, False), (16, True)]):
+            if tuned.value and ms[k] >= 0:
+                cands[key] = float(ms[k])
+        return {"items_per_thread": ipt.value, "persistent": bool(per.value), "tuned": bool(tuned.value),
+                "candidate_ms": cands}
 
     def __del__(self):
         try:

@@ -37,9 +37,16 @@ struct BatchInst {
     // arena offsets
     size_t o_rs = 0, o_ci = 0, o_val = 0, o_tiles = 0, o_part = 0, o_carry = 0, o_vec = 0;
     int32_t n_tiles = 0;
+    int32_t cap_tiles = 0;  // tile slots in the arena (the autotuner re-plans within them)
     std::vector<pwmk::ETile> tiles;  // host copy of the plan
     int slot = -1;
 };
+
+// The autotuner's tile sizes: 256 x ipt merge items for ipt in kTuneIpts.
+constexpr int kTuneIpts[] = {8, 16};
+constexpr int kTuneMinIpt = 8;
+// Steps of every instance the autotuner times per candidate schedule.
+constexpr int64_t kTuneSteps = 6;
 
 }  // namespace
 
@@ -56,6 +63,9 @@ struct pwm_batch {
     int use_graph = 1;
     int persistent = 0;
     int bf16_exact = 0;     // FP32 batch: BF16 storage for instances exact in BF16
+    int autotune = 0;       // time the tile schedules at the first solve after an upload
+    bool tuned = false;     // the schedule in ipt / persistent was chosen by the autotuner
+    std::vector<float> tune_ms;  // ms per step of (ipt 8, 16) x (plain, persistent); -1 where not timed
     pwm_impl::Arena arena;
     pwm_impl::Engine* eng = nullptr;
     std::vector<double> lambda, ynorm;
@@ -178,6 +188,13 @@ extern "C" int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value
         b->persistent = value != 0;
         return PWM_OK;
     }
+    if (k == "autotune") {
+        if (b->eng && b->eng->active() != 0) return bfail(b, PWM_ERR_STATE, "pwm_batch_set_option: solve in flight");
+        b->autotune = value != 0;
+        b->tuned = false;
+        b->uploaded = false;  // the arena then keeps room for the smallest tiles
+        return PWM_OK;
+    }
     if (k == "bf16_when_exact") {
         if (value != 0 && b->vtype != PWM_VALUES_F32)
             return bfail(b, PWM_ERR_INVALID, "pwm_batch_set_option: bf16_when_exact needs an FP32 batch");
@@ -211,10 +228,11 @@ static int batch_upload(pwm_batch* b) {
         I.vt16 = b->bf16_exact && (I.kind == 1 || exact_in_bf16(I.val));
         const size_t vs = I.vt16 ? 2 : pwm_impl::value_size(b->vtype);
         I.n_tiles = pwmk::engine_tiles_for(I.n, I.nnz, b->ipt);
+        I.cap_tiles = b->autotune ? std::max(I.n_tiles, pwmk::engine_tiles_for(I.n, I.nnz, kTuneMinIpt)) : I.n_tiles;
         const int64_t zp = I.nnz + pwm_impl::kPadEntries;  // tiles read aligned groups of 4
         total += Arena::round((I.n + 1) * 4) + Arena::round(zp * 4) + Arena::round(zp * vs) +
-                 Arena::round((size_t)I.n_tiles * sizeof(pwmk::ETile)) + Arena::round((size_t)I.n_tiles * 24) +
-                 Arena::round((size_t)I.n_tiles * acc) + Arena::round(2 * I.n * xs);
+                 Arena::round((size_t)I.cap_tiles * sizeof(pwmk::ETile)) + Arena::round((size_t)I.cap_tiles * 24) +
+                 Arena::round((size_t)I.cap_tiles * acc) + Arena::round(2 * I.n * xs);
     }
     hipError_t e = b->arena.reserve(total);
     if (e != hipSuccess) return bhip(b, e, "batch allocation");
@@ -228,8 +246,9 @@ static int batch_upload(pwm_batch* b) {
         const size_t vs = pwm_impl::value_size(vt);
         const int64_t zp = I.nnz + pwm_impl::kPadEntries;
         bool ok = A.alloc((I.n + 1) * 4, &I.o_rs) && A.alloc(zp * 4, &I.o_ci) && A.alloc(zp * vs, &I.o_val) &&
-                  A.alloc((size_t)I.n_tiles * sizeof(pwmk::ETile), &I.o_tiles) && A.alloc((size_t)I.n_tiles * 24, &I.o_part) &&
-                  A.alloc((size_t)I.n_tiles * acc, &I.o_carry) && A.alloc(2 * I.n * xs, &I.o_vec);
+                  A.alloc((size_t)I.cap_tiles * sizeof(pwmk::ETile), &I.o_tiles) &&
+                  A.alloc((size_t)I.cap_tiles * 24, &I.o_part) && A.alloc((size_t)I.cap_tiles * acc, &I.o_carry) &&
+                  A.alloc(2 * I.n * xs, &I.o_vec);
         if (!ok) return bfail(b, PWM_ERR_NOMEM, "batch arena exhausted");
         int32_t* rs = (int32_t*)A.at(I.o_rs);
         e = hipMemsetAsync((char*)A.at(I.o_ci) + I.nnz * 4, 0, pwm_impl::kPadEntries * 4, b->stream);
@@ -258,6 +277,127 @@ static int batch_upload(pwm_batch* b) {
     if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return bhip(b, e, "batch upload");
     b->uploaded = true;
     b->planned_ipt = b->ipt;
+    b->tuned = false;
+    return PWM_OK;
+}
+
+// Re-plans every instance's tiles for `ipt` items per thread inside the tile
+// slots the upload reserved (the autotuner's candidates).
+static int batch_replan(pwm_batch* b, int ipt) {
+    hipError_t e = hipSuccess;
+    for (BatchInst& I : b->inst) {
+        const int32_t nt = pwmk::engine_tiles_for(I.n, I.nnz, ipt);
+        if (nt > I.cap_tiles) return bfail(b, PWM_ERR_STATE, "batch re-plan: tiles exceed the reserved slots");
+        I.n_tiles = nt;
+        e = pwmk::engine_plan_tiles((const int32_t*)b->arena.at(I.o_rs), I.n, I.nnz, ipt,
+                                    (pwmk::ETile*)b->arena.at(I.o_tiles), b->stream);
+        I.tiles.resize(nt);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(I.tiles.data(), b->arena.at(I.o_tiles), (size_t)nt * sizeof(pwmk::ETile),
+                               hipMemcpyDeviceToHost, b->stream);
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return bhip(b, e, "batch re-plan");
+    b->planned_ipt = ipt;
+    return PWM_OK;
+}
+
+// Hands every instance to the engine as a job of `max_it` steps (tol < 0: no
+// stop test).
+static void batch_add_jobs(pwm_batch* b, int64_t max_it, double tol) {
+    const size_t xs = pwm_impl::vector_size(b->vtype);
+    pwm_impl::Arena& A = b->arena;
+    for (BatchInst& I : b->inst) {
+        pwmk::EJob j;
+        memset(&j, 0, sizeof(j));
+        j.rs = (const int32_t*)A.at(I.o_rs);
+        j.ci = (const int32_t*)A.at(I.o_ci);
+        j.val = A.at(I.o_val);
+        j.buf0 = A.at(I.o_vec);
+        j.buf1 = (char*)A.at(I.o_vec) + I.n * xs;
+        j.tiles = (const pwmk::ETile*)A.at(I.o_tiles);
+        j.partials = (double*)A.at(I.o_part);
+        j.carry = A.at(I.o_carry);
+        j.n = I.n;
+        j.nnz = I.nnz;
+        j.max_it = max_it;
+        j.tol = tol;
+        j.n_tiles = I.n_tiles;
+        j.vt16 = I.vt16;
+        I.slot = b->eng->add(j, I.tiles.data());
+    }
+}
+
+static void batch_release_jobs(pwm_batch* b) {
+    for (BatchInst& I : b->inst) {
+        b->eng->release(I.slot);
+        I.slot = -1;
+    }
+}
+
+// Chooses the tile schedule: for 256 x ipt merge items per tile (ipt in
+// kTuneIpts) and plain or persistent tile grids, runs kTuneSteps steps of
+// every instance twice (the first run builds the lists and the graph) and
+// times the second; the fastest candidate's ipt and persistent flag are kept
+// in the batch and its tiles stay planned.  The timed steps start from the
+// seed-0 start vectors; the solve that follows resets them.
+static int batch_autotune(pwm_batch* b) {
+    pwm_impl::Engine& E = *b->eng;
+    struct Cand {
+        int ipt, persistent;
+    };
+    std::vector<Cand> cands;
+    for (int ipt : kTuneIpts) {
+        cands.push_back({ipt, 0});
+        if (!b->bf16_exact) cands.push_back({ipt, 1});  // the mixed-storage engine has no persistent grid
+    }
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < b->inst.size() && e == hipSuccess; ++i)
+        e = pwmk::engine_start_vector(b->arena.at(b->inst[i].o_vec), b->inst[i].n, b->vtype, 0, (uint64_t)i, b->stream);
+    if (e != hipSuccess) return bhip(b, e, "batch autotune start vectors");
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return bhip(b, e, "batch autotune");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return bhip(b, e, "batch autotune");
+    }
+    b->tune_ms.assign(2 * (sizeof(kTuneIpts) / sizeof(kTuneIpts[0])), -1.0f);
+    std::vector<float> ms_of(cands.size(), 0.0f);
+    int rc = PWM_OK, best = -1;
+    std::vector<int> fin;
+    std::string err;
+    for (size_t c = 0; c < cands.size() && rc == PWM_OK; ++c) {
+        if (b->planned_ipt != cands[c].ipt && (rc = batch_replan(b, cands[c].ipt)) != PWM_OK) break;
+        if ((rc = E.set_ipt(cands[c].ipt)) != PWM_OK) {
+            rc = bfail(b, rc, "batch autotune: items_per_thread");
+            break;
+        }
+        E.persistent = cands[c].persistent != 0;
+        batch_add_jobs(b, 2 * kTuneSteps, -1.0);
+        float ms = 0;
+        if ((rc = E.run((int)kTuneSteps, fin, err)) == PWM_OK) {
+            (void)hipEventRecord(e0, b->stream);
+            rc = E.run((int)kTuneSteps, fin, err);
+            (void)hipEventRecord(e1, b->stream);
+            if (rc == PWM_OK && (e = hipEventSynchronize(e1)) == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (rc == PWM_OK && e != hipSuccess) rc = bhip(b, e, "batch autotune timing");
+        } else {
+            rc = bfail(b, rc, err);
+        }
+        batch_release_jobs(b);
+        if (rc != PWM_OK) break;
+        ms_of[c] = ms / (float)kTuneSteps;
+        b->tune_ms[(c / (b->bf16_exact ? 1 : 2)) * 2 + cands[c].persistent] = ms_of[c];
+        if (best < 0 || ms_of[c] < ms_of[best]) best = (int)c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != PWM_OK) return rc;
+    b->ipt = cands[best].ipt;
+    b->persistent = cands[best].persistent;
+    if (b->planned_ipt != b->ipt && (rc = batch_replan(b, b->ipt)) != PWM_OK) return rc;
+    b->tuned = true;
     return PWM_OK;
 }
 
@@ -285,11 +425,11 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
         return bfail(b, PWM_ERR_STATE, "pwm_batch_solve: engine busy");
     }
     pwm_impl::Engine& E = *b->eng;
-    if (E.ipt() != b->ipt && (rc = E.set_ipt(b->ipt)) != PWM_OK) return bfail(b, rc, "pwm_batch_solve: items_per_thread");
     E.chunk = b->chunk;
     E.use_graph = b->use_graph != 0;
+    if (b->autotune && !b->tuned && (rc = batch_autotune(b)) != PWM_OK) return rc;
+    if (E.ipt() != b->ipt && (rc = E.set_ipt(b->ipt)) != PWM_OK) return bfail(b, rc, "pwm_batch_solve: items_per_thread");
     E.persistent = b->persistent != 0;
-    const size_t xs = pwm_impl::vector_size(b->vtype);
     pwm_impl::Arena& A = b->arena;
     hipError_t e = hipSuccess;
     for (int i = 0; i < ni && e == hipSuccess; ++i) {
@@ -301,26 +441,7 @@ extern "C" int pwm_batch_solve(pwm_batch* b, int64_t max_it, double tol, uint64_
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, b->stream);
-    for (int i = 0; i < ni; ++i) {
-        BatchInst& I = b->inst[i];
-        pwmk::EJob j;
-        memset(&j, 0, sizeof(j));
-        j.rs = (const int32_t*)A.at(I.o_rs);
-        j.ci = (const int32_t*)A.at(I.o_ci);
-        j.val = A.at(I.o_val);
-        j.buf0 = A.at(I.o_vec);
-        j.buf1 = (char*)A.at(I.o_vec) + I.n * xs;
-        j.tiles = (const pwmk::ETile*)A.at(I.o_tiles);
-        j.partials = (double*)A.at(I.o_part);
-        j.carry = A.at(I.o_carry);
-        j.n = I.n;
-        j.nnz = I.nnz;
-        j.max_it = max_it;
-        j.tol = tol;
-        j.n_tiles = I.n_tiles;
-        j.vt16 = I.vt16;
-        I.slot = E.add(j, I.tiles.data());
-    }
+    batch_add_jobs(b, max_it, tol);
     std::vector<int> fin;
     std::string err;
     const int64_t max_runs = (max_it + E.chunk - 1) / E.chunk + 1;
@@ -423,5 +544,18 @@ extern "C" int pwm_batch_stats(const pwm_batch* b, int64_t* launches, int64_t* g
     if (launches) *launches = b->eng ? b->eng->launches : 0;
     if (graphs) *graphs = b->eng ? b->eng->graph_builds : 0;
     if (lists) *lists = b->eng ? b->eng->list_builds : 0;
+    return PWM_OK;
+}
+
+// The tile schedule the next solve uses and, when the autotuner chose it, the
+// ms per step it timed for each candidate.
+extern "C" int pwm_batch_schedule(const pwm_batch* b, int32_t* items_per_thread, int32_t* persistent, int32_t* tuned,
+                                  float* candidate_ms) {
+    if (!b) return bfail(nullptr, PWM_ERR_INVALID, "pwm_batch_schedule: NULL");
+    if (items_per_thread) *items_per_thread = b->ipt;
+    if (persistent) *persistent = b->persistent;
+    if (tuned) *tuned = b->tuned ? 1 : 0;
+    if (candidate_ms)
+        for (int c = 0; c < 4; ++c) candidate_ms[c] = b->tuned ? b->tune_ms[c] : -1.0f;
     return PWM_OK;
 }

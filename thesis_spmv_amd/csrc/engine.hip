@@ -489,6 +489,10 @@ int Engine::set_ipt(int v) {
     if (v != 4 && v != 8 && v != 16) return PWM_ERR_INVALID;
     for (int s = 0; s < max_slots_; ++s)
         if (in_use_[s]) return PWM_ERR_STATE;
+    if (v != ipt_ && graph_) {  // the captured steps launch the old tile kernel
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
     ipt_ = v;
     return PWM_OK;
 }

@@ -78,6 +78,7 @@ typedef struct {
     int32_t unroll;            /* CSB slab-stream kernel: groups of 4 entries per thread per step */
     int32_t non_temporal;      /* CSB slab streams / CRS tiles: 1 = non-temporal entry loads */
     int32_t layout;            /* CSB: 1 slab pieces, 2 slab streams, 3 wide slab streams, 4 swept slabs */
+    int32_t kernels;           /* kernel launches of one pwm_mv_device */
 } pwm_info_t;
 
 /* Constructor SparseMatrix(int threads) (Matrix/SparseMatrix.hpp:30) through

@@ -924,6 +924,15 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
     out->balanced = h->format == PWM_FORMAT_CSB ? h->csb.balanced : 0;
     out->unroll = h->format == PWM_FORMAT_CSB && h->csb.stream ? h->csb.unroll : 0;
     out->layout = h->format != PWM_FORMAT_CSB ? 0 : h->csb.sweep ? 4 : !h->csb.stream ? 1 : h->csb.wide ? 3 : 2;
+    switch (h->format) {
+        case PWM_FORMAT_CRS:
+            out->kernels = 1 + (h->crs_used == 2 ? (h->tiles_carry ? 1 : 0) : (h->lr.n_rows > 0 ? 1 : 0));
+            break;
+        case PWM_FORMAT_CRS_MERGE: out->kernels = 2; break;
+        case PWM_FORMAT_CSB: out->kernels = h->csb.sweep ? 1 : 1 + (h->csb.groups > 1 ? 1 : 0); break;
+        case PWM_FORMAT_BLOCKCOH: out->kernels = 1 + (h->coh.n_atomic > 0 ? 1 : 0); break;
+    }
+    if (h->format == PWM_FORMAT_CSB && h->csb.n_tasks <= 0) out->kernels = 0;
     out->non_temporal = h->format == PWM_FORMAT_CSB                         ? h->csb.nt
                         : (h->format == PWM_FORMAT_CRS && h->crs_used == 2) ? h->crs_nt_used
                                                                            : 0;

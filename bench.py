@@ -617,7 +617,7 @@ def bench_serve(args):
             srv.add_matrix(*p["crs"], borrow=True)
 
     def run_server(reqs):
-        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph)
         resident_ids = {}  # pool index -> list of server matrix ids
         lat, jobs, pending, k = [], {}, list(reqs), 0
         start = time.perf_counter()
@@ -686,7 +686,7 @@ def bench_serve(args):
         rounding errors; lambda itself can sit near 0 on the nonsymmetric
         power-law matrices), relative where lambda_64 is the dominant scale,
         and the max-norm difference of the normalised final vectors."""
-        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk)
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph)
         mids = {}
         jobs = {}
         for r in reqs:
@@ -892,6 +892,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
     ap.add_argument("--serve-max-n", type=float, default=1e7)
     ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per pwm_server_step")
+    ap.add_argument("--serve-graph", type=int, default=1, help="0: the server launches its steps one by one "
+                    "instead of replaying them as hipGraphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "

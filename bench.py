@@ -106,6 +106,11 @@ def _host_cores():
         return os.cpu_count() or 1
 
 
+# read before OpenMP starts: with OMP_PROC_BIND set, libgomp pins the calling
+# thread to one place, and the affinity mask then shows a single core
+HOST_CORES_AT_START = _host_cores()
+
+
 def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=5.0, power_budget_s=6.0, y_gpu=None):
     """The oracle's restatements of CRSOMP::mv (Env_Implementations/CRSOMP.hpp:110-123,
     OpenMP dynamic,512) and CRS_Merge::mv (SOA_Implementations/CRS_Merge.hpp:95-141,
@@ -154,7 +159,7 @@ def cpu_baseline(n, rs, ci, dv, x, bytes_per_mv, threads, budget_s=5.0, power_bu
         y_ref = oracle.crs_mv(A, xs)
         err = float(np.max(np.abs(y_gpu - y_ref)) / max(float(np.max(np.abs(y_ref))), 1e-300))
     return {"value": round(bytes_per_mv / best / 1e9, 2), "unit": "GB/s", "cores": threads, "kind": "port",
-            "host_cores_visible": _host_cores(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "host_cores_visible": HOST_CORES_AT_START, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
             "gpu_y_rel_err_vs_oracle": err,
             "sample": f"median of {k_omp} CRSOMP::mv and {k_mrg} CRS_Merge::mv (P={threads}) full SpMVs of the same "
                       f"matrix after {w_omp} / {w_mrg} warm-ups: {t_omp * 1e3:.3f} ms and {t_mrg * 1e3:.3f} ms "

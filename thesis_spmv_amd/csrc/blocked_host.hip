@@ -202,8 +202,12 @@ static int build_swept(pwm_matrix* h, const int32_t* rs) {
         while (wb > 12 && (((int64_t)1 << (32 - wb)) - 1) < rows_needed) --wb;
     }
     const int64_t rows_cap = std::min<int64_t>(((lds_cu - 1024) / acc) & ~(int64_t)63, ((int64_t)1 << (32 - wb)) - 1);
-    if (rows_cap <= 0 || blocked::csb_sweep_blocks_per_cu(h->vtype, (int)rows_cap) < 1) return PWM_ERR_UNSUPPORTED;
+    if (rows_cap <= 0 || blocked::csb_sweep_blocks_per_cu(h->vtype, (int)rows_cap) < 1)
+        return fail(h, PWM_ERR_UNSUPPORTED, "CSB swept slabs: no workgroup with LDS rows fits a CU");
     const int rc = csb_sweep_device(h, rs, std::max(1, cus), rows_cap, cus, wb);
+    if (rc == PWM_ERR_UNSUPPORTED)
+        return fail(h, rc, "CSB swept slabs: the rows do not fit " + std::to_string(cus) + " slabs of at most " +
+                               std::to_string(rows_cap) + " rows (the LDS of one workgroup per CU)");
     if (rc != PWM_OK) return rc;
     c.nt = h->csb_nt >= 0 ? h->csb_nt : 1;
     return PWM_OK;
@@ -285,11 +289,8 @@ int build_csb(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double*
             }
             if ((rc = time_csb(h, &swept_ms)) != PWM_OK) return rc;
             swept.take(c);
-        } else if (rc != PWM_ERR_UNSUPPORTED) {
-            return rc;
-        } else if (h->csb_layout == 4) {
-            return fail(h, PWM_ERR_UNSUPPORTED,
-                        "CSB swept slabs: the rows do not fit the LDS of one workgroup per CU");
+        } else if (rc != PWM_ERR_UNSUPPORTED || h->csb_layout == 4) {
+            return rc;  // with the message of build_swept
         }
         rc = PWM_OK;
     }

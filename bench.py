@@ -79,7 +79,7 @@ def poisson_crs(m, n):
     return N, rs, C[K].astype(np.int32), V[K]
 
 
-def make_workload(name):
+def make_workload(name, stencil_random=2):
     import thesis_spmv_amd as pw
     t0 = time.time()
     if name == "powerlaw":
@@ -89,8 +89,8 @@ def make_workload(name):
     elif name == "stencil9":
         side = 2896
         n = side * side
-        rs, ci, dv = pw.gen_stencil9(side, 2, 3)
-        desc = "9-point stencil 2896x2896 + 2 random off-band nnz per row"
+        rs, ci, dv = pw.gen_stencil9(side, stencil_random, 3)
+        desc = f"9-point stencil 2896x2896 + {stencil_random} random off-band nnz per row"
     elif name == "poisson":
         n, rs, ci, dv = poisson_crs(1024, 1024)
         desc = "2D 5-point Poisson 1024x1024"
@@ -177,7 +177,7 @@ def bench_spmv(args, workload, cpu=True):
     the gather probe, the device power method and the CPU baseline."""
     import thesis_spmv_amd as pw
     values = args.values or ("f64,f32" if workload == "stencil9" else "f64")
-    n, rs, ci, dv, desc, gen_s = make_workload(workload)
+    n, rs, ci, dv, desc, gen_s = make_workload(workload, args.stencil9_random)
     z = int(rs[-1])
     x_host = pw.gen_uniform(n, 0)
     bytes_mv = algorithmic_bytes(n, z)
@@ -872,6 +872,9 @@ def main():
                     help="CSB layout: 0 = automatic, 1 slab pieces, 2 slab streams, 3 wide streams, 4 swept slabs")
     ap.add_argument("--csb-window-bits", type=int, default=0, help="swept slabs: log2 columns per window (0 = 1 MB)")
     ap.add_argument("--csb-sync-every", type=int, default=0, help="swept slabs: windows between workgroup barriers")
+    ap.add_argument("--stencil9-random", type=int, default=2,
+                    help="random off-band nonzeros per row of the stencil9 workload (2 in the configuration; 0 gives "
+                         "the plain 9-point stencil)")
     ap.add_argument("--csb-nt", type=int, default=-1,
                     help="CSB stream kernel non-temporal entry loads: 1, 0, or -1 = time both at load")
     ap.add_argument("--crs-nt", type=int, default=-1,

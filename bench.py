@@ -478,7 +478,10 @@ def bench_batch(args):
            "dtype": "f32+bf16" if mixed else args.batch_values, "data": "synthetic",
            "config": {"workload": "batched power method, 512 SPD instances (Poisson sides U[32,1024], power-law "
                                   "SPD N log-U[1e3,2e5])", "instances": n_inst, "rows": rows, "nnz": nnz,
-                      "max_it": args.max_it, "tol": args.tol},
+                      "max_it": args.max_it, "tol": args.tol,
+                      "storage": "FP32 values, BF16 for the instances whose values are exact in BF16 (the Poisson "
+                                 "grids); FP32 vectors and row sums; FP64 norms and Rayleigh quotients" if mixed else
+                                 f"{args.batch_values} values; FP64 norms and Rayleigh quotients"},
            "iterations": {"min": int(its.min()), "median": float(np.median(its)), "max": int(its.max())},
            "engine": b.stats(),
            "schedule": {"items_per_thread": sched["items_per_thread"], "persistent": sched["persistent"],
@@ -886,7 +889,7 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-values", default="f32")
     ap.add_argument("--batch-ipt", type=int, default=16, help="merge items per thread of a batch tile (4, 8, 16)")
-    ap.add_argument("--batch-bf16-exact", type=int, default=0,
+    ap.add_argument("--batch-bf16-exact", type=int, default=1,
                     help="FP32 batch: store instances whose values are exact in BF16 as BF16 (1) or not (0)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="batch steps per graph replay")
     ap.add_argument("--batch-autotune", type=int, default=1,

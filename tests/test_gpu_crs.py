@@ -478,6 +478,39 @@ def test_time_mv_back_to_back(cls):
     assert ms2 > 0 and 0 < kms2 <= ms2 * 1.5
 
 
+def test_time_mv_rotating_results_and_kernel_counts():
+    """pwm_time_mv_rotating runs one SpMV of each handle in turn on the first
+    handle's stream: every y holds its own matrix's product afterwards, a
+    handle may appear twice, and each handle keeps working on its own stream.
+    pwm_info's kernel count matches the launches of each format's SpMV."""
+    mats, xs, ys, refs = [], [], [], []
+    for k, (cls, opt) in enumerate([(pw.CRSHip, {"crs_kernel": 2}), (pw.CRSMergeHip, {}),
+                                    (pw.CSBHip, {"csb_layout": 4}), (pw.CSBHip, {"csb_groups": 8}),
+                                    (pw.BlockCOHHip, {})]):
+        rs, ci, dv = pw.gen_powerlaw(40000, struct_seed=20 + k, value_seed=30 + k, cap=2000)
+        A = oracle.CRSMatrix(40000, 40000, rs, ci, dv)
+        m = _load(cls, A, **opt)
+        xh = pw.gen_uniform(A.noc, 40 + k)
+        mats.append(m)
+        xs.append(pw.DeviceVector.from_array(xh))
+        ys.append(pw.DeviceVector(A.nor))
+        refs.append(oracle.crs_mv(A, xh))
+    kernels = [m.info()["kernels"] for m in mats]
+    print(f"kernels per SpMV: {kernels}", flush=True)
+    assert kernels[1] == 2 and kernels[2] == 1 and kernels[3] == 2 and kernels[0] in (1, 2) and kernels[4] in (1, 2)
+    for per_step in (True, False):
+        for y in ys:
+            y.upload(np.full(y.n, np.nan))
+        ms, kms = pw.time_mv_rotating(mats + [mats[0]], xs + [xs[0]], ys + [ys[0]], 3, per_step=per_step)
+        assert ms > 0 and ((kms is None) != per_step)
+        for y, ref in zip(ys, refs):
+            assert np.max(np.abs(y.download() - ref)) <= 1e-13 * np.max(np.abs(ref))
+    for m, x, y, ref in zip(mats, xs, ys, refs):  # own streams again
+        y.upload(np.zeros(y.n))
+        m.mv_device(x, y)
+        assert np.max(np.abs(y.download() - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
 def test_info_reports_tuned_choices():
     """pwm_info reports the kernel variant a handle runs: the options the
     caller fixed, and a valid choice when the load-time tuner picked it."""

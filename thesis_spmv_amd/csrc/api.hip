@@ -1532,6 +1532,12 @@ extern "C" int pwm_time_mv_rotating(pwm_matrix* const* hs, const void* const* xs
     }
     pwm_matrix* h0 = hs[0];
     const hipStream_t s = h0->stream;
+    // work still queued on a handle's own stream finishes before its SpMVs
+    // run on the first handle's stream
+    for (int k = 1; k < count; ++k) {
+        const hipError_t es = hipStreamSynchronize(hs[k]->stream);
+        if (es != hipSuccess) return hip_fail(hs[k], es, "pwm_time_mv_rotating");
+    }
     std::vector<hipStream_t> saved(count);
     for (int k = 0; k < count; ++k) {
         saved[k] = hs[k]->stream;

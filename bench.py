@@ -625,7 +625,8 @@ def bench_serve(args):
             srv.add_matrix(*p["crs"], borrow=True)
 
     def run_server(reqs):
-        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph)
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph,
+                        bf16_when_exact=args.serve_bf16_exact)
         resident_ids = {}  # pool index -> list of server matrix ids
         lat, jobs, pending, k = [], {}, list(reqs), 0
         start = time.perf_counter()
@@ -694,7 +695,8 @@ def bench_serve(args):
         rounding errors; lambda itself can sit near 0 on the nonsymmetric
         power-law matrices), relative where lambda_64 is the dominant scale,
         and the max-norm difference of the normalised final vectors."""
-        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph)
+        srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph,
+                        bf16_when_exact=args.serve_bf16_exact)
         mids = {}
         jobs = {}
         for r in reqs:
@@ -903,6 +905,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
     ap.add_argument("--serve-max-n", type=float, default=1e7)
     ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per pwm_server_step")
+    ap.add_argument("--serve-bf16-exact", type=int, default=0,
+                    help="1: the server stores matrices whose values are exact in BF16 (Poisson) as BF16")
     ap.add_argument("--serve-graph", type=int, default=1, help="0: the server launches its steps one by one "
                     "instead of replaying them as hipGraphs")
     args = ap.parse_args()

@@ -393,7 +393,11 @@ int pwm_server_destroy(pwm_server* s);
 const char* pwm_server_last_error(const pwm_server* s);
 /* "chunk" (steps per pwm_server_step, default 8), "graph" (0: plain
  * launches), "persistent" (as for pwm_batch_set_option), "items_per_thread"
- * (4, 8, 16, default 16; only with nothing resident). */
+ * (4, 8, 16, default 16; only with nothing resident), "bf16_when_exact"
+ * (FP32 servers, before any matrix is added: 1 stores each matrix whose
+ * values are all exact in BF16, and every Poisson matrix, as BF16; vectors
+ * stay FP32; a job's result is bit-identical to the same job on a pure BF16
+ * or pure FP32 server). */
 int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
 /* PWM_SERVER_BORROW: keep the caller's arrays instead of a copy.  They must
  * stay alive and unchanged until the matrix is removed; they are checked

@@ -165,3 +165,47 @@ def test_errors():
         srv.submit(p, 10)
     with pytest.raises(pw.PwmError):
         srv.result(j)  # already collected
+
+
+def test_bf16_where_exact_is_bitwise_the_pure_servers():
+    """With bf16_when_exact an FP32 server stores the Poisson matrices and the
+    matrices whose values are exact in BF16 as BF16, the others as FP32; each
+    job's result equals the same job on a pure BF16 or pure FP32 server, bit
+    for bit."""
+    rng = np.random.default_rng(9)
+    A = _powerlaw(rng, 2500, long_rows=[(7, 2000)])  # values in (0.1, 1.1): FP32 storage
+    B = _powerlaw(rng, 1200)
+    B = oracle.CRSMatrix(B.nor, B.noc, B.row_start, B.col_ind, np.round(B.data * 8) + 1.0)  # small integers: BF16
+
+    def run(values, **opt):
+        srv = pw.Server(values, budget_bytes=128 << 20, max_jobs=16, chunk=4, **opt)
+        ids = {}
+        if values != "bf16":
+            ids["A"] = srv.add_matrix(A.row_start, A.col_ind, A.data)
+        ids["B"] = srv.add_matrix(B.row_start, B.col_ind, B.data)
+        ids["P"] = srv.add_poisson(45, 30)
+        n = {"A": A.nor, "B": B.nor, "P": 45 * 30}
+        jobs = {}
+        for k, (name, L, tol) in enumerate([("A", 300, 1e-9), ("B", 40, -1.0), ("P", 400, 1e-10), ("B", 250, 1e-9),
+                                            ("A", 17, -1.0), ("P", 5, -1.0)]):
+            if name in ids:
+                jobs[srv.submit(ids[name], L, tol, seed=50 + k)] = (k, n[name])
+        res = _drain(srv, {j: nn for j, (k, nn) in jobs.items()})
+        return {jobs[j][0]: r for j, r in res.items()}
+
+    mixed = run("f32", bf16_when_exact=1)
+    f32 = run("f32")
+    b16 = run("bf16")
+    for k, (lam, yn, its, vec) in mixed.items():
+        ref = b16[k] if k in b16 else f32[k]
+        if k in (0, 4):
+            ref = f32[k]
+        assert its == ref[2] and lam == ref[0] and yn == ref[1] and np.array_equal(vec, ref[3]), k
+    with pytest.raises(pw.PwmError) as e:
+        pw.Server("f64", budget_bytes=1 << 20, max_jobs=2, bf16_when_exact=1)
+    assert e.value.code == -1
+    srv = pw.Server("f32", budget_bytes=1 << 20, max_jobs=2)
+    srv.add_poisson(4, 4)
+    with pytest.raises(pw.PwmError) as e:
+        srv.set_option("bf16_when_exact", 1)
+    assert e.value.code == -6

@@ -264,6 +264,19 @@ int hip_fail(pwm_matrix* h, hipError_t e, const char* what) {
     return fail(h, e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP, m);
 }
 size_t value_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : (vtype == PWM_VALUES_F32 ? 4 : 2); }
+
+bool exact_in_bf16(const double* v, int64_t n) {
+    std::atomic<bool> ok{true};
+    parallel_for(n, 1 << 20, [&](int64_t b, int64_t e, int) {
+        for (int64_t k = b; k < e && ok.load(std::memory_order_relaxed); ++k) {
+            const float f = (float)v[k];
+            uint32_t bits;
+            memcpy(&bits, &f, 4);
+            if ((double)f != v[k] || (bits & 0xffffu)) ok = false;
+        }
+    });
+    return ok;
+}
 size_t vector_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : 4; }
 
 static CrsDev crs_of(const pwm_matrix* h) {

@@ -249,6 +249,8 @@ int coo_to_crs_device(pwm_matrix* h, int64_t nor, int64_t noc, int64_t nnz, cons
                       const double* vals, DevBuf& rs, DevBuf& ci, DevBuf& dv);
 int hip_fail(pwm_matrix* h, hipError_t e, const char* what);
 size_t value_size(int vtype);
+// True when every value is exact in BF16 (float-exact with a zero low half).
+bool exact_in_bf16(const double* v, int64_t n);
 size_t vector_size(int vtype);
 
 // per-format planning after the CRS arrays are on the device

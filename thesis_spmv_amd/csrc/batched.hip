@@ -208,16 +208,6 @@ extern "C" int pwm_batch_set_option(pwm_batch* b, const char* key, int64_t value
     return bfail(b, PWM_ERR_INVALID, "pwm_batch_set_option: unknown option " + k);
 }
 
-static bool exact_in_bf16(const std::vector<double>& v) {
-    for (double d : v) {
-        const float f = (float)d;
-        uint32_t bits;
-        memcpy(&bits, &f, 4);
-        if ((double)f != d || (bits & 0xffffu)) return false;
-    }
-    return true;
-}
-
 static int batch_upload(pwm_batch* b) {
     if (b->uploaded && b->planned_ipt == b->ipt) return PWM_OK;
     const size_t xs = pwm_impl::vector_size(b->vtype);
@@ -225,7 +215,7 @@ static int batch_upload(pwm_batch* b) {
     using pwm_impl::Arena;
     size_t total = 0;
     for (BatchInst& I : b->inst) {
-        I.vt16 = b->bf16_exact && (I.kind == 1 || exact_in_bf16(I.val));
+        I.vt16 = b->bf16_exact && (I.kind == 1 || pwm_impl::exact_in_bf16(I.val.data(), (int64_t)I.val.size()));
         const size_t vs = I.vt16 ? 2 : pwm_impl::value_size(b->vtype);
         I.n_tiles = pwmk::engine_tiles_for(I.n, I.nnz, b->ipt);
         I.cap_tiles = b->autotune ? std::max(I.n_tiles, pwmk::engine_tiles_for(I.n, I.nnz, kTuneMinIpt)) : I.n_tiles;

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -37,6 +38,7 @@ struct SrvMatrix {
     size_t off = 0, bytes = 0;
     size_t o_ci = 0, o_val = 0, o_tiles = 0;  // inside the block
     int32_t n_tiles = 0;
+    bool vt16 = false;  // BF16 values (bf16_when_exact and every value exact in BF16)
     std::vector<pwmk::ETile> tiles;  // host copy of the plan while resident
     int refs = 0;
     uint64_t last_use = 0;
@@ -66,7 +68,9 @@ struct pwm_server {
     hipStream_t stream = nullptr;
     std::string err;
     pwm_impl::Arena arena;
-    pwm_impl::Engine eng;
+    std::unique_ptr<pwm_impl::Engine> eng;
+    int bf16_exact = 0;  // FP32 server: BF16 storage for matrices exact in BF16
+    int persistent = 0;
     std::vector<SrvMatrix> mats;
     std::map<int64_t, SrvJob> jobs;
     std::deque<int64_t> queue;
@@ -95,9 +99,13 @@ static int shp(pwm_server* s, hipError_t e, const char* what) {
 
 static size_t acc_size(int vtype) { return vtype == PWM_VALUES_F64 ? 8 : 4; }
 
+static size_t matrix_value_size(const pwm_server* s, const SrvMatrix& M) {
+    return M.vt16 ? 2 : pwm_impl::value_size(s->vtype);
+}
+
 static void matrix_layout(const pwm_server* s, SrvMatrix& M) {
     using pwm_impl::Arena;
-    const size_t vs = pwm_impl::value_size(s->vtype);
+    const size_t vs = matrix_value_size(s, M);
     M.n_tiles = pwmk::engine_tiles_for(M.n, M.nnz, s->ipt);
     const int64_t zp = M.nnz + pwm_impl::kPadEntries;  // tiles read aligned groups of 4
     M.o_ci = Arena::round((M.n + 1) * 4);
@@ -133,13 +141,15 @@ extern "C" int pwm_server_create(int value_type, int64_t budget_bytes, int32_t m
         return sfail(nullptr, rc, std::string("pwm_server_create: ") + hipGetErrorString(e));
     }
     std::string err;
-    const int rc = s->eng.init(value_type, max_jobs, s->stream, err);
+    s->eng.reset(new pwm_impl::Engine());
+    const int rc = s->eng->init(value_type, max_jobs, s->stream, err);
     if (rc != PWM_OK) {
+        s->eng.reset();
         (void)hipStreamDestroy(s->stream);
         delete s;
         return sfail(nullptr, rc, err);
     }
-    s->eng.chunk = s->chunk;
+    s->eng->chunk = s->chunk;
     s->slot_job.assign(max_jobs, -1);
     *out = s;
     return PWM_OK;
@@ -162,16 +172,37 @@ extern "C" int pwm_server_set_option(pwm_server* s, const char* key, int64_t val
     if (k == "chunk") {
         if (value < 1 || value > 4096) return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: chunk must be in [1, 4096]");
         s->chunk = (int)value;
-        s->eng.chunk = s->chunk;
+        s->eng->chunk = s->chunk;
         return PWM_OK;
     }
     if (k == "graph") {
         s->use_graph = value != 0;
-        s->eng.use_graph = s->use_graph != 0;
+        s->eng->use_graph = s->use_graph != 0;
         return PWM_OK;
     }
     if (k == "persistent") {
-        s->eng.persistent = value != 0;
+        s->persistent = value != 0;
+        s->eng->persistent = s->persistent != 0;
+        return PWM_OK;
+    }
+    if (k == "bf16_when_exact") {
+        if (value != 0 && s->vtype != PWM_VALUES_F32)
+            return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: bf16_when_exact needs an FP32 server");
+        if (!s->mats.empty() || !s->jobs.empty())
+            return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: bf16_when_exact is set before any matrix is added");
+        if ((value != 0) == (s->bf16_exact != 0)) return PWM_OK;
+        // the engine of a mixed server branches per tile on its job's storage
+        std::unique_ptr<pwm_impl::Engine> eng(new pwm_impl::Engine());
+        std::string err;
+        const int rc = eng->init(value != 0 ? pwmk::VT_MIXED16 : s->vtype, (int)s->slot_job.size(), s->stream, err);
+        if (rc != PWM_OK) return sfail(s, rc, err);
+        if (s->ipt != eng->ipt() && eng->set_ipt(s->ipt) != PWM_OK)
+            return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: items_per_thread");
+        eng->chunk = s->chunk;
+        eng->use_graph = s->use_graph != 0;
+        eng->persistent = s->persistent != 0;
+        s->eng.swap(eng);
+        s->bf16_exact = value != 0;
         return PWM_OK;
     }
     if (k == "items_per_thread") {
@@ -179,7 +210,7 @@ extern "C" int pwm_server_set_option(pwm_server* s, const char* key, int64_t val
             return sfail(s, PWM_ERR_INVALID, "pwm_server_set_option: items_per_thread must be 4, 8 or 16");
         for (const SrvMatrix& M : s->mats)
             if (M.alive && M.resident) return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: matrices are resident");
-        if (s->eng.set_ipt((int)value) != PWM_OK) return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: jobs in flight");
+        if (s->eng->set_ipt((int)value) != PWM_OK) return sfail(s, PWM_ERR_STATE, "pwm_server_set_option: jobs in flight");
         s->ipt = (int)value;
         for (SrvMatrix& M : s->mats)
             if (M.alive) matrix_layout(s, M);
@@ -201,7 +232,7 @@ static int add_matrix_common(pwm_server* s, SrvMatrix&& M, int32_t* mat) {
 // uploaded: non-decreasing row_start from 0 to nnz, columns in [0, n), and
 // for BF16 storage values exact in BF16.  Runs on the host thread pool.
 static int check_crs(pwm_server* s, int64_t n, int64_t nnz, const int32_t* rs, const int32_t* ci, const double* vals,
-                     const char* who) {
+                     const char* who, bool need_bf16) {
     if (rs[0] != 0 || rs[n] != nnz) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": bad row_start");
     std::atomic<int> bad{0};  // 1 row_start, 2 column, 3 BF16 value
     pwm_impl::parallel_for(n, 1 << 18, [&](int64_t b, int64_t e, int) {
@@ -215,7 +246,7 @@ static int check_crs(pwm_server* s, int64_t n, int64_t nnz, const int32_t* rs, c
                     bad = 2;
                     return;
                 }
-                if (s->vtype == PWM_VALUES_BF16) {
+                if (need_bf16) {
                     const float f = (float)vals[k];
                     uint32_t bits;
                     memcpy(&bits, &f, 4);
@@ -236,11 +267,12 @@ extern "C" int pwm_server_add_matrix(pwm_server* s, int64_t n, const int32_t* rs
     const int64_t nnz = rs[n];
     if (rs[0] != 0 || nnz < 0 || (nnz > 0 && (!ci || !vals)) || n + nnz > INT32_MAX)
         return sfail(s, PWM_ERR_INVALID, "pwm_server_add_matrix: bad row_start or NULL arrays");
-    int rc = check_crs(s, n, nnz, rs, ci, vals, "pwm_server_add_matrix");
+    int rc = check_crs(s, n, nnz, rs, ci, vals, "pwm_server_add_matrix", s->vtype == PWM_VALUES_BF16);
     if (rc != PWM_OK) return rc;
     SrvMatrix M;
     M.n = n;
     M.nnz = nnz;
+    M.vt16 = s->bf16_exact && pwm_impl::exact_in_bf16(vals, nnz);
     if (flags & 1) {
         M.borrowed = true;
         M.rs_p = rs;
@@ -267,6 +299,7 @@ extern "C" int pwm_server_add_poisson(pwm_server* s, int64_t m, int64_t n, int32
     M.pn = n;
     M.n = m * n;
     M.nnz = n * (m + 2 * (m - 1)) + 2 * (n - 1) * m;
+    M.vt16 = s->bf16_exact != 0;  // 4 and -1
     return add_matrix_common(s, std::move(M), mat);
 }
 
@@ -315,22 +348,25 @@ static int make_resident(pwm_server* s, int mi) {
     SrvMatrix& M = s->mats[mi];
     char* base = (char*)s->arena.at(M.off);
     int32_t* rs = (int32_t*)base;
-    const size_t vs = pwm_impl::value_size(s->vtype);
+    const size_t vs = matrix_value_size(s, M);
+    const int vt = M.vt16 ? PWM_VALUES_BF16 : s->vtype;  // storage of this matrix
     hipError_t e = hipMemsetAsync(base + M.o_ci + M.nnz * 4, 0, pwm_impl::kPadEntries * 4, s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(base + M.o_val + M.nnz * vs, 0, pwm_impl::kPadEntries * vs, s->stream);
     if (e == hipSuccess && M.kind == 1) {
-        e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, s->vtype, s->stream);
+        e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, vt, s->stream);
     } else if (e == hipSuccess) {
         if (M.borrowed) {
             // borrowed arrays may have changed since registration: check them
-            // again before the kernels index x with them
-            const int rc = check_crs(s, M.n, M.nnz, M.rs_p, M.ci_p, M.val_p, "server: borrowed matrix");
+            // again before the kernels index x with them (and, for BF16
+            // storage, before their values are narrowed)
+            const int rc = check_crs(s, M.n, M.nnz, M.rs_p, M.ci_p, M.val_p, "server: borrowed matrix",
+                                     vt == PWM_VALUES_BF16);
             if (rc != PWM_OK) return rc;
         }
         e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->stream);
         if (e == hipSuccess) e = pwm_impl::upload_staged(base + M.o_ci, M.ci_p, M.nnz * 4, s->stream);
         if (e == hipSuccess && M.nnz) {
-            if (s->vtype == PWM_VALUES_F64) {
+            if (vt == PWM_VALUES_F64) {
                 e = pwm_impl::upload_staged(base + M.o_val, M.val_p, M.nnz * 8, s->stream);
             } else {
                 // narrow on the host (FP32, or BF16 = the upper half of the
@@ -340,7 +376,7 @@ static int make_resident(pwm_server* s, int mi) {
                 const void* src;
                 size_t nb;
                 const double* v = M.val_p;
-                if (s->vtype == PWM_VALUES_F32) {
+                if (vt == PWM_VALUES_F32) {
                     f32.resize(M.nnz);
                     float* o = f32.data();
                     pwm_impl::parallel_for(M.nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
@@ -398,7 +434,7 @@ extern "C" int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, dou
 static int admit(pwm_server* s) {
     const size_t xs = pwm_impl::vector_size(s->vtype);
     while (!s->queue.empty()) {
-        if (s->eng.active() >= s->eng.max_slots()) break;
+        if (s->eng->active() >= s->eng->max_slots()) break;
         const int64_t id = s->queue.front();
         SrvJob& J = s->jobs[id];
         SrvMatrix& M = s->mats[J.mat];
@@ -436,7 +472,8 @@ static int admit(pwm_server* s) {
         j.max_it = J.max_it;
         j.tol = J.tol;
         j.n_tiles = M.n_tiles;
-        const int slot = s->eng.add(j, M.tiles.data());
+        j.vt16 = M.vt16;
+        const int slot = s->eng->add(j, M.tiles.data());
         if (slot < 0) {  // cannot happen: a slot was free above
             s->arena.release(J.off, J.bytes);
             J.has_block = false;
@@ -459,14 +496,14 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
     const double t1 = now_s();
     s->t_admit += t1 - t0;
     if (rc != PWM_OK) return rc;
-    if (s->eng.active() == 0 && !s->queue.empty())
+    if (s->eng->active() == 0 && !s->queue.empty())
         return sfail(s, PWM_ERR_BUDGET,
                      "pwm_server_step: the next job does not fit the budget with nothing running "
                      "(collect finished results to free their memory)");
     std::vector<int> fin;
-    if (s->eng.active() > 0) {
+    if (s->eng->active() > 0) {
         std::string err;
-        if ((rc = s->eng.run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
+        if ((rc = s->eng->run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
     }
     const double t2 = now_s();
     s->t_run += t2 - t1;
@@ -476,14 +513,14 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
         const int64_t id = s->slot_job[slot];
         s->last_finished.push_back(id);
         SrvJob& J = s->jobs[id];
-        const pwmk::EState& S = s->eng.state(slot);
+        const pwmk::EState& S = s->eng->state(slot);
         J.lambda = S.lambda;
         J.ynorm = sqrt(S.yy);
         J.iters = S.it;
-        J.last = s->eng.last_buffer(slot);
+        J.last = s->eng->last_buffer(slot);
         J.state = 2;
         if (e == hipSuccess) e = pwmk::engine_normalize(J.last, s->mats[J.mat].n, s->vtype, 1.0 / J.ynorm, s->stream);
-        s->eng.release(slot);
+        s->eng->release(slot);
         s->slot_job[slot] = -1;
         s->mats[J.mat].refs--;
         ++s->finished_total;
@@ -491,7 +528,7 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
     if (e == hipSuccess && !fin.empty()) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return shp(s, e, "job retirement");
     s->t_retire += now_s() - t2;
-    if (active) *active = s->eng.active();
+    if (active) *active = s->eng->active();
     if (finished) *finished = (int32_t)fin.size();
     return PWM_OK;
 }
@@ -536,7 +573,7 @@ extern "C" int pwm_server_stats(const pwm_server* s, int64_t* used_bytes, int32_
         *resident = r;
     }
     if (queued) *queued = (int32_t)s->queue.size();
-    if (running) *running = s->eng.active();
+    if (running) *running = s->eng->active();
     if (finished) *finished = s->finished_total;
     return PWM_OK;
 }

@@ -152,10 +152,15 @@ def _instance(sides, spd, i):
     return oracle.CRSMatrix(len(rs) - 1, len(rs) - 1, rs, ci, dv)
 
 
-@pytest.mark.parametrize("values,tol", [("f32", 1e-6), ("f64", 1e-10)])
-def test_batch_512_instances(batch_workload, values, tol):
+@pytest.mark.parametrize("values,tol,opt", [("f32", 1e-6, {}), ("f64", 1e-10, {}),
+                                            ("f32", 1e-6, {"bf16_when_exact": 1, "autotune": 1})],
+                         ids=["f32", "f64", "f32-bf16_where_exact-autotuned"])
+def test_batch_512_instances(batch_workload, values, tol, opt):
+    """The benchmark's 512 instances, K plain steps, lambda of sampled
+    instances (the largest grid and power-law instance included) against the
+    FP64 oracle; the last case is the benchmark's own configuration."""
     sides, spd = batch_workload
-    b = pw.Batch(values)
+    b = pw.Batch(values, **opt)
     for s in sides:
         b.add_poisson(int(s), int(s))
     for rs, ci, dv in spd:
@@ -165,7 +170,7 @@ def test_batch_512_instances(batch_workload, values, tol):
     assert n_inst == 512
     K = 30
     lam, its, ms = b.solve(K, -1.0, seed=0)
-    print(f"  {K} steps in {ms:.1f} ms", flush=True)
+    print(f"  {K} steps in {ms:.1f} ms; schedule {b.schedule()}", flush=True)
     assert np.all(its == K)
     big_side = int(np.argmax(sides))
     big_n = len(sides) + int(np.argmax([len(m[0]) for m in spd]))

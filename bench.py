@@ -453,7 +453,8 @@ def bench_batch(args):
     opts = {"bf16_when_exact": 1} if mixed else {}
     if args.batch_autotune:
         opts["autotune"] = 1
-    b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk, **opts)
+    b = pw.Batch(args.batch_values, items_per_thread=args.batch_ipt, chunk=args.batch_chunk, graph=args.batch_graph,
+                 **opts)
     ids = [b.add_poisson(int(s), int(s)) for s in sides] + [b.add_crs(*m) for m in spd]
     n_inst, rows, nnz = b.size()
     log(f"# batch: {n_inst} instances, {rows} rows, {nnz} nnz, built in {time.time() - t0:.1f}s")
@@ -894,6 +895,8 @@ def main():
     ap.add_argument("--batch-bf16-exact", type=int, default=1,
                     help="FP32 batch: store instances whose values are exact in BF16 as BF16 (1) or not (0)")
     ap.add_argument("--batch-chunk", type=int, default=32, help="batch steps per graph replay")
+    ap.add_argument("--batch-graph", type=int, default=1, help="0: the batch launches its steps one by one "
+                    "instead of replaying them as hipGraphs")
     ap.add_argument("--batch-autotune", type=int, default=1,
                     help="1: the warm-up solve times tile sizes 256x8 / 256x16 with plain and persistent grids and "
                          "keeps the fastest (overrides --batch-ipt)")

@@ -267,24 +267,63 @@ __global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
     const ETile* __restrict__ tiles = J.tiles;
     const int nt = J.n_tiles;
     double yy = 0.0, xy = 0.0, xx = 0.0;
-    for (int t = threadIdx.x; t < nt; t += kEngBS) {
-        yy += J.partials[3 * (int64_t)t + 0];
-        xy += J.partials[3 * (int64_t)t + 1];
-        xx += J.partials[3 * (int64_t)t + 2];
-        const int32_t r = tiles[t].a1;
-        if (r >= J.n || (t > 0 && tiles[t - 1].a1 == r)) continue;
-        Acc sum = 0;
-        int q = t;
-        for (; q < nt && tiles[q].a1 == r; ++q) sum += cv[q];
-        // tile q completes row r; when row r starts at or after that tile's
-        // first nonzero the carries are empty and tile q counted the row
-        if (q >= nt || J.rs[r] >= tiles[q].b0) continue;
-        const X out = (X)((double)y[r] + (double)sum * scale);
-        y[r] = out;
-        const double yd = (double)out, xd = (double)x[r] * scale;
-        yy += yd * yd;
-        xy += xd * yd;
-        xx += xd * xd;
+    // Each thread takes tiles t = tid, tid + 256, ... in order, kU at a time:
+    // the loads of kU tiles (their partials, their end row and its start,
+    // the carry and the next tile's bounds) are issued together, then the
+    // tiles are summed in order.  Tile t ends in row r = a1; when the tiles
+    // after t share that end row the row spans them, and the carries of
+    // tiles t, t+1, ... (in tile order) complete it in the first tile that
+    // ends past it, unless that tile starts at row r's start (then tile t's
+    // carry is empty and that tile counted the row).
+    constexpr int kU = 4;
+    for (int t0 = threadIdx.x; t0 < nt; t0 += kU * kEngBS) {
+        double p0[kU], p1[kU], p2[kU];
+        int32_t r[kU], rprev[kU], nx_a1[kU], nx_b0[kU], rs_r[kU];
+        Acc c[kU];
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int t = t0 + k * kEngBS;
+            if (t >= nt) break;
+            p0[k] = J.partials[3 * (int64_t)t + 0];
+            p1[k] = J.partials[3 * (int64_t)t + 1];
+            p2[k] = J.partials[3 * (int64_t)t + 2];
+            r[k] = tiles[t].a1;
+            rprev[k] = t > 0 ? tiles[t - 1].a1 : -1;
+            c[k] = cv[t];
+            nx_a1[k] = t + 1 < nt ? tiles[t + 1].a1 : -1;
+            nx_b0[k] = t + 1 < nt ? tiles[t + 1].b0 : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int t = t0 + k * kEngBS;
+            if (t >= nt) break;
+            rs_r[k] = r[k] < J.n ? J.rs[r[k]] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kU; ++k) {
+            const int t = t0 + k * kEngBS;
+            if (t >= nt) break;
+            yy += p0[k];
+            xy += p1[k];
+            xx += p2[k];
+            const int32_t row = r[k];
+            if (row >= J.n || (t > 0 && rprev[k] == row)) continue;
+            Acc sum = 0;
+            sum += c[k];
+            int q = t + 1;
+            int32_t b0 = nx_b0[k];
+            if (q < nt && nx_a1[k] == row) {  // the row spans further tiles: rare
+                for (; q < nt && tiles[q].a1 == row; ++q) sum += cv[q];
+                b0 = q < nt ? tiles[q].b0 : 0;
+            }
+            if (q >= nt || rs_r[k] >= b0) continue;
+            const X out = (X)((double)y[row] + (double)sum * scale);
+            y[row] = out;
+            const double yd = (double)out, xd = (double)x[row] * scale;
+            yy += yd * yd;
+            xy += xd * yd;
+            xx += xd * xd;
+        }
     }
     const double YY = block_sum<kEngBS>(yy, red);
     const double XY = block_sum<kEngBS>(xy, red);

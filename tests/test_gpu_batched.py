@@ -213,6 +213,31 @@ def test_autotune_keeps_a_timed_schedule(values, opt):
         assert abs(lam4[i] - ref) <= tol * abs(ref), (i, lam4[i], ref)
 
 
+def test_autotune_edge_cases():
+    """Autotuning a batch whose caller fixed 256 x 4 tiles (more tile slots
+    than any candidate needs), a one-instance batch, and a batch that gets a
+    new instance after it was tuned (the next solve uploads and tunes again)."""
+    rng = np.random.default_rng(13)
+    mats = _instances(rng)
+    b = _mixed_batch("f32", mats, items_per_thread=4, autotune=1)
+    lam, its, _ = b.solve(30, -1.0, seed=3)
+    s = b.schedule()
+    assert s["tuned"] and s["items_per_thread"] in (8, 16)
+    fixed = _mixed_batch("f32", mats, items_per_thread=s["items_per_thread"], persistent=int(s["persistent"]))
+    lam2, _, _ = fixed.solve(30, -1.0, seed=3)
+    assert np.array_equal(lam, lam2)
+    one = pw.Batch("f64", autotune=1)
+    one.add_poisson(3, 2)
+    lam1, its1, _ = one.solve(25, -1.0, seed=1)
+    A = oracle.poisson_crs(3, 2)
+    xr, yr = oracle.crs_power_iteration(A, one.start_vector(0, seed=1), 24)
+    assert abs(lam1[0] - _rq(A, xr)) <= 1e-10 * abs(lam1[0]) and one.schedule()["tuned"]  # 24 steps: in x
+    b.add_poisson(20, 20)
+    assert not b.schedule()["tuned"]
+    lam3, its3, _ = b.solve(30, -1.0, seed=3)
+    assert b.schedule()["tuned"] and len(lam3) == len(mats) + 1
+
+
 def test_errors():
     b = pw.Batch("bf16")
     with pytest.raises(pw.PwmError):

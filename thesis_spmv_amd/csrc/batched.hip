@@ -145,6 +145,7 @@ extern "C" int pwm_batch_add_crs(pwm_batch* b, int64_t n, const int32_t* rs, con
     b->rows += n;
     b->nnz += nnz;
     b->uploaded = false;
+    b->tuned = false;  // the next solve uploads and, with autotune, times the schedules again
     return PWM_OK;
 }
 
@@ -163,6 +164,7 @@ extern "C" int pwm_batch_add_poisson(pwm_batch* b, int64_t m, int64_t n, int32_t
     b->inst.push_back(std::move(I));
     b->where.clear();  // results of an earlier solve do not cover the new instance
     b->uploaded = false;
+    b->tuned = false;
     return PWM_OK;
 }
 

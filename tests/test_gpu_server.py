@@ -209,3 +209,15 @@ def test_bf16_where_exact_is_bitwise_the_pure_servers():
     with pytest.raises(pw.PwmError) as e:
         srv.set_option("bf16_when_exact", 1)
     assert e.value.code == -6
+    # a borrowed matrix stored as BF16 whose caller changes a value to one
+    # BF16 cannot hold: the upload refuses it instead of narrowing it
+    srv = pw.Server("f32", budget_bytes=16 << 20, max_jobs=2, bf16_when_exact=1)
+    rs = np.array([0, 2, 3, 5], np.int32)
+    ci = np.array([0, 2, 1, 0, 2], np.int32)
+    dv = np.array([2.0, 1.0, 3.0, 1.0, 2.0])
+    mat = srv.add_matrix(rs, ci, dv, borrow=True)
+    srv._held[mat][2][1] = 0.1
+    srv.submit(mat, 5, -1.0, seed=1)
+    with pytest.raises(pw.PwmError) as e:
+        srv.step()
+    assert e.value.code == -1

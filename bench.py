@@ -403,6 +403,7 @@ def bench_spmv(args, workload, cpu=True):
              "partitions": v["info"]["threads"], "group_size": v["info"]["group_size"],
              "long_rows": v["info"]["long_rows"], "block_bits": v["info"]["block_bits"],
              "unroll": v["info"]["unroll"], "non_temporal": v["info"]["non_temporal"], "layout": v["info"]["layout"],
+             "high_occupancy": v["info"]["high_occupancy"],
              "load_s": round(v["load_s"], 2), "sum_abs_y": v["y_checksum"],
              "host_array_mv_us": round(v["host_ms"] * 1e3, 1)}
         if k == "crs":

@@ -79,6 +79,7 @@ typedef struct {
     int32_t non_temporal;      /* CSB slab streams / CRS tiles: 1 = non-temporal entry loads */
     int32_t layout;            /* CSB: 1 slab pieces, 2 slab streams, 3 wide slab streams, 4 swept slabs */
     int32_t kernels;           /* kernel launches of one pwm_mv_device */
+    int32_t high_occupancy;    /* BlockCOH: 1 when the kernel instance with registers capped for 8 waves per SIMD runs */
 } pwm_info_t;
 
 /* Constructor SparseMatrix(int threads) (Matrix/SparseMatrix.hpp:30) through
@@ -161,7 +162,9 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
  * "csb_build" (CSB slab-piece layout built on the device, 0, or on the host,
  * 1; both give the same arrays; the slab streams are always built on the
  * device), "coh_build" (BlockCOH layout built on the device, 0, or on the
- * host, 1; both give the same arrays), "crs_kernel" (CRS: 0 = time the vector kernel at
+ * host, 1; both give the same arrays), "coh_occ8" (BlockCOH kernel instance
+ * with registers capped for 8 waves per SIMD: 1, the compiler's budget: 0,
+ * or -1 = time both at load, the default), "crs_kernel" (CRS: 0 = time the vector kernel at
  * three lane counts and the row-aligned tiles kernel at load and keep the
  * fastest, 1 = vector, 2 = tiles), "crs_nt" (CRS tiles kernel: 1 =
  * non-temporal col_ind/value loads, 0 = plain, -1 = time both at load, the

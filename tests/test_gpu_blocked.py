@@ -59,6 +59,7 @@ def test_golden(goldens, cls, idx):
     A, t = load_case_matrix(case)
     print(f"case {case['name']} {case['index_type']} {cls.__name__}", flush=True)
     runs = ([(t, {}) for t in (1, 2, 3, 4, 8, -1)] + [(t, {"coh_build": 1}) for t in (1, 3, -1)]
+            + [(t, {"coh_occ8": o}) for t in (1, -1) for o in (0, 1)]
             if cls is pw.BlockCOHHip
             else [(-1, dict(v)) for v in CSB_VARIANTS])
     for threads, opts in runs:
@@ -220,7 +221,7 @@ def test_spmv_random_and_stencil(cls):
                     {"csb_unroll": 4, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 8}, {"csb_nt": 1, "csb_groups": 1},
                     {"csb_nt": 0, "csb_unroll": 4}, {"csb_layout": 3}, {"csb_layout": 3, "csb_groups": 8},
                     {"csb_layout": 3, "csb_nt": 1, "csb_unroll": 4, "csb_groups": 16})
-        for opt in csb_opts if cls is pw.CSBHip else ({}, {"coh_build": 1}):
+        for opt in csb_opts if cls is pw.CSBHip else ({}, {"coh_build": 1}, {"coh_occ8": 0}, {"coh_occ8": 1}):
             for threads in ((-1,) if cls is pw.CSBHip else (-1, 1, 2, 5)):
                 m = _load(cls, A, threads, **opt)
                 y = np.zeros(A.nor)

@@ -521,6 +521,8 @@ def test_info_reports_tuned_choices():
         assert inf["group_size"] == 0 and inf["non_temporal"] == nt
         inf = _load(pw.CSBHip, A, csb_groups=8, csb_unroll=4, csb_nt=nt).info()
         assert (inf["group_size"], inf["unroll"], inf["non_temporal"]) == (8, 4, nt)
+        inf = _load(pw.BlockCOHHip, A, coh_occ8=nt).info()
+        assert inf["high_occupancy"] == nt
     inf = _load(pw.CSBHip, A).info()
     assert inf["unroll"] in (2, 4) and inf["non_temporal"] in (0, 1) and inf["group_size"] in (1, 8, 16)
     inf = _load(pw.CRSHip, A, crs_kernel=1).info()

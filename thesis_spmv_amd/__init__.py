@@ -42,7 +42,7 @@ class Info(C.Structure):
     _fields_ = [("nor", I64), ("noc", I64), ("nnz", I64), ("format", I32), ("value_type", I32),
                 ("threads", I32), ("group_size", I32), ("device_bytes", I64), ("long_rows", I64),
                 ("block_bits", I32), ("balanced", I32), ("unroll", I32), ("non_temporal", I32), ("layout", I32),
-                ("kernels", I32)]
+                ("kernels", I32), ("high_occupancy", I32)]
 
 
 _SIGS = {

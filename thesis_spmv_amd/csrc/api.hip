@@ -663,6 +663,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return fail(h, PWM_ERR_INVALID, "csb_sync_every must be 1, 2, 4 or 8");
         h->csb_sync_every = (int32_t)value;
+    } else if (k == "coh_occ8") {
+        if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "coh_occ8 must be -1 (timed), 0 or 1");
+        h->coh_occ8 = (int32_t)value;
     } else if (k == "csb_unroll") {
         if (value != 0 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_unroll must be 0, 2 or 4");
         h->csb_unroll = (int32_t)value;
@@ -946,6 +949,7 @@ extern "C" int pwm_info(const pwm_matrix* h, pwm_info_t* out) {
         case PWM_FORMAT_BLOCKCOH: out->kernels = 1 + (h->coh.n_atomic > 0 ? 1 : 0); break;
     }
     if (h->format == PWM_FORMAT_CSB && h->csb.n_tasks <= 0) out->kernels = 0;
+    out->high_occupancy = h->format == PWM_FORMAT_BLOCKCOH ? h->coh.occ8 : 0;
     out->non_temporal = h->format == PWM_FORMAT_CSB                         ? h->csb.nt
                         : (h->format == PWM_FORMAT_CRS && h->crs_used == 2) ? h->crs_nt_used
                                                                            : 0;

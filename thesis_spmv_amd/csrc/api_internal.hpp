@@ -138,12 +138,16 @@ struct CohDev {
     DevBuf task_list;      // CohTask per workgroup
     DevBuf atomic_bands;   // bands accumulated with global atomics
     int64_t n_tasks = 0, total_blocks = 0, n_atomic = 0;
+    int32_t lds_rows = 0;  // rows of the largest band accumulated in LDS (the kernel's LDS window)
+    int32_t occ8 = 0;      // kernel instance with 8 waves per SIMD (timed at load)
     void clear() {
         bands.clear();
         for (DevBuf* b : {&row_jump, &col_jump, &data, &row_jump_block, &col_jump_block, &block_nnz, &block_coords,
                           &band_table, &tasks, &task_list, &atomic_bands})
             b->release();
         n_tasks = total_blocks = n_atomic = 0;
+        lds_rows = 0;
+        occ8 = 0;
     }
 };
 
@@ -193,6 +197,7 @@ struct pwm_matrix {
     int32_t csb_layout = 0;  // 0 = automatic, 1 = pieces, 2 = streams, 3 = wide streams, 4 = swept slabs
     int32_t csb_window_bits = 0;  // swept slabs: log2 of the columns per window (0 = 1 MB of x)
     int32_t csb_sync_every = 2;   // swept slabs: windows between workgroup barriers
+    int32_t coh_occ8 = -1;        // BlockCOH kernel at 8 waves per SIMD: -1 timed at load, 0 no, 1 yes
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
     int32_t csb_build = 0;   // CSB slab pieces: 0 = built on the device, 1 = on the host
@@ -239,7 +244,7 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
 // states left zero.
 int coh_layout_device(pwm_matrix* h, int64_t lds_rows);
 void coh_plan_tasks(const std::vector<CohBandMeta>& B, int64_t lds_rows, std::vector<blocked::CohTask>& tasks,
-                    std::vector<int32_t>& atomic_bands);
+                    std::vector<int32_t>& atomic_bands, int32_t* max_lds_rows);
 // Statistics of the loaded CRS arrays (stats.hip) into h->stats; needs the
 // device row_start / col_ind.
 int compute_matrix_stats(pwm_matrix* h);

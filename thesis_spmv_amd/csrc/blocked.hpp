@@ -110,6 +110,8 @@ struct CohArgs {
     const uint16_t* row_jump = nullptr;
     const void* val = nullptr;
     int vtype = 0;
+    int32_t lds_rows = 0;  // LDS window of the kernel: rows of the largest LDS-accumulated band
+    int32_t occ8 = 0;      // 1: the kernel instance capped to 8 waves per SIMD
 };
 
 // part: 0 = the whole SpMV, 1 = the slab kernel only, 2 = the column-group

@@ -553,14 +553,16 @@ static void coh_bands(int64_t nor, int64_t nnz, const int32_t* rs, int threads, 
 // entries fit one task, atomic tasks of at most kCohTaskChunks chunks
 // otherwise (decode states left zero for the caller to fill).
 void coh_plan_tasks(const std::vector<CohBandMeta>& B, int64_t lds_rows, std::vector<blocked::CohTask>& tasks,
-                    std::vector<int32_t>& atomic_bands) {
+                    std::vector<int32_t>& atomic_bands, int32_t* max_lds_rows) {
     tasks.clear();
     atomic_bands.clear();
+    *max_lds_rows = 0;
     for (int p = 0; p < (int)B.size(); ++p) {
         const CohBandMeta& m = B[p];
         const int64_t nch = (m.nnz + blocked::kCohChunk - 1) / blocked::kCohChunk;
         if (m.rows <= lds_rows && nch <= blocked::kCohTaskChunks) {
             if (m.rows > 0) tasks.push_back({p, 0, (int32_t)nch, 0, 0, 0});
+            *max_lds_rows = std::max(*max_lds_rows, (int32_t)m.rows);
             continue;
         }
         atomic_bands.push_back(p);
@@ -571,7 +573,7 @@ void coh_plan_tasks(const std::vector<CohBandMeta>& B, int64_t lds_rows, std::ve
 
 // rs: host row_start; ci / vals: host CRS arrays, needed only by the host
 // layout build (option coh_build = 1; the device build reads the device CRS).
-int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
+static int build_blockcoh_layout(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
     const int64_t nor = h->nor, noc = h->noc, nnz = h->nnz;
     const bool device = h->coh_build != 1;
     int rc = PWM_OK;
@@ -746,7 +748,7 @@ int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const do
     // entry before its first one
     std::vector<blocked::CohTask> tasks;
     std::vector<int32_t> atomic_bands;
-    coh_plan_tasks(B, lds_rows, tasks, atomic_bands);
+    coh_plan_tasks(B, lds_rows, tasks, atomic_bands, &c.lds_rows);
     for (size_t t0 = 0; t0 < tasks.size();) {
         size_t t1 = t0 + 1;
         while (t1 < tasks.size() && tasks[t1].band == tasks[t0].band) ++t1;
@@ -854,10 +856,58 @@ hipError_t coh_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.row_jump = h->coh.row_jump.as<uint16_t>();
     a.val = h->coh.data.p;
     a.vtype = h->vtype;
+    a.lds_rows = h->coh.lds_rows;
+    a.occ8 = h->coh.occ8;
     hipError_t e = blocked::launch_coh(a, h->coh.atomic_bands.as<int32_t>(), (int)h->coh.n_atomic, x, y, pw, h->stream);
     if (e == hipSuccess && pw)
         e = blocked::launch_blocked_finalize(h->vtype, (int)coh_partials(h), *pw, h->stream);
     return e;
+}
+
+// Mean milliseconds of 5 BlockCOH SpMVs (after 2 warm-up ones) on x = 1.
+static int time_coh(pwm_matrix* h, float* ms) {
+    DevBuf tx, ty;
+    const size_t vs = vector_size(h->vtype);
+    hipError_t e;
+    if ((e = tx.alloc(std::max<int64_t>(h->noc, 1) * vs)) != hipSuccess ||
+        (e = ty.alloc(std::max<int64_t>(h->nor, 1) * vs)) != hipSuccess)
+        return hip_fail(h, e, "BlockCOH autotune vectors");
+    e = pwmk::launch_fill(tx.p, h->noc, h->vtype == PWM_VALUES_F64, 1.0, h->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = coh_launch(h, tx.p, ty.p, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+    for (int r = 0; r < 5 && e == hipSuccess; ++r) e = coh_launch(h, tx.p, ty.p, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(h, e, "BlockCOH autotune");
+    *ms = t / 5.0f;
+    return PWM_OK;
+}
+
+// The BlockCOH layout, then the kernel instance: the compiler's register
+// budget (FP64: 66 VGPRs, 7 waves per SIMD, 3 workgroups of 8 waves per CU)
+// or one capped for 8 waves per SIMD (4 workgroups when the LDS window
+// allows), the faster of the two on x = 1 unless the option coh_occ8 fixes it.
+int build_blockcoh(pwm_matrix* h, const int32_t* rs, const int32_t* ci, const double* vals) {
+    int rc = build_blockcoh_layout(h, rs, ci, vals);
+    if (rc != PWM_OK) return rc;
+    if (h->coh_occ8 >= 0 || h->coh.n_tasks <= 0) {
+        h->coh.occ8 = h->coh_occ8 > 0 ? 1 : 0;
+        return PWM_OK;
+    }
+    float ms0 = 0.0f, ms1 = 0.0f;
+    h->coh.occ8 = 0;
+    if ((rc = time_coh(h, &ms0)) != PWM_OK) return rc;
+    h->coh.occ8 = 1;
+    if ((rc = time_coh(h, &ms1)) != PWM_OK) return rc;
+    h->coh.occ8 = ms1 < ms0 ? 1 : 0;
+    return PWM_OK;
 }
 
 }  // namespace pwm_impl

@@ -430,7 +430,7 @@ int coh_layout_device(pwm_matrix* h, int64_t lds_rows) {
     if (off_blk != total) return fail(h, PWM_ERR_HIP, "BlockCOH build: band block counts disagree");
     std::vector<blocked::CohTask> tasks;
     std::vector<int32_t> atomic_bands;
-    coh_plan_tasks(B, lds_rows, tasks, atomic_bands);
+    coh_plan_tasks(B, lds_rows, tasks, atomic_bands, &c.lds_rows);
     c.n_atomic = (int64_t)atomic_bands.size();
     c.n_tasks = (int64_t)tasks.size();
     if ((e = c.band_table.alloc(P * sizeof(blocked::CohBand))) != hipSuccess ||

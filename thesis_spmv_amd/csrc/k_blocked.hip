@@ -572,13 +572,17 @@ int csb_sweep_blocks_per_cu(int vtype, int rows) {
 //   R = running sum of the row_jump entered at each row change: the row.
 // LDS tasks own their band (rows <= LDS window) and write it once; atomic
 // tasks add into y (zeroed by k_coh_zero) with global atomics.
-template <typename VT, bool POWER>
-__global__ __launch_bounds__(kBS) void k_coh(CohArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
+// OCC8: registers capped so that 8 waves per SIMD fit (4 workgroups per CU
+// when the LDS window allows it) instead of the compiler's 7.
+template <typename VT, bool POWER, bool OCC8>
+__global__ __launch_bounds__(kBS, OCC8 ? 8 : 1) void k_coh(CohArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                              typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
-    constexpr int LR = LdsRows<Acc>::v;
-    __shared__ Acc ylds[LR];
+    // the LDS window holds the rows of the largest band accumulated in LDS
+    // (at most LdsRows<Acc>::v), sized at launch
+    extern __shared__ __align__(16) unsigned char coh_smem[];
+    Acc* ylds = (Acc*)coh_smem;
     __shared__ int32_t bstart[kCohChunk + 2];  // up to kCohChunk+1 blocks and a sentinel
     // the records of the chunk's blocks when there are at most kCohStage of
     // them (the usual case), so the decode reads them from LDS
@@ -874,21 +878,23 @@ template <typename VT>
 static hipError_t coh_t(const CohArgs& a, const int32_t* atomic_bands, int n_atomic, const void* x, void* y,
                         const PowerArgs* pw, hipStream_t s) {
     using X = typename Arith<VT>::X;
+    using Acc = typename Arith<VT>::A;
     if (a.n_tasks <= 0) return hipSuccess;
+    const size_t lds = (size_t)std::max<int32_t>(a.lds_rows, 1) * sizeof(Acc);
     if (pw) {
         if (n_atomic > 0)
             hipLaunchKernelGGL((k_coh_zero<VT, true>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, (X*)nullptr,
                                *pw);
-        hipLaunchKernelGGL((k_coh<VT, true>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)nullptr,
-                           (X*)nullptr, *pw);
+        auto kp = a.occ8 ? k_coh<VT, true, true> : k_coh<VT, true, false>;
+        hipLaunchKernelGGL(kp, dim3((unsigned)a.n_tasks), dim3(kBS), lds, s, a, (const X*)nullptr, (X*)nullptr, *pw);
         if (n_atomic > 0)
             hipLaunchKernelGGL((k_coh_stats<VT>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, *pw);
     } else {
         if (n_atomic > 0)
             hipLaunchKernelGGL((k_coh_zero<VT, false>), dim3(n_atomic), dim3(kBS), 0, s, a, atomic_bands, (X*)y,
                                PowerArgs{});
-        hipLaunchKernelGGL((k_coh<VT, false>), dim3((unsigned)a.n_tasks), dim3(kBS), 0, s, a, (const X*)x, (X*)y,
-                           PowerArgs{});
+        auto kf = a.occ8 ? k_coh<VT, false, true> : k_coh<VT, false, false>;
+        hipLaunchKernelGGL(kf, dim3((unsigned)a.n_tasks), dim3(kBS), lds, s, a, (const X*)x, (X*)y, PowerArgs{});
     }
     return hipGetLastError();
 }

@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 FORMATS = [("crs_tiles", pw.CRSHip, {"crs_kernel": 2}, True), ("crs_vector", pw.CRSHip, {"crs_kernel": 1}, True),
            ("merge_path", pw.CRSMergeHip, {}, True), ("csb", pw.CSBHip, {}, False),
+           ("csb_swept", pw.CSBHip, {"csb_layout": 4}, False),
            ("blockcoh", pw.BlockCOHHip, {}, False)]
 
 

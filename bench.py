@@ -193,7 +193,8 @@ def bench_spmv(args, workload, cpu=True):
     csb_opts = {"block_bits": 14} if workload == "stencil9" else {}
     csb_opts.update({k: v for k, v in (("csb_groups", args.csb_groups), ("csb_unroll", args.csb_unroll),
                                        ("csb_layout", args.csb_layout), ("csb_window_bits", args.csb_window_bits),
-                                       ("csb_sync_every", args.csb_sync_every))
+                                       ("csb_sync_every", args.csb_sync_every),
+                                       ("csb_sweep_unroll", args.csb_sweep_unroll))
                      if v})
     if args.csb_nt >= 0:
         csb_opts["csb_nt"] = args.csb_nt
@@ -879,6 +880,8 @@ def main():
                     help="CSB layout: 0 = automatic, 1 slab pieces, 2 slab streams, 3 wide streams, 4 swept slabs")
     ap.add_argument("--csb-window-bits", type=int, default=0, help="swept slabs: log2 columns per window (0 = 1 MB)")
     ap.add_argument("--csb-sync-every", type=int, default=0, help="swept slabs: windows between workgroup barriers")
+    ap.add_argument("--csb-sweep-unroll", type=int, default=0,
+                    help="swept slabs: groups of 4 entries per lane per chunk (1, 2, 4; 0 = the library's 2)")
     ap.add_argument("--stencil9-random", type=int, default=2,
                     help="random off-band nonzeros per row of the stencil9 workload (2 in the configuration; 0 gives "
                          "the plain 9-point stencil)")

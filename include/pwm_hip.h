@@ -155,7 +155,10 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
  * walking the windows in order; 0 = automatic: 2 when slabs of at least 256
  * rows pack, else 3, replaced by 4 when x exceeds 2 MB, the slabs fit and 4
  * runs faster in the load-time timing), "csb_window_bits" (swept slabs: log2 of the
- * columns per window, 0 = 1 MB of x), "csb_unroll" (slab-stream kernel groups of 4 entries
+ * columns per window, 0 = 1 MB of x), "csb_sync_every" (swept slabs: windows
+ * between workgroup barriers, 1, 2, 4 or 8, default 2), "csb_sweep_unroll"
+ * (swept slabs: groups of 4 entries per lane per chunk, 1, 2 or 4, default 2),
+ * "csb_unroll" (slab-stream kernel groups of 4 entries
  * per thread per step, 2 or 4; 0 = time both with each group count),
  * "csb_nt" (slab-stream kernel: 1 = non-temporal entry loads and partial-slab
  * stores, 0 = plain, -1 = time both at load, the default),

@@ -663,6 +663,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return fail(h, PWM_ERR_INVALID, "csb_sync_every must be 1, 2, 4 or 8");
         h->csb_sync_every = (int32_t)value;
+    } else if (k == "csb_sweep_unroll") {
+        if (value != 1 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_sweep_unroll must be 1, 2 or 4");
+        h->csb_sweep_unroll = (int32_t)value;
     } else if (k == "coh_occ8") {
         if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "coh_occ8 must be -1 (timed), 0 or 1");
         h->coh_occ8 = (int32_t)value;

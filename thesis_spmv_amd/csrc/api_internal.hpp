@@ -197,6 +197,7 @@ struct pwm_matrix {
     int32_t csb_layout = 0;  // 0 = automatic, 1 = pieces, 2 = streams, 3 = wide streams, 4 = swept slabs
     int32_t csb_window_bits = 0;  // swept slabs: log2 of the columns per window (0 = 1 MB of x)
     int32_t csb_sync_every = 2;   // swept slabs: windows between workgroup barriers
+    int32_t csb_sweep_unroll = 2; // swept slabs: groups of 4 entries per lane per chunk (1, 2 or 4)
     int32_t coh_occ8 = -1;        // BlockCOH kernel at 8 waves per SIMD: -1 timed at load, 0 no, 1 yes
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)

@@ -835,6 +835,7 @@ hipError_t csb_launch(pwm_matrix* h, const void* x, void* y, const pwmk::PowerAr
     a.W = h->csb.sweep_W;
     a.max_rows = h->csb.sweep_rows;
     a.sync_every = h->csb_sync_every;
+    if (a.sweep) a.unroll = h->csb_sweep_unroll;
     a.sstart = h->csb.sweep_start.as<int32_t>();
     hipError_t e = blocked::launch_csb(a, x, y, pw, h->stream, part);
     if (e == hipSuccess && pw && part != 1)

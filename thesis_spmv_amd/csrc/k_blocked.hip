@@ -459,12 +459,12 @@ __device__ __forceinline__ void sweep_add(const SweepChunk<VT, kSU, NT>& c, int3
             if (ok[u][j]) atomicAdd(&ylds[(uint32_t)c.w[u][j] >> wb], (Acc)c.v[u].get(j) * (Acc)xv[u][j]);
 }
 
-template <typename VT, bool POWER, bool NT>
+template <typename VT, bool POWER, bool NT, int SU = 2>
 __global__ __launch_bounds__(kSwBS) void k_csb_sweep(CsbArgs a, const typename Arith<VT>::X* __restrict__ x_arg,
                                                      typename Arith<VT>::X* __restrict__ y_arg, PowerArgs pw) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
-    constexpr int kSU = 2;
+    constexpr int kSU = SU;  // groups of 4 entries per lane per chunk
     constexpr int kWv = kSwBS / 64;
     constexpr int kChunk = 64 * kSU;  // groups of 4 entries per wave per chunk
     extern __shared__ __align__(16) unsigned char csb_smem[];
@@ -821,8 +821,14 @@ static hipError_t csb_t(const CsbArgs& a, const void* x, void* y, const PowerArg
     if (a.sweep) {
         if (part == 2) return hipSuccess;  // no column-group reduction
         const size_t lds = (size_t)a.max_rows * sizeof(Acc);
-        auto kp = a.nt ? k_csb_sweep<VT, true, true> : k_csb_sweep<VT, true, false>;
-        auto kf = a.nt ? k_csb_sweep<VT, false, true> : k_csb_sweep<VT, false, false>;
+        auto pick = [&](auto power) {
+            constexpr bool P = decltype(power)::value;
+            if (a.unroll == 1) return a.nt ? k_csb_sweep<VT, P, true, 1> : k_csb_sweep<VT, P, false, 1>;
+            if (a.unroll == 4) return a.nt ? k_csb_sweep<VT, P, true, 4> : k_csb_sweep<VT, P, false, 4>;
+            return a.nt ? k_csb_sweep<VT, P, true, 2> : k_csb_sweep<VT, P, false, 2>;
+        };
+        auto kp = pick(std::true_type{});
+        auto kf = pick(std::false_type{});
         if (pw)
             hipLaunchKernelGGL(kp, dim3((unsigned)a.n_tasks), dim3(kSwBS), lds, s, a, (const X*)nullptr, (X*)nullptr,
                                *pw);

@@ -31,13 +31,17 @@
 
 namespace pwmk {
 
+// prod holds the tile's products at index k - (b0 & ~3) (entry k), so that
+// each aligned group of 4 entries is one 16-byte-aligned LDS store; the
+// products of [b0, b1) start at offset b0 & 3.
 template <typename Acc, int BS, int ITEMS>
 struct MergeTileSmem {
     int32_t rowend[BS * ITEMS];
-    Acc prod[BS * ITEMS];
+    alignas(16) Acc prod[BS * ITEMS + 8];
     int32_t wkey[BS / 64];
     Acc wval[BS / 64];
 };
+
 
 template <typename T>
 __device__ __forceinline__ T shfl_up64(T v, int off) {
@@ -50,6 +54,18 @@ typedef float vf4 __attribute__((ext_vector_type(4)));
 typedef double vd2 __attribute__((ext_vector_type(2)));
 typedef int vi4 __attribute__((ext_vector_type(4)));
 typedef unsigned vu2 __attribute__((ext_vector_type(2)));
+template <typename Acc>
+__device__ __forceinline__ void store4(Acc* p, const Acc (&v)[4]);
+template <>
+__device__ __forceinline__ void store4<float>(float* p, const float (&v)[4]) {
+    *(vf4*)p = vf4{v[0], v[1], v[2], v[3]};
+}
+template <>
+__device__ __forceinline__ void store4<double>(double* p, const double (&v)[4]) {
+    ((vd2*)p)[0] = vd2{v[0], v[1]};
+    ((vd2*)p)[1] = vd2{v[2], v[3]};
+}
+
 template <typename VT> struct Quad;
 template <> struct Quad<float> {
     vf4 q;
@@ -150,7 +166,8 @@ __device__ __forceinline__ void stream_xrows(const typename Arith<VT>::X* __rest
 }
 
 // Row ends rs[a0+1+i] (i < na) to sm.rowend[i]; products of [b0,b1) to
-// sm.prod[k-b0].  Every load is issued (with clamped, always valid indices)
+// sm.prod[k-(b0&~3)], a whole aligned group of 4 per store.  Every load is
+// issued (with clamped, always valid indices)
 // before the first one is used, so the thread waits for memory once.  The
 // caller places the barrier.
 template <typename VT, int BS, int ITEMS>
@@ -195,12 +212,12 @@ __device__ __forceinline__ void tile_stage(const TileRegs<VT, ITEMS>& R, const i
     for (int u = 0; u < VU; ++u) {
         const int q = tid + u * BS;
         if (q < ng) {
-            const int64_t g = g0 + 4 * (int64_t)q;
+            // the whole group (entries outside [b0, b1) are neighbours' or
+            // padding, never read back)
+            Acc pr[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t k = g + e;
-                if (k >= b0 && k < b1) sm.prod[k - b0] = (Acc)R.vq[u].get(e) * (Acc)xv[u][e];
-            }
+            for (int e = 0; e < 4; ++e) pr[e] = (Acc)R.vq[u].get(e) * (Acc)xv[u][e];
+            store4<Acc>(&sm.prod[4 * q], pr);
         }
     }
 }
@@ -218,12 +235,13 @@ __device__ __forceinline__ void stream_compute(typename Arith<VT>::X* __restrict
     const int tid = threadIdx.x;
     const int na = (int)(a1 - a0);
     const int nu = (na + BS - 1) / BS;
+    const int off = (int)(b0 & 3);  // products of entry k at k - (b0 & ~3)
 #pragma unroll
     for (int u = 0; u < ITEMS; ++u) {
         const int r = tid + u * BS;
         if (u < nu && r < na) {
-            const int beg = r == 0 ? 0 : (int)(sm.rowend[r - 1] - b0);
-            const int end = (int)(sm.rowend[r] - b0);
+            const int beg = (r == 0 ? 0 : (int)(sm.rowend[r - 1] - b0)) + off;
+            const int end = (int)(sm.rowend[r] - b0) + off;
             // left to right, four LDS reads in flight at a time
             Acc s = 0;
             int k = beg;
@@ -277,6 +295,7 @@ __device__ __forceinline__ typename Arith<VT>::A merge_compute(
             hi = piv;
     }
     int ai = lo, bi = d0 - lo;
+    const int poff = (int)(b0 & 3);  // products of entry k at k - (b0 & ~3)
     const int a_start = ai;
     Acc run = 0;
     int first = -1;
@@ -306,7 +325,7 @@ __device__ __forceinline__ typename Arith<VT>::A merge_compute(
             run = 0;
             ++ai;
         } else {
-            run += sm.prod[bi];
+            run += sm.prod[bi + poff];
             ++bi;
         }
     }

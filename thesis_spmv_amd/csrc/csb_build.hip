@@ -16,6 +16,12 @@
 // same keys with one slab per block row (cell = br*H + bc), packs each index
 // as (row in block << 16) | column in block, and finds each block's pieces by
 // binary search for its Z-order prefixes.
+//
+// The swept-slab layout (csb_sweep_device) cuts the rows into nnz-balanced
+// slabs on the host (one per CU), keys entry k of slab t, column c with
+// t*W + (c >> w) (W windows of 2^w columns), sorts stably (so a window's
+// entries stay in CRS order: rows ascending) and packs each index as
+// (row in slab << w) | (c mod 2^w); soff gives every (slab, window) run.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "api_internal.hpp"

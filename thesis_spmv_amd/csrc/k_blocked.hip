@@ -13,6 +13,13 @@
 // (beta * 8 bytes, 128-256 KB) stays in the XCD's 4 MB L2 instead of each
 // gather missing to the Infinity Fabric as in a row-ordered CRS sweep over
 // random columns.
+//
+// Swept slabs (k_csb_sweep) carry this to matrices whose x outgrows the L2:
+// one persistent 1024-thread workgroup per CU owns an nnz-balanced row slab
+// (y-slice in all of the CU's LDS) and walks x in windows of 1 MB, the slab's
+// entries stored window by window.  The 32 slabs of an XCD start together and
+// sweep at about the same rate, so the XCD fetches each window about once per
+// SpMV and the gathers hit its L2; no partial sums are written.
 #include "blocked.hpp"
 #include "kernels.hpp"
 #include "merge_tile.hpp"

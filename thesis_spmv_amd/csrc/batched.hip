@@ -10,6 +10,11 @@
 // With the option bf16_when_exact an FP32 batch stores each instance whose
 // values are all exact in BF16 (every Poisson instance: 4 and -1) as BF16
 // and the others as FP32, in one launch per step.
+//
+// Tile schedule: 256 x items_per_thread merge items per tile, plain or
+// persistent tile grid; with the option autotune the first solve after an
+// upload times 6 steps of the whole batch for each candidate (re-planning the
+// tiles in place) and keeps the fastest.
 #include <math.h>
 #include <string.h>
 

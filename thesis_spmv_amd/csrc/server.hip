@@ -7,7 +7,10 @@
 // to make room), runs one chunk of steps of every running job, and retires
 // the jobs that stopped.  Jobs that arrive while others run join at the next
 // chunk; a job's arithmetic depends only on its own matrix tiles, so its
-// result is bit-identical whatever else is in flight.
+// result is bit-identical whatever else is in flight.  With the option
+// bf16_when_exact an FP32 server stores Poisson matrices and matrices whose
+// values are exact in BF16 as BF16 (the engine's tiles branch on their job's
+// storage).
 #include <math.h>
 #include <string.h>
 

@@ -570,12 +570,15 @@ def bench_batch(args):
 
 # ---------------------------------------------------------------- serve
 
-def _serve_requests(rng, n_req, rate, n_pool):
+def _serve_requests(n_req, rate, n_pool, seed=51):
     """Poisson arrivals at `rate` req/s; each request is an SpMV chain
     (length U{1..64}, 75%) or a power solve (tol 1e-7, <= 1000 steps, 25%);
     20% reuse a matrix that is resident, the rest bring a fresh copy of a
-    pool matrix (upload and tile plan included in their latency)."""
-    t = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    pool matrix (upload and tile plan included in their latency).  The job
+    mix and the unit-rate arrival gaps come from fixed seeds, so every rate
+    serves the same requests and only their arrival times scale with 1/rate."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0, n_req)) / rate
     reqs = []
     for i in range(n_req):
         chain = rng.random() < 0.75
@@ -735,7 +738,7 @@ def bench_serve(args):
 
     rows = []
     for rate in args.serve_rates:
-        reqs = _serve_requests(np.random.default_rng(int(rate)), args.serve_requests, rate, n_pool)
+        reqs = _serve_requests(args.serve_requests, rate, n_pool)
         lat, span = run_server(reqs)
         row = {"rate": rate, "requests": len(reqs), "server": {"solves_per_s": round(len(lat) / span, 2),
                                                                **_percentiles(lat)}}
@@ -758,14 +761,15 @@ def bench_serve(args):
                       "rates": args.serve_rates, "requests_per_rate": args.serve_requests},
            "p50_ms": top["server"]["p50_ms"], "p99_ms": top["server"]["p99_ms"],
            "capacity_solves_per_s": capacity,
-           "capacity_note": "the server's completion rate at saturation (the highest over the swept rates); "
-                            "rates below it keep p99 bounded by the longest job",
+           "capacity_note": "the server's completion rate at saturation (the highest over the swept rates, "
+                            "which all serve the same requests); rates below it keep p99 bounded by the longest "
+                            "job",
            "rates": rows}
     if not args.no_baseline:
         out["speedup_vs_fp64_at_top_rate"] = round(top["server"]["solves_per_s"] /
                                                    top["fp64_per_request"]["solves_per_s"], 2)
         err_rate = args.serve_rates[0]
-        reqs = _serve_requests(np.random.default_rng(int(err_rate)), args.serve_requests, err_rate, n_pool)
+        reqs = _serve_requests(args.serve_requests, err_rate, n_pool)
         out["error_vs_fp64"] = {"rate": err_rate, **result_errors(reqs)}
         log(f"# served results vs FP64 at equal steps (rate {err_rate}/s requests): {out['error_vs_fp64']}")
     return out

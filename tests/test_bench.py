@@ -1,7 +1,8 @@
 """CPU checks of bench.py's measurement contract: the algorithmic bytes it
 divides by (the per-SpMV formula b_v*Z + 4Z + 4(N+1) + 2*b_x*N, pinned to
 the configurations' closed-form sizes), its Poisson generator against the
-oracle's, and its refusal of more than one GPU."""
+oracle's, the served stream's request law, and its refusal of more than one
+GPU."""
 import os
 import subprocess
 import sys
@@ -40,3 +41,17 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
+
+
+def test_served_requests_are_the_same_at_every_rate():
+    """The served-stream sweep compares rates on one job mix: only the
+    arrival times change, in proportion to 1/rate."""
+    a = bench._serve_requests(60, 10.0, 12)
+    b = bench._serve_requests(60, 2000.0, 12)
+    strip = [{k: v for k, v in r.items() if k != "t"} for r in a]
+    assert strip == [{k: v for k, v in r.items() if k != "t"} for r in b]
+    ta, tb = np.array([r["t"] for r in a]), np.array([r["t"] for r in b])
+    assert np.all(np.diff(ta) > 0) and np.allclose(ta * 10.0, tb * 2000.0, rtol=1e-12)
+    assert {r["mat"] for r in a} <= set(range(12))
+    solves = [r for r in a if r["tol"] > 0]
+    assert all(r["steps"] == 1000 for r in solves) and all(1 <= r["steps"] <= 64 for r in a if r["tol"] < 0)

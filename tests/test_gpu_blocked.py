@@ -375,7 +375,7 @@ def test_csb_wide_streams_past_2_24_columns():
 
 @pytest.mark.parametrize("values,tol", [("f64", 1e-13), ("f32", 2e-5), ("bf16", 2e-5)])
 @pytest.mark.parametrize("opt", [{}, {"csb_window_bits": 12}, {"csb_nt": 0}, {"csb_sweep_unroll": 1},
-                                 {"csb_sweep_unroll": 4, "csb_window_bits": 12}],
+                                 {"csb_sweep_unroll": 4, "csb_window_bits": 12}, {"csb_sweep_waves": 2}],
                          ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()) or "default")
 def test_csb_swept_slabs(values, tol, opt):
     """Swept slabs on power-law (FP64/FP32) and Poisson (BF16, exact values)

@@ -82,16 +82,18 @@ def test_power_solves_converge_to_closed_form():
         assert its < 200000 and abs(lam - exact) <= 1e-9 * exact
 
 
+@pytest.mark.parametrize("overlap", [1, 0])
 @pytest.mark.parametrize("values", ["f32", "f64"])
-def test_alone_equals_in_the_mix(values):
+def test_alone_equals_in_the_mix(values, overlap):
     """A job's result is bit-identical whether it runs alone or joins while
-    other jobs are running and others arrive after it."""
+    other jobs are running and others arrive after it, with the joining jobs
+    staged while a chunk runs (overlap 1) or before it (overlap 0)."""
     rng = np.random.default_rng(5)
     A = _powerlaw(rng, 3000, long_rows=[(5, 3000)])
     B = _powerlaw(rng, 800)
 
     def fresh():
-        srv = pw.Server(values, budget_bytes=128 << 20, max_jobs=32, chunk=3)
+        srv = pw.Server(values, budget_bytes=128 << 20, max_jobs=32, chunk=3, overlap=overlap)
         a = srv.add_matrix(A.row_start, A.col_ind, A.data)
         b = srv.add_matrix(B.row_start, B.col_ind, B.data)
         p = srv.add_poisson(40, 25)
@@ -132,10 +134,17 @@ def test_budget_admission_and_eviction():
     again = _drain(srv, {j: 40000})[j]
     first = refs[min(refs)]
     assert again[0] == first[0] and np.array_equal(again[3], first[3])
-    # many jobs at once: they queue and run as memory frees up
+    # many jobs at once: they queue and run as memory frees up (staged while
+    # other jobs run, evicting only matrices no staged or running job uses)
     jobs = {srv.submit(ids[k % 3], 15, -1.0, seed=k): 40000 for k in range(9)}
     res = _drain(srv, dict(jobs))
     assert len(res) == 9
+    assert all(its == 15 and np.isfinite(lam) for lam, yn, its, vec in res.values())
+    # the first three jobs again, submitted together: identical results
+    one = {srv.submit(ids[k % 3], 20, -1.0, seed=k): 40000 for k in range(3)}
+    for j, r in _drain(srv, dict(one)).items():
+        ref = refs[min(refs) + (j - min(one))]
+        assert r[0] == ref[0] and np.array_equal(r[3], ref[3])
 
 
 def test_errors():
@@ -221,3 +230,24 @@ def test_bf16_where_exact_is_bitwise_the_pure_servers():
     with pytest.raises(pw.PwmError) as e:
         srv.step()
     assert e.value.code == -1
+
+
+@pytest.mark.parametrize("values", ["f32", "f64"])
+def test_borrowed_columns_checked_at_upload(values):
+    """A borrowed matrix whose caller moves a column index out of range after
+    registration is refused when it is uploaded (the columns are checked as
+    they are copied), before any kernel reads it; once repaired it serves."""
+    A = oracle.poisson_crs(30, 20)
+    for bad in (A.nor, -1):
+        srv = pw.Server(values, budget_bytes=16 << 20, max_jobs=2)
+        rs, ci, dv = A.row_start.astype(np.int32), A.col_ind.astype(np.int32), A.data.copy()
+        mat = srv.add_matrix(rs, ci, dv, borrow=True)
+        held = srv._held[mat][1]
+        held[len(held) // 2] = bad
+        j = srv.submit(mat, 5, -1.0, seed=1)
+        with pytest.raises(pw.PwmError, match="column index out of range") as e:
+            srv.step()
+        assert e.value.code == -1 and not srv.resident(mat)
+        held[len(held) // 2] = A.col_ind[len(held) // 2]
+        res = _drain(srv, {j: A.nor})  # the queued job runs on the repaired arrays
+        assert res[j][2] == 5

@@ -191,14 +191,12 @@ static bool stage_ready() {
     return !g_stage_failed;
 }
 
-hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (bytes == 0) return hipSuccess;
-    std::lock_guard<std::mutex> lk(g_stage_mu);
-    if (!stage_ready()) {  // no pinned memory: a plain copy from pageable memory
-        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-        return e == hipSuccess ? hipStreamSynchronize(s) : e;
-    }
-    // host threads fill one buffer while the other is in flight
+// Host -> device through the pinned pair: fill(stage, a0, a1) writes bytes
+// [a0, a1) of the transfer into `stage` (called by host threads on pieces of
+// a chunk); host threads fill one buffer while the other is in flight.  The
+// caller holds g_stage_mu and stage_ready() is true.
+template <typename Fill>
+static hipError_t staged_upload(void* dst, size_t bytes, hipStream_t s, Fill fill) {
     hipError_t e = hipSuccess;
     bool busy[2] = {false, false};
     int b = 0;
@@ -208,9 +206,7 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
         const size_t n = std::min(chunk, bytes - off);
         if (busy[b] && (e = hipEventSynchronize(g_stage_ev[b])) != hipSuccess) break;
         char* stage = (char*)g_stage[b];
-        const char* from = (const char*)src + off;
-        parallel_for((int64_t)n, piece,
-                     [&](int64_t a0, int64_t a1, int) { memcpy(stage + a0, from + a0, (size_t)(a1 - a0)); });
+        parallel_for((int64_t)n, piece, [&](int64_t a0, int64_t a1, int) { fill(stage + a0, off + a0, off + a1); });
         e = hipMemcpyAsync((char*)dst + off, stage, n, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(g_stage_ev[b], s);
         busy[b] = true;
@@ -218,6 +214,87 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
     // the buffers are the next upload's: wait until the copies have read them
     const hipError_t e2 = hipStreamSynchronize(s);
     return e != hipSuccess ? e : e2;
+}
+
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!stage_ready()) {  // no pinned memory: a plain copy from pageable memory
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    const char* from = (const char*)src;
+    return staged_upload(dst, bytes, s,
+                         [from](char* to, size_t a0, size_t a1) { memcpy(to, from + a0, a1 - a0); });
+}
+
+hipError_t upload_indices_staged(void* dst, const int32_t* src, int64_t n, int64_t bound, bool* out_of_range,
+                                 hipStream_t s) {
+    std::atomic<bool> bad{false};
+    *out_of_range = false;
+    if (n <= 0) return hipSuccess;
+    const uint64_t ub = (uint64_t)bound;
+    auto scan = [&bad, ub](const int32_t* p, int64_t m) {
+        bool b = false;
+        for (int64_t k = 0; k < m; ++k) b |= (uint64_t)(uint32_t)p[k] >= ub;  // negative values wrap above bound
+        if (b) bad = true;
+    };
+    hipError_t e;
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        if (!stage_ready()) {  // no pinned memory: check, then a plain copy from pageable memory
+            parallel_for(n, 1 << 20, [&](int64_t a0, int64_t a1, int) { scan(src + a0, a1 - a0); });
+            if (bad) {
+                *out_of_range = true;
+                return hipSuccess;
+            }
+            e = hipMemcpyAsync(dst, src, (size_t)n * 4, hipMemcpyHostToDevice, s);
+            return e == hipSuccess ? hipStreamSynchronize(s) : e;
+        }
+        // pieces start and end on whole indices (multiples of 64 KB)
+        const char* from = (const char*)src;
+        e = staged_upload(dst, (size_t)n * 4, s, [from, &scan](char* to, size_t a0, size_t a1) {
+            memcpy(to, from + a0, a1 - a0);
+            scan((const int32_t*)to, (int64_t)((a1 - a0) / 4));
+        });
+    }
+    *out_of_range = bad;
+    return e;
+}
+
+// FP64 value k as stored: FP32, or BF16 as the upper half of the FP32 value
+// (exact for values exact in BF16, which the callers check).
+static inline void narrow_value(const double* v, int vt, int64_t k, char* out) {
+    const float f = (float)v[k];
+    if (vt == PWM_VALUES_F32) {
+        memcpy(out, &f, 4);
+    } else {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        const uint16_t h = (uint16_t)(u >> 16);
+        memcpy(out, &h, 2);
+    }
+}
+
+hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s) {
+    if (vt == PWM_VALUES_F64) return upload_staged(dst, src, (size_t)n * 8, s);
+    if (n <= 0) return hipSuccess;
+    const size_t es = vt == PWM_VALUES_F32 ? 4 : 2;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    if (!stage_ready()) {  // no pinned memory: narrow into host memory, then a plain copy
+        std::vector<char> tmp((size_t)n * es);
+        char* o = tmp.data();
+        parallel_for(n, 1 << 20, [&](int64_t a0, int64_t a1, int) {
+            for (int64_t k = a0; k < a1; ++k) narrow_value(src, vt, k, o + k * es);
+        });
+        hipError_t e = hipMemcpyAsync(dst, o, tmp.size(), hipMemcpyHostToDevice, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    // pieces of a chunk start and end on whole values (chunks and pieces are
+    // multiples of 256 KB, and every range below is split at multiples of es)
+    return staged_upload(dst, (size_t)n * es, s, [src, vt, es](char* to, size_t a0, size_t a1) {
+        for (size_t k = a0 / es; k < a1 / es; ++k) narrow_value(src, vt, (int64_t)k, to + (k - a0 / es) * es);
+    });
 }
 
 hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
@@ -666,6 +743,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_sweep_unroll") {
         if (value != 1 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_sweep_unroll must be 1, 2 or 4");
         h->csb_sweep_unroll = (int32_t)value;
+    } else if (k == "csb_sweep_waves") {
+        if (value < 1 || value > 8) return fail(h, PWM_ERR_INVALID, "csb_sweep_waves must be 1..8");
+        h->csb_sweep_waves = (int32_t)value;
     } else if (k == "coh_occ8") {
         if (value < -1 || value > 1) return fail(h, PWM_ERR_INVALID, "coh_occ8 must be -1 (timed), 0 or 1");
         h->coh_occ8 = (int32_t)value;
@@ -704,13 +784,8 @@ static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
     hipError_t e = alloc_padded(h->val, nnz, value_size(h->vtype));
     if (e != hipSuccess) return hip_fail(h, e, "value allocation");
     if (nnz == 0) return PWM_OK;
-    if (h->vtype == PWM_VALUES_F64) {
-        e = upload_staged(h->val.p, vals, nnz * 8, h->stream);
-        if (e != hipSuccess) return hip_fail(h, e, "value upload");
-        return PWM_OK;
-    }
-    if (h->vtype == PWM_VALUES_BF16) {
-        for (int64_t i = 0; i < nnz; ++i) {
+    if (h->vtype == PWM_VALUES_BF16 && !exact_in_bf16(vals, nnz)) {
+        for (int64_t i = 0; i < nnz; ++i) {  // name the first value that is not
             const float f = (float)vals[i];
             uint32_t b;
             memcpy(&b, &f, 4);
@@ -719,13 +794,8 @@ static int upload_values(pwm_matrix* h, const double* vals, int64_t nnz) {
                                                     std::to_string(i) + ")");
         }
     }
-    DevBuf tmp;
-    if ((e = tmp.alloc(nnz * 8)) != hipSuccess) return hip_fail(h, e, "staging allocation");
-    if ((e = upload_staged(tmp.p, vals, nnz * 8, h->stream)) != hipSuccess)
-        return hip_fail(h, e, "value upload");
-    e = pwmk::launch_convert_values(tmp.as<double>(), h->val.p, nnz, h->vtype, h->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    if (e != hipSuccess) return hip_fail(h, e, "value conversion");
+    e = upload_values_staged(h->val.p, vals, nnz, h->vtype, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "value upload");
     return PWM_OK;
 }
 

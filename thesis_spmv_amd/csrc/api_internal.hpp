@@ -198,6 +198,7 @@ struct pwm_matrix {
     int32_t csb_window_bits = 0;  // swept slabs: log2 of the columns per window (0 = 1 MB of x)
     int32_t csb_sync_every = 2;   // swept slabs: windows between workgroup barriers
     int32_t csb_sweep_unroll = 2; // swept slabs: groups of 4 entries per lane per chunk (1, 2 or 4)
+    int32_t csb_sweep_waves = 1;  // swept slabs: slabs per CU (workgroups run in that many waves)
     int32_t coh_occ8 = -1;        // BlockCOH kernel at 8 waves per SIMD: -1 timed at load, 0 no, 1 yes
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)
@@ -223,6 +224,15 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // threads fill one while the other is in flight); returns once the copy has
 // completed.  A plain copy when pinned memory cannot be allocated.
 hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
+// n FP64 host values uploaded as stored in value type vt (FP64, FP32, or
+// BF16 for values exact in BF16), narrowed by host threads into the pinned
+// buffers on the way
+hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s);
+// n int32 indices uploaded as they are, each checked on the way against
+// [0, bound): *out_of_range tells whether one was not (the device copy is
+// then complete but must not be used)
+hipError_t upload_indices_staged(void* dst, const int32_t* src, int64_t n, int64_t bound, bool* out_of_range,
+                                 hipStream_t s);
 // Device -> host the same way (after the work queued on s).
 hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
 // CSB slab-stream layout built on the device from the device CRS arrays

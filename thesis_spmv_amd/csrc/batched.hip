@@ -233,10 +233,6 @@ static int batch_upload(pwm_batch* b) {
     }
     hipError_t e = b->arena.reserve(total);
     if (e != hipSuccess) return bhip(b, e, "batch allocation");
-    int64_t max_stage = 0;
-    for (const BatchInst& I : b->inst) max_stage = std::max<int64_t>(max_stage, (int64_t)I.val.size());
-    pwm_impl::DevBuf tmp;
-    if (max_stage > 0 && (e = tmp.alloc(max_stage * 8)) != hipSuccess) return bhip(b, e, "batch staging");
     Arena& A = b->arena;
     for (BatchInst& I : b->inst) {
         const int vt = I.vt16 ? PWM_VALUES_BF16 : b->vtype;
@@ -255,10 +251,7 @@ static int batch_upload(pwm_batch* b) {
         if (I.kind == 0) {
             e = pwm_impl::upload_staged(rs, I.rs.data(), (I.n + 1) * 4, b->stream);
             if (e == hipSuccess) e = pwm_impl::upload_staged(A.at(I.o_ci), I.ci.data(), I.nnz * 4, b->stream);
-            if (e == hipSuccess) e = pwm_impl::upload_staged(tmp.p, I.val.data(), I.nnz * 8, b->stream);
-            if (e == hipSuccess && I.nnz)
-                e = pwmk::launch_convert_values(tmp.as<double>(), A.at(I.o_val), I.nnz, vt, b->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+            if (e == hipSuccess) e = pwm_impl::upload_values_staged(A.at(I.o_val), I.val.data(), I.nnz, vt, b->stream);
         } else {
             e = pwmk::launch_fill_poisson(I.pm, I.pn, rs, (int32_t*)A.at(I.o_ci), A.at(I.o_val), vt, b->stream);
         }

@@ -183,7 +183,8 @@ static int time_csb(pwm_matrix* h, float* ms) {
 
 // Swept-slab layout: as many nnz-balanced slabs as the chip has CUs (one
 // persistent workgroup of 1024 threads each, with all of the CU's LDS for its
-// y-slice), windows of 1 MB of x unless csb_window_bits says otherwise.
+// y-slice), or csb_sweep_waves times as many (the workgroups then run in that
+// many waves), windows of 1 MB of x unless csb_window_bits says otherwise.
 // PWM_ERR_UNSUPPORTED when the slabs' rows cannot fit.
 static int build_swept(pwm_matrix* h, const int32_t* rs) {
     CsbDev& c = h->csb;
@@ -192,21 +193,22 @@ static int build_swept(pwm_matrix* h, const int32_t* rs) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 0);
     if (e != hipSuccess) return hip_fail(h, e, "CSB swept slabs: device attributes");
     const int acc = h->vtype == PWM_VALUES_F64 ? 8 : 4;
+    const int64_t slabs = (int64_t)std::max(1, cus) * h->csb_sweep_waves;
     int wb = h->csb_window_bits;
     if (wb == 0) {
         wb = 10;
         while (((int64_t)acc << (wb + 1)) <= ((int64_t)1 << 20)) ++wb;
         // the packed index keeps 32 - wb bits for the row in the slab: narrow
         // the window while the slabs' expected rows (a fifth of slack) need more
-        const int64_t rows_needed = (h->nor + cus - 1) / std::max(1, cus) * 6 / 5;
+        const int64_t rows_needed = (h->nor + slabs - 1) / slabs * 6 / 5;
         while (wb > 12 && (((int64_t)1 << (32 - wb)) - 1) < rows_needed) --wb;
     }
     const int64_t rows_cap = std::min<int64_t>(((lds_cu - 1024) / acc) & ~(int64_t)63, ((int64_t)1 << (32 - wb)) - 1);
     if (rows_cap <= 0 || blocked::csb_sweep_blocks_per_cu(h->vtype, (int)rows_cap) < 1)
         return fail(h, PWM_ERR_UNSUPPORTED, "CSB swept slabs: no workgroup with LDS rows fits a CU");
-    const int rc = csb_sweep_device(h, rs, std::max(1, cus), rows_cap, cus, wb);
+    const int rc = csb_sweep_device(h, rs, slabs, rows_cap, slabs, wb);
     if (rc == PWM_ERR_UNSUPPORTED)
-        return fail(h, rc, "CSB swept slabs: the rows do not fit " + std::to_string(cus) + " slabs of at most " +
+        return fail(h, rc, "CSB swept slabs: the rows do not fit " + std::to_string(slabs) + " slabs of at most " +
                                std::to_string(rows_cap) + " rows (the LDS of one workgroup per CU)");
     if (rc != PWM_OK) return rc;
     c.nt = h->csb_nt >= 0 ? h->csb_nt : 1;

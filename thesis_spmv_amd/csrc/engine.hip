@@ -685,6 +685,18 @@ int Engine::build_lists(std::string& err) {
 
 int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
     finished.clear();
+    const int rc = launch(steps, err);
+    if (rc != PWM_OK) {
+        if (pending_) {  // let what was queued finish before the caller goes on
+            (void)hipStreamSynchronize(stream_);
+            pending_ = false;
+        }
+        return rc;
+    }
+    return collect(finished, err);
+}
+
+int Engine::launch(int steps, std::string& err) {
     if (n_active_ == 0) return PWM_OK;
     // no more steps than any running job can still take (max_it bounds it)
     int64_t most = 1;
@@ -759,7 +771,19 @@ int Engine::run(int steps, std::vector<int>& finished, std::string& err) {
     launches += 2 * steps;
     if (e == hipSuccess)
         e = hipMemcpyAsync(h_state_, d_state_, (size_t)max_slots_ * sizeof(pwmk::EState), hipMemcpyDeviceToHost, stream_);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    pending_ = true;  // collect() waits for the stream even after an error
+    if (e != hipSuccess) {
+        err = std::string("engine step: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
+    }
+    return PWM_OK;
+}
+
+int Engine::collect(std::vector<int>& finished, std::string& err) {
+    finished.clear();
+    if (!pending_) return PWM_OK;
+    pending_ = false;
+    const hipError_t e = hipStreamSynchronize(stream_);
     if (e != hipSuccess) {
         err = std::string("engine step: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;

@@ -160,6 +160,14 @@ class Engine {
     // the slots whose jobs stopped; their state is in state(slot) and the
     // final iterate (not yet normalised) in buffer last_buffer(slot).
     int run(int steps, std::vector<int>& finished, std::string& err);
+    // run() in two halves: launch() queues the steps and the state download
+    // and returns at once; collect() waits for them and lists the finished
+    // slots.  Between the two the host may do other work, but must not call
+    // add(), release(), set_ipt() or launch() (the download is writing the
+    // state mirror).
+    int launch(int steps, std::string& err);
+    int collect(std::vector<int>& finished, std::string& err);
+    bool pending() const { return pending_; }
     const pwmk::EState& state(int slot) const { return h_state_[slot]; }
     const pwmk::EJob& job(int slot) const { return h_jobs_[slot]; }
     void* last_buffer(int slot) const;
@@ -185,6 +193,7 @@ class Engine {
     size_t work_cap_alloc_ = 0;
     std::vector<char> in_use_, running_, fresh_;
     int n_active_ = 0;
+    bool pending_ = false;  // launched steps not yet collected
     bool lists_dirty_ = true;
     int n_work_ = 0, n_slots_ = 0, n_stale_work_ = 0;
     int grid_work_ = 0, grid_slots_ = 0;

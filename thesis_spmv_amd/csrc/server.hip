@@ -5,9 +5,11 @@
 // and device memory are free (memory comes from one arena of `budget`
 // bytes; matrices no running job uses are evicted least recently used first
 // to make room), runs one chunk of steps of every running job, and retires
-// the jobs that stopped.  Jobs that arrive while others run join at the next
-// chunk; a job's arithmetic depends only on its own matrix tiles, so its
-// result is bit-identical whatever else is in flight.  With the option
+// the jobs that stopped.  While a chunk runs, the queued jobs that free slots
+// can take are staged on a second stream (matrix upload or generation, tile
+// plan, job block and start vector), so uploads overlap the running jobs;
+// they join at the next chunk.  A job's arithmetic depends only on its own
+// matrix tiles, so its result is bit-identical whatever else is in flight.  With the option
 // bf16_when_exact an FP32 server stores Poisson matrices and matrices whose
 // values are exact in BF16 as BF16 (the engine's tiles branch on their job's
 // storage).
@@ -55,7 +57,7 @@ struct SrvJob {
     int state = 0;  // 0 queued, 1 running, 2 finished
     int slot = -1;
     size_t off = 0, bytes = 0;
-    bool has_block = false;
+    bool has_block = false;  // staged: job block allocated, start vector written, matrix referenced
     double lambda = 0.0, ynorm = 0.0;
     int64_t iters = 0;
     void* last = nullptr;
@@ -68,7 +70,9 @@ struct pwm_server {
     int ipt = 16;
     int chunk = 8;
     int use_graph = 1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the engine's steps, retirement, result downloads
+    hipStream_t ustream = nullptr;  // staging: uploads, Poisson generation, tile plans, start vectors
+    int overlap = 1;                // stage queued jobs while a chunk runs
     std::string err;
     pwm_impl::Arena arena;
     std::unique_ptr<pwm_impl::Engine> eng;
@@ -136,10 +140,12 @@ extern "C" int pwm_server_create(int value_type, int64_t budget_bytes, int32_t m
     pwm_server* s = new pwm_server();
     s->vtype = value_type;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->ustream, hipStreamNonBlocking);
     if (e == hipSuccess) e = s->arena.reserve((size_t)budget_bytes);
     if (e != hipSuccess) {
         const int rc = e == hipErrorOutOfMemory ? PWM_ERR_NOMEM : PWM_ERR_HIP;
         if (s->stream) (void)hipStreamDestroy(s->stream);
+        if (s->ustream) (void)hipStreamDestroy(s->ustream);
         delete s;
         return sfail(nullptr, rc, std::string("pwm_server_create: ") + hipGetErrorString(e));
     }
@@ -149,6 +155,7 @@ extern "C" int pwm_server_create(int value_type, int64_t budget_bytes, int32_t m
     if (rc != PWM_OK) {
         s->eng.reset();
         (void)hipStreamDestroy(s->stream);
+        (void)hipStreamDestroy(s->ustream);
         delete s;
         return sfail(nullptr, rc, err);
     }
@@ -160,10 +167,12 @@ extern "C" int pwm_server_create(int value_type, int64_t budget_bytes, int32_t m
 
 extern "C" int pwm_server_destroy(pwm_server* s) {
     if (!s) return PWM_OK;
-    hipStream_t st = s->stream;
+    hipStream_t st = s->stream, ust = s->ustream;
     if (st) (void)hipStreamSynchronize(st);
+    if (ust) (void)hipStreamSynchronize(ust);
     delete s;  // engine and arena free their device memory here
     if (st) (void)hipStreamDestroy(st);
+    if (ust) (void)hipStreamDestroy(ust);
     return PWM_OK;
 }
 
@@ -181,6 +190,10 @@ extern "C" int pwm_server_set_option(pwm_server* s, const char* key, int64_t val
     if (k == "graph") {
         s->use_graph = value != 0;
         s->eng->use_graph = s->use_graph != 0;
+        return PWM_OK;
+    }
+    if (k == "overlap") {
+        s->overlap = value != 0;
         return PWM_OK;
     }
     if (k == "persistent") {
@@ -232,20 +245,21 @@ static int add_matrix_common(pwm_server* s, SrvMatrix&& M, int32_t* mat) {
 }
 
 // Checks a host CRS matrix of n rows with `nnz` entries before it is
-// uploaded: non-decreasing row_start from 0 to nnz, columns in [0, n), and
-// for BF16 storage values exact in BF16.  Runs on the host thread pool.
+// uploaded: non-decreasing row_start from 0 to nnz, columns in [0, n) (unless
+// check_cols is false: the upload checks them), and for BF16 storage values
+// exact in BF16.  Runs on the host thread pool.
 static int check_crs(pwm_server* s, int64_t n, int64_t nnz, const int32_t* rs, const int32_t* ci, const double* vals,
-                     const char* who, bool need_bf16) {
+                     const char* who, bool need_bf16, bool check_cols = true) {
     if (rs[0] != 0 || rs[n] != nnz) return sfail(s, PWM_ERR_INVALID, std::string(who) + ": bad row_start");
     std::atomic<int> bad{0};  // 1 row_start, 2 column, 3 BF16 value
     pwm_impl::parallel_for(n, 1 << 18, [&](int64_t b, int64_t e, int) {
         for (int64_t r = b; r < e; ++r)
             if (rs[r + 1] < rs[r]) bad = 1;
     });
-    if (bad == 0)
+    if (bad == 0 && (check_cols || need_bf16))
         pwm_impl::parallel_for(nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
             for (int64_t k = b; k < e; ++k) {
-                if (ci[k] < 0 || ci[k] >= n) {
+                if (check_cols && (ci[k] < 0 || ci[k] >= n)) {
                     bad = 2;
                     return;
                 }
@@ -347,71 +361,42 @@ static bool alloc_evicting(pwm_server* s, size_t bytes, size_t* off, int keep) {
     }
 }
 
+// Upload (or generate) matrix `mi` into its arena block and plan its tiles,
+// on the staging stream; returns once they are done.
 static int make_resident(pwm_server* s, int mi) {
     SrvMatrix& M = s->mats[mi];
     char* base = (char*)s->arena.at(M.off);
     int32_t* rs = (int32_t*)base;
     const size_t vs = matrix_value_size(s, M);
     const int vt = M.vt16 ? PWM_VALUES_BF16 : s->vtype;  // storage of this matrix
-    hipError_t e = hipMemsetAsync(base + M.o_ci + M.nnz * 4, 0, pwm_impl::kPadEntries * 4, s->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(base + M.o_val + M.nnz * vs, 0, pwm_impl::kPadEntries * vs, s->stream);
+    hipError_t e = hipMemsetAsync(base + M.o_ci + M.nnz * 4, 0, pwm_impl::kPadEntries * 4, s->ustream);
+    if (e == hipSuccess) e = hipMemsetAsync(base + M.o_val + M.nnz * vs, 0, pwm_impl::kPadEntries * vs, s->ustream);
     if (e == hipSuccess && M.kind == 1) {
-        e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, vt, s->stream);
+        e = pwmk::launch_fill_poisson(M.pm, M.pn, rs, (int32_t*)(base + M.o_ci), base + M.o_val, vt, s->ustream);
     } else if (e == hipSuccess) {
         if (M.borrowed) {
             // borrowed arrays may have changed since registration: check them
             // again before the kernels index x with them (and, for BF16
-            // storage, before their values are narrowed)
+            // storage, before their values are narrowed); the columns are
+            // checked as they are copied
             const int rc = check_crs(s, M.n, M.nnz, M.rs_p, M.ci_p, M.val_p, "server: borrowed matrix",
-                                     vt == PWM_VALUES_BF16);
+                                     vt == PWM_VALUES_BF16, false);
             if (rc != PWM_OK) return rc;
         }
-        e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->stream);
-        if (e == hipSuccess) e = pwm_impl::upload_staged(base + M.o_ci, M.ci_p, M.nnz * 4, s->stream);
-        if (e == hipSuccess && M.nnz) {
-            if (vt == PWM_VALUES_F64) {
-                e = pwm_impl::upload_staged(base + M.o_val, M.val_p, M.nnz * 8, s->stream);
-            } else {
-                // narrow on the host (FP32, or BF16 = the upper half of the
-                // exact FP32 value), then one copy
-                std::vector<float> f32;
-                std::vector<uint16_t> b16;
-                const void* src;
-                size_t nb;
-                const double* v = M.val_p;
-                if (vt == PWM_VALUES_F32) {
-                    f32.resize(M.nnz);
-                    float* o = f32.data();
-                    pwm_impl::parallel_for(M.nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
-                        for (int64_t k = b; k < e; ++k) o[k] = (float)v[k];
-                    });
-                    src = f32.data();
-                    nb = M.nnz * 4;
-                } else {
-                    b16.resize(M.nnz);
-                    uint16_t* o = b16.data();
-                    pwm_impl::parallel_for(M.nnz, 1 << 20, [&](int64_t b, int64_t e, int) {
-                        for (int64_t k = b; k < e; ++k) {
-                            const float f = (float)v[k];
-                            uint32_t u;
-                            memcpy(&u, &f, 4);
-                            o[k] = (uint16_t)(u >> 16);
-                        }
-                    });
-                    src = b16.data();
-                    nb = M.nnz * 2;
-                }
-                e = pwm_impl::upload_staged(base + M.o_val, src, nb, s->stream);  // returns once copied
-            }
-        }
+        bool bad_col = false;
+        e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->ustream);
+        if (e == hipSuccess) e = pwm_impl::upload_indices_staged(base + M.o_ci, M.ci_p, M.nnz, M.n, &bad_col, s->ustream);
+        if (e == hipSuccess && bad_col)
+            return sfail(s, PWM_ERR_INVALID, "server: borrowed matrix: column index out of range");
+        if (e == hipSuccess) e = pwm_impl::upload_values_staged(base + M.o_val, M.val_p, M.nnz, vt, s->ustream);
     }
     if (e == hipSuccess)
-        e = pwmk::engine_plan_tiles(rs, M.n, M.nnz, s->ipt, (pwmk::ETile*)(base + M.o_tiles), s->stream);
+        e = pwmk::engine_plan_tiles(rs, M.n, M.nnz, s->ipt, (pwmk::ETile*)(base + M.o_tiles), s->ustream);
     M.tiles.resize(M.n_tiles);
     if (e == hipSuccess)
         e = hipMemcpyAsync(M.tiles.data(), base + M.o_tiles, (size_t)M.n_tiles * sizeof(pwmk::ETile),
-                           hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+                           hipMemcpyDeviceToHost, s->ustream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ustream);
     if (e != hipSuccess) return shp(s, e, "matrix upload");
     M.resident = true;
     return PWM_OK;
@@ -433,33 +418,66 @@ extern "C" int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, dou
     return PWM_OK;
 }
 
-// Admit queued jobs in arrival order; stop at the first that does not fit.
-static int admit(pwm_server* s) {
-    const size_t xs = pwm_impl::vector_size(s->vtype);
-    while (!s->queue.empty()) {
-        if (s->eng->active() >= s->eng->max_slots()) break;
-        const int64_t id = s->queue.front();
-        SrvJob& J = s->jobs[id];
+// Stage up to `max_jobs` queued jobs in arrival order (jobs staged before
+// count and are skipped): make each one's matrix resident and give it a job
+// block with its start vector, on the staging stream.  A staged job holds a
+// reference to its matrix, so the matrix is not evicted before the job ends.
+// Stops at the first job that does not fit; returns once the staging stream
+// has finished.
+static int stage(pwm_server* s, size_t max_jobs) {
+    int rc = PWM_OK;
+    bool wrote = false;
+    size_t k = 0;
+    for (auto it = s->queue.begin(); it != s->queue.end() && k < max_jobs; ++it, ++k) {
+        SrvJob& J = s->jobs[*it];
+        if (J.has_block) continue;
         SrvMatrix& M = s->mats[J.mat];
         if (!M.resident) {
             if (!alloc_evicting(s, M.bytes, &M.off, J.mat)) break;
-            const int rc = make_resident(s, J.mat);
+            rc = make_resident(s, J.mat);
             ++s->uploads;
             if (rc != PWM_OK) {
                 s->arena.release(M.off, M.bytes);
-                return rc;
+                break;
             }
         }
+        M.refs++;
+        M.last_use = ++s->clock;
         const size_t jb = job_bytes(s, M);
-        if (!alloc_evicting(s, jb, &J.off, J.mat)) break;
+        if (!alloc_evicting(s, jb, &J.off, J.mat)) {
+            M.refs--;
+            break;
+        }
         J.bytes = jb;
         J.has_block = true;
+        wrote = true;
+        const hipError_t e = pwmk::engine_start_vector(s->arena.at(J.off), M.n, s->vtype, J.seed, 0, s->ustream);
+        if (e != hipSuccess) {
+            rc = shp(s, e, "job start vector");
+            break;
+        }
+    }
+    if (wrote) {
+        const hipError_t e = hipStreamSynchronize(s->ustream);
+        if (e != hipSuccess && rc == PWM_OK) rc = shp(s, e, "job staging");
+    }
+    return rc;
+}
+
+// Admit queued jobs in arrival order while engine slots are free, staging
+// those not staged yet; stop at the first that does not fit.
+static int admit(pwm_server* s) {
+    const size_t xs = pwm_impl::vector_size(s->vtype);
+    const int rc = stage(s, (size_t)std::max(0, s->eng->max_slots() - s->eng->active()));
+    while (!s->queue.empty() && s->eng->active() < s->eng->max_slots()) {
+        const int64_t id = s->queue.front();
+        SrvJob& J = s->jobs[id];
+        if (!J.has_block) break;  // did not fit
+        SrvMatrix& M = s->mats[J.mat];
         char* base = (char*)s->arena.at(J.off);
         const char* mb = (const char*)s->arena.at(M.off);
         const size_t o_part = pwm_impl::Arena::round(2 * M.n * xs);
         const size_t o_carry = o_part + pwm_impl::Arena::round((size_t)M.n_tiles * 24);
-        hipError_t e = pwmk::engine_start_vector(base, M.n, s->vtype, J.seed, 0, s->stream);
-        if (e != hipSuccess) return shp(s, e, "job start vector");
         pwmk::EJob j;
         memset(&j, 0, sizeof(j));
         j.rs = (const int32_t*)mb;
@@ -477,39 +495,18 @@ static int admit(pwm_server* s) {
         j.n_tiles = M.n_tiles;
         j.vt16 = M.vt16;
         const int slot = s->eng->add(j, M.tiles.data());
-        if (slot < 0) {  // cannot happen: a slot was free above
-            s->arena.release(J.off, J.bytes);
-            J.has_block = false;
-            break;
-        }
+        if (slot < 0) break;  // cannot happen: a slot was free above
         J.slot = slot;
         J.state = 1;
         s->slot_job[slot] = id;
-        M.refs++;
-        M.last_use = ++s->clock;
         s->queue.pop_front();
     }
-    return PWM_OK;
+    return rc;
 }
 
-extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
-    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
-    const double t0 = now_s();
-    int rc = admit(s);
-    const double t1 = now_s();
-    s->t_admit += t1 - t0;
-    if (rc != PWM_OK) return rc;
-    if (s->eng->active() == 0 && !s->queue.empty())
-        return sfail(s, PWM_ERR_BUDGET,
-                     "pwm_server_step: the next job does not fit the budget with nothing running "
-                     "(collect finished results to free their memory)");
-    std::vector<int> fin;
-    if (s->eng->active() > 0) {
-        std::string err;
-        if ((rc = s->eng->run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
-    }
-    const double t2 = now_s();
-    s->t_run += t2 - t1;
+// Record the finished slots' results, normalise their final iterates and
+// free their engine slots.
+static int retire(pwm_server* s, const std::vector<int>& fin) {
     hipError_t e = hipSuccess;
     s->last_finished.clear();
     for (int slot : fin) {
@@ -530,7 +527,57 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
     }
     if (e == hipSuccess && !fin.empty()) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return shp(s, e, "job retirement");
-    s->t_retire += now_s() - t2;
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
+    std::vector<int> fin;
+    std::string err;
+    int rc = PWM_OK;
+    if (s->eng->active() > 0) {
+        // one chunk of the running jobs; while it runs, the queued jobs that
+        // the free slots can take are staged, and they join at the next step
+        const double t0 = now_s();
+        rc = s->eng->launch(s->chunk, err);
+        double staged = 0.0;
+        int stage_rc = PWM_OK;
+        if (rc == PWM_OK && s->overlap) {
+            const double ts = now_s();
+            stage_rc = stage(s, (size_t)(s->eng->max_slots() - s->eng->active()));
+            staged = now_s() - ts;
+        }
+        const int crc = s->eng->collect(fin, err);  // waits even after a failed launch
+        if (rc == PWM_OK) rc = crc;
+        const double t1 = now_s();
+        s->t_admit += staged;
+        s->t_run += t1 - t0 - staged;
+        if (rc != PWM_OK) return sfail(s, rc, err);
+        if ((rc = retire(s, fin)) != PWM_OK) return rc;
+        const double t2 = now_s();
+        s->t_retire += t2 - t1;
+        if (stage_rc != PWM_OK) return stage_rc;
+        rc = admit(s);
+        s->t_admit += now_s() - t2;
+        if (rc != PWM_OK) return rc;
+    } else {
+        const double t0 = now_s();
+        rc = admit(s);
+        const double t1 = now_s();
+        s->t_admit += t1 - t0;
+        if (rc != PWM_OK) return rc;
+        if (s->eng->active() == 0 && !s->queue.empty())
+            return sfail(s, PWM_ERR_BUDGET,
+                         "pwm_server_step: the next job does not fit the budget with nothing running "
+                         "(collect finished results to free their memory)");
+        if (s->eng->active() > 0) {
+            if ((rc = s->eng->run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
+        }
+        const double t2 = now_s();
+        s->t_run += t2 - t1;
+        if ((rc = retire(s, fin)) != PWM_OK) return rc;
+        s->t_retire += now_s() - t2;
+    }
     if (active) *active = s->eng->active();
     if (finished) *finished = (int32_t)fin.size();
     return PWM_OK;

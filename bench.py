@@ -614,15 +614,15 @@ def bench_serve(args):
         kind = kinds[i % 3]
         if kind == "poisson":
             side = max(2, int(np.sqrt(n)))
-            pool.append({"kind": kind, "side": side, "n": side * side})
+            pool.append({"kind": kind, "side": side, "n": side * side, "nnz": 5 * side * side - 4 * side})
         elif kind == "powerlaw":
             rs, ci, dv = pw.gen_powerlaw(int(n), alpha=1.8, xm=7.3333, cap=50000, struct_seed=100 + i,
                                          value_seed=200 + i)
-            pool.append({"kind": kind, "n": int(n), "crs": (rs, ci, dv)})
+            pool.append({"kind": kind, "n": int(n), "nnz": len(ci), "crs": (rs, ci, dv)})
         else:
             side = max(3, int(np.sqrt(n)))
             rs, ci, dv = pw.gen_stencil9(side, 2, 300 + i)
-            pool.append({"kind": kind, "n": side * side, "crs": (rs, ci, dv)})
+            pool.append({"kind": kind, "n": side * side, "nnz": len(ci), "crs": (rs, ci, dv)})
     log(f"# serve pool: {n_pool} matrices, N {min(p['n'] for p in pool)}..{max(p['n'] for p in pool)}, "
         f"built in {time.time() - t0:.1f}s")
 
@@ -635,6 +635,7 @@ def bench_serve(args):
                         bf16_when_exact=args.serve_bf16_exact)
         resident_ids = {}  # pool index -> list of server matrix ids
         lat, jobs, pending, k = [], {}, list(reqs), 0
+        served_bytes = 0  # algorithmic bytes of every step served (FP32 values and vectors)
         start = time.perf_counter()
         while k < len(pending) or jobs:
             now = time.perf_counter() - start
@@ -656,14 +657,16 @@ def bench_serve(args):
             done_at = time.perf_counter() - start
             for j in srv.finished():
                 r = jobs.pop(j)
-                srv.result(j)
+                its = srv.result(j)[2]
+                p = pool[r["mat"]]
+                served_bytes += its * algorithmic_bytes(p["n"], p["nnz"], 4, 4)
                 lat.append(done_at - r["t"])
         makespan = time.perf_counter() - start
         tm = srv.timing()
         log(f"#   server phases: admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), run {tm['run_s']:.2f}s, "
-            f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s")
+            f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s; {served_bytes / 1e9:.1f} GB of steps served")
         del srv
-        return lat, makespan
+        return lat, makespan, served_bytes
 
     def fp64_handle(p):
         m = pw.CRSHip(-1, "f64")
@@ -739,9 +742,11 @@ def bench_serve(args):
     rows = []
     for rate in args.serve_rates:
         reqs = _serve_requests(args.serve_requests, rate, n_pool)
-        lat, span = run_server(reqs)
+        lat, span, served = run_server(reqs)
         row = {"rate": rate, "requests": len(reqs), "server": {"solves_per_s": round(len(lat) / span, 2),
-                                                               **_percentiles(lat)}}
+                                                               **_percentiles(lat),
+                                                               "served_GB": round(served / 1e9, 2),
+                                                               "effective_GBps": round(served / span / 1e9, 1)}}
         log(f"# rate {rate}/s: server {row['server']}")
         if not args.no_baseline:
             lat64, span64 = run_fp64(reqs)
@@ -764,6 +769,12 @@ def bench_serve(args):
            "capacity_note": "the server's completion rate at saturation (the highest over the swept rates, "
                             "which all serve the same requests); rates below it keep p99 bounded by the longest "
                             "job",
+           "roofline": {"bound": "hbm", "achieved": top["server"]["effective_GBps"], "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(top["server"]["effective_GBps"] / HBM_PEAK_GBS, 4),
+                        "traffic": None,
+                        "timed": "the top rate's whole served stream: algorithmic bytes of every step served "
+                                 "(FP32 values and vectors, 8Z + 4(N+1) + 8N per step) over the makespan, uploads "
+                                 "and host work included"},
            "rates": rows}
     if not args.no_baseline:
         out["speedup_vs_fp64_at_top_rate"] = round(top["server"]["solves_per_s"] /

@@ -393,7 +393,10 @@ int pwm_batch_schedule(const pwm_batch* b, int32_t* items_per_thread, int32_t* p
  * and stays resident, while unreferenced it may be evicted (least recently
  * used first) to admit another job.  pwm_server_step admits queued jobs in
  * arrival order, runs `chunk` steps of every running job and retires the
- * finished ones; a job's result does not depend on what else is in flight.
+ * finished ones; while chunks run it stages further queued jobs (uploads,
+ * tile plans, start vectors) on a second stream and starts another chunk with
+ * them whenever one finishes, so one call may run several chunks.  A job's
+ * result does not depend on what else is in flight.
  * Start vectors x0[i] = U[-1,1) hashed from (seed, i).  Single-threaded: calls
  * on one server must not overlap. */
 typedef struct pwm_server pwm_server;
@@ -406,7 +409,8 @@ const char* pwm_server_last_error(const pwm_server* s);
  * (FP32 servers, before any matrix is added: 1 stores each matrix whose
  * values are all exact in BF16, and every Poisson matrix, as BF16; vectors
  * stay FP32; a job's result is bit-identical to the same job on a pure BF16
- * or pure FP32 server). */
+ * or pure FP32 server), "overlap" (1, the default: stage queued jobs while
+ * chunks run; 0: stage and admit, then run one chunk). */
 int pwm_server_set_option(pwm_server* s, const char* key, int64_t value);
 /* PWM_SERVER_BORROW: keep the caller's arrays instead of a copy.  They must
  * stay alive and unchanged until the matrix is removed; they are checked
@@ -426,8 +430,8 @@ int pwm_server_step(pwm_server* s, int32_t* running, int32_t* finished);
  * their number into *count. */
 int pwm_server_finished(const pwm_server* s, int64_t* ids, int32_t cap, int32_t* count);
 /* Seconds spent so far in pwm_server_step's phases: admission (including
- * matrix uploads and tile plans), running the engine, retiring jobs; and the
- * number of matrix uploads. */
+ * matrix uploads and tile plans, also those made while chunks run), waiting
+ * for the engine's chunks, retiring jobs; and the number of matrix uploads. */
 int pwm_server_timing(const pwm_server* s, double* admit_s, double* run_s, double* retire_s, int64_t* uploads);
 /* Finished job: eigenvalue estimate, |y| of the last step, steps taken and
  * (vec non-NULL, n doubles) the normalised final iterate.  PWM_ERR_STATE

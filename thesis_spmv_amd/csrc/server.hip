@@ -5,11 +5,13 @@
 // and device memory are free (memory comes from one arena of `budget`
 // bytes; matrices no running job uses are evicted least recently used first
 // to make room), runs one chunk of steps of every running job, and retires
-// the jobs that stopped.  While a chunk runs, the queued jobs that free slots
-// can take are staged on a second stream (matrix upload or generation, tile
-// plan, job block and start vector), so uploads overlap the running jobs;
-// they join at the next chunk.  A job's arithmetic depends only on its own
-// matrix tiles, so its result is bit-identical whatever else is in flight.  With the option
+// the jobs that stopped.  While chunks run, the queued jobs that free slots
+// can take are staged one at a time on a second stream (matrix upload or
+// generation, tile plan, job block and start vector); whenever the chunk in
+// flight has finished, the next one starts with the jobs staged so far, so
+// uploads overlap the running jobs (option overlap = 0: stage, then run one
+// chunk).  A job's arithmetic depends only on its own matrix tiles, so its
+// result is bit-identical whatever else is in flight.  With the option
 // bf16_when_exact an FP32 server stores Poisson matrices and matrices whose
 // values are exact in BF16 as BF16 (the engine's tiles branch on their job's
 // storage).
@@ -418,17 +420,17 @@ extern "C" int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, dou
     return PWM_OK;
 }
 
-// Stage up to `max_jobs` queued jobs in arrival order (jobs staged before
-// count and are skipped): make each one's matrix resident and give it a job
-// block with its start vector, on the staging stream.  A staged job holds a
-// reference to its matrix, so the matrix is not evicted before the job ends.
-// Stops at the first job that does not fit; returns once the staging stream
-// has finished.
-static int stage(pwm_server* s, size_t max_jobs) {
+// Stage at most `max_new` queued jobs among the first `max_jobs` in arrival
+// order (jobs staged before count and are skipped): make each one's matrix
+// resident and give it a job block with its start vector, on the staging
+// stream.  A staged job holds a reference to its matrix, so the matrix is not
+// evicted before the job ends.  Stops at the first job that does not fit;
+// returns once the staging stream has finished.  *staged = jobs staged.
+static int stage(pwm_server* s, size_t max_jobs, size_t max_new, int* staged) {
     int rc = PWM_OK;
-    bool wrote = false;
+    *staged = 0;
     size_t k = 0;
-    for (auto it = s->queue.begin(); it != s->queue.end() && k < max_jobs; ++it, ++k) {
+    for (auto it = s->queue.begin(); it != s->queue.end() && k < max_jobs && (size_t)*staged < max_new; ++it, ++k) {
         SrvJob& J = s->jobs[*it];
         if (J.has_block) continue;
         SrvMatrix& M = s->mats[J.mat];
@@ -450,29 +452,30 @@ static int stage(pwm_server* s, size_t max_jobs) {
         }
         J.bytes = jb;
         J.has_block = true;
-        wrote = true;
+        ++*staged;
         const hipError_t e = pwmk::engine_start_vector(s->arena.at(J.off), M.n, s->vtype, J.seed, 0, s->ustream);
         if (e != hipSuccess) {
             rc = shp(s, e, "job start vector");
             break;
         }
     }
-    if (wrote) {
+    if (*staged > 0) {
         const hipError_t e = hipStreamSynchronize(s->ustream);
         if (e != hipSuccess && rc == PWM_OK) rc = shp(s, e, "job staging");
     }
     return rc;
 }
 
-// Admit queued jobs in arrival order while engine slots are free, staging
-// those not staged yet; stop at the first that does not fit.
-static int admit(pwm_server* s) {
+static size_t free_slots(const pwm_server* s) { return (size_t)std::max(0, s->eng->max_slots() - s->eng->active()); }
+
+// Hand the staged jobs at the front of the queue to the engine while slots
+// are free.  Only while no steps are in flight (Engine::add).
+static void add_staged(pwm_server* s) {
     const size_t xs = pwm_impl::vector_size(s->vtype);
-    const int rc = stage(s, (size_t)std::max(0, s->eng->max_slots() - s->eng->active()));
     while (!s->queue.empty() && s->eng->active() < s->eng->max_slots()) {
         const int64_t id = s->queue.front();
         SrvJob& J = s->jobs[id];
-        if (!J.has_block) break;  // did not fit
+        if (!J.has_block) break;
         SrvMatrix& M = s->mats[J.mat];
         char* base = (char*)s->arena.at(J.off);
         const char* mb = (const char*)s->arena.at(M.off);
@@ -501,14 +504,21 @@ static int admit(pwm_server* s) {
         s->slot_job[slot] = id;
         s->queue.pop_front();
     }
+}
+
+// Admit queued jobs in arrival order while engine slots are free, staging
+// those not staged yet; stop at the first that does not fit.
+static int admit(pwm_server* s) {
+    int n = 0;
+    const int rc = stage(s, free_slots(s), SIZE_MAX, &n);
+    add_staged(s);
     return rc;
 }
 
-// Record the finished slots' results, normalise their final iterates and
-// free their engine slots.
+// Record the finished slots' results (appended to last_finished), normalise
+// their final iterates and free their engine slots.
 static int retire(pwm_server* s, const std::vector<int>& fin) {
     hipError_t e = hipSuccess;
-    s->last_finished.clear();
     for (int slot : fin) {
         const int64_t id = s->slot_job[slot];
         s->last_finished.push_back(id);
@@ -530,56 +540,95 @@ static int retire(pwm_server* s, const std::vector<int>& fin) {
     return PWM_OK;
 }
 
-extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
-    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
+// Wait for the steps in flight (if any), retire the jobs that stopped, hand
+// the staged jobs to the engine and, with `relaunch`, start the next chunk.
+static int cycle(pwm_server* s, bool relaunch) {
     std::vector<int> fin;
     std::string err;
+    const double t0 = now_s();
+    int rc = s->eng->collect(fin, err);
+    const double t1 = now_s();
+    s->t_run += t1 - t0;
+    if (rc != PWM_OK) return sfail(s, rc, err);
+    rc = retire(s, fin);
+    s->t_retire += now_s() - t1;
+    if (rc != PWM_OK) return rc;
+    add_staged(s);
+    if (relaunch && s->eng->active() > 0 && (rc = s->eng->launch(s->chunk, err)) != PWM_OK) {
+        std::vector<int> none;
+        std::string e2;
+        (void)s->eng->collect(none, e2);  // let what was queued finish
+        return sfail(s, rc, err);
+    }
+    return PWM_OK;
+}
+
+extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished) {
+    if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_step: NULL");
+    s->last_finished.clear();
+    std::string err;
     int rc = PWM_OK;
-    if (s->eng->active() > 0) {
-        // one chunk of the running jobs; while it runs, the queued jobs that
-        // the free slots can take are staged, and they join at the next step
-        const double t0 = now_s();
-        rc = s->eng->launch(s->chunk, err);
-        double staged = 0.0;
-        int stage_rc = PWM_OK;
-        if (rc == PWM_OK && s->overlap) {
-            const double ts = now_s();
-            stage_rc = stage(s, (size_t)(s->eng->max_slots() - s->eng->active()));
-            staged = now_s() - ts;
-        }
-        const int crc = s->eng->collect(fin, err);  // waits even after a failed launch
-        if (rc == PWM_OK) rc = crc;
-        const double t1 = now_s();
-        s->t_admit += staged;
-        s->t_run += t1 - t0 - staged;
-        if (rc != PWM_OK) return sfail(s, rc, err);
-        if ((rc = retire(s, fin)) != PWM_OK) return rc;
-        const double t2 = now_s();
-        s->t_retire += t2 - t1;
-        if (stage_rc != PWM_OK) return stage_rc;
-        rc = admit(s);
-        s->t_admit += now_s() - t2;
-        if (rc != PWM_OK) return rc;
-    } else {
+    if (!s->overlap) {
+        // stage and admit, then one chunk of every running job
         const double t0 = now_s();
         rc = admit(s);
-        const double t1 = now_s();
-        s->t_admit += t1 - t0;
+        s->t_admit += now_s() - t0;
         if (rc != PWM_OK) return rc;
         if (s->eng->active() == 0 && !s->queue.empty())
             return sfail(s, PWM_ERR_BUDGET,
                          "pwm_server_step: the next job does not fit the budget with nothing running "
                          "(collect finished results to free their memory)");
-        if (s->eng->active() > 0) {
-            if ((rc = s->eng->run(s->chunk, fin, err)) != PWM_OK) return sfail(s, rc, err);
+        if (s->eng->active() > 0 && (rc = s->eng->launch(s->chunk, err)) != PWM_OK) {
+            std::vector<int> none;
+            std::string e2;
+            (void)s->eng->collect(none, e2);
+            return sfail(s, rc, err);
         }
-        const double t2 = now_s();
-        s->t_run += t2 - t1;
-        if ((rc = retire(s, fin)) != PWM_OK) return rc;
-        s->t_retire += now_s() - t2;
+        if ((rc = cycle(s, false)) != PWM_OK) return rc;
+    } else {
+        // staged jobs join now; with nothing running, the first queued job is
+        // staged at once so that the GPU has work
+        const double t0 = now_s();
+        add_staged(s);
+        if (s->eng->active() == 0 && !s->queue.empty()) {
+            int n = 0;
+            rc = stage(s, 1, 1, &n);
+            add_staged(s);
+            if (rc != PWM_OK) return rc;
+            if (s->eng->active() == 0)
+                return sfail(s, PWM_ERR_BUDGET,
+                             "pwm_server_step: the next job does not fit the budget with nothing running "
+                             "(collect finished results to free their memory)");
+        }
+        s->t_admit += now_s() - t0;
+        if (s->eng->active() > 0 && (rc = s->eng->launch(s->chunk, err)) != PWM_OK) {
+            std::vector<int> none;
+            std::string e2;
+            (void)s->eng->collect(none, e2);
+            return sfail(s, rc, err);
+        }
+        // the other queued jobs that free slots can take are staged one at a
+        // time while chunks run; whenever the chunk in flight has finished,
+        // the next one starts with the jobs staged so far
+        int stage_rc = PWM_OK;
+        for (;;) {
+            int n = 0;
+            const double ts = now_s();
+            stage_rc = stage(s, free_slots(s), 1, &n);
+            s->t_admit += now_s() - ts;
+            if (stage_rc != PWM_OK || n == 0) break;
+            bool done = !s->eng->pending();
+            if (!done) {
+                done = hipStreamQuery(s->stream) == hipSuccess;
+                (void)hipGetLastError();  // a "not ready" answer is no error for later checks
+            }
+            if (done && (rc = cycle(s, true)) != PWM_OK) return rc;
+        }
+        if ((rc = cycle(s, false)) != PWM_OK) return rc;  // the chunk in flight
+        if (stage_rc != PWM_OK) return stage_rc;
     }
     if (active) *active = s->eng->active();
-    if (finished) *finished = (int32_t)fin.size();
+    if (finished) *finished = (int32_t)s->last_finished.size();
     return PWM_OK;
 }
 
@@ -629,7 +678,8 @@ extern "C" int pwm_server_stats(const pwm_server* s, int64_t* used_bytes, int32_
 }
 
 // Seconds spent in pwm_server_step's phases so far (admission including
-// matrix uploads, running the engine, retiring jobs) and the uploads made.
+// matrix uploads, waiting for the engine's chunks, retiring jobs) and the
+// uploads made.
 extern "C" int pwm_server_timing(const pwm_server* s, double* admit_s, double* run_s, double* retire_s,
                                  int64_t* uploads) {
     if (!s) return sfail(nullptr, PWM_ERR_INVALID, "pwm_server_timing: NULL");

@@ -632,7 +632,7 @@ def bench_serve(args):
 
     def run_server(reqs):
         srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph,
-                        bf16_when_exact=args.serve_bf16_exact)
+                        bf16_when_exact=args.serve_bf16_exact, items_per_thread=args.serve_ipt)
         resident_ids = {}  # pool index -> list of server matrix ids
         lat, jobs, pending, k = [], {}, list(reqs), 0
         served_bytes = 0  # algorithmic bytes of every step served (FP32 values and vectors)
@@ -705,7 +705,7 @@ def bench_serve(args):
         power-law matrices), relative where lambda_64 is the dominant scale,
         and the max-norm difference of the normalised final vectors."""
         srv = pw.Server("f32", budget_bytes=64 << 30, max_jobs=1024, chunk=args.serve_chunk, graph=args.serve_graph,
-                        bf16_when_exact=args.serve_bf16_exact)
+                        bf16_when_exact=args.serve_bf16_exact, items_per_thread=args.serve_ipt)
         mids = {}
         jobs = {}
         for r in reqs:
@@ -926,7 +926,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=60, help="requests per arrival rate")
     ap.add_argument("--serve-pool", type=int, default=12, help="distinct matrices the requests draw from")
     ap.add_argument("--serve-max-n", type=float, default=1e7)
-    ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per pwm_server_step")
+    ap.add_argument("--serve-chunk", type=int, default=8, help="server steps per chunk")
+    ap.add_argument("--serve-ipt", type=int, default=16, choices=(4, 8, 16),
+                    help="server tile size: 256 x this many merge items")
     ap.add_argument("--serve-bf16-exact", type=int, default=0,
                     help="1: the server stores matrices whose values are exact in BF16 (Poisson) as BF16")
     ap.add_argument("--serve-graph", type=int, default=1, help="0: the server launches its steps one by one "

@@ -56,7 +56,8 @@ __global__ void k_eng_plan(const int32_t* __restrict__ rs, int64_t n, int64_t nn
     int stream = T.b0 == rs[T.a0] && T.b1 == rs[T.a1];
     for (int32_t r = T.a0; stream && r < T.a1; ++r) stream = rs[r + 1] - rs[r] <= kStreamRow;
     T.stream = stream;
-    T.pad[0] = T.pad[1] = T.pad[2] = 0;
+    T.rs_end = rs[T.a1];
+    T.pad[0] = T.pad[1] = 0;
     tiles[t] = T;
 }
 
@@ -250,8 +251,12 @@ __global__ __launch_bounds__(kEngBS) void k_eng_tiles_p(const EWork* __restrict_
 // A run of tiles t..q-1 carrying into the same row r is summed in tile
 // order and added to y[r] once; the tiles' partials are summed in a fixed
 // thread/tile order, so the result does not depend on scheduling.
+// One 1024-thread workgroup per job: a job of T tiles takes T / 4096 rounds
+// of dependent loads, so the largest jobs of a step do not hold it up.
+constexpr int kFinBS = 1024;
+
 template <typename VT>
-__global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
+__global__ __launch_bounds__(kFinBS) void k_eng_finish(EArgs a) {
     using X = typename Arith<VT>::X;
     using Acc = typename Arith<VT>::A;
     __shared__ double red[16];
@@ -267,27 +272,28 @@ __global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
     const ETile* __restrict__ tiles = J.tiles;
     const int nt = J.n_tiles;
     double yy = 0.0, xy = 0.0, xx = 0.0;
-    // Each thread takes tiles t = tid, tid + 256, ... in order, kU at a time:
-    // the loads of kU tiles (their partials, their end row and its start,
-    // the carry and the next tile's bounds) are issued together, then the
-    // tiles are summed in order.  Tile t ends in row r = a1; when the tiles
+    // Each thread takes tiles t = tid, tid + 1024, ... in order, kU at a time:
+    // the loads of kU tiles (their partials, their end row and its start from
+    // the plan, the carry and the next tile's bounds) are issued together,
+    // then the tiles are summed in order.  Tile t ends in row r = a1; when the tiles
     // after t share that end row the row spans them, and the carries of
     // tiles t, t+1, ... (in tile order) complete it in the first tile that
     // ends past it, unless that tile starts at row r's start (then tile t's
     // carry is empty and that tile counted the row).
     constexpr int kU = 4;
-    for (int t0 = threadIdx.x; t0 < nt; t0 += kU * kEngBS) {
+    for (int t0 = threadIdx.x; t0 < nt; t0 += kU * kFinBS) {
         double p0[kU], p1[kU], p2[kU];
         int32_t r[kU], rprev[kU], nx_a1[kU], nx_b0[kU], rs_r[kU];
         Acc c[kU];
 #pragma unroll
         for (int k = 0; k < kU; ++k) {
-            const int t = t0 + k * kEngBS;
+            const int t = t0 + k * kFinBS;
             if (t >= nt) break;
             p0[k] = J.partials[3 * (int64_t)t + 0];
             p1[k] = J.partials[3 * (int64_t)t + 1];
             p2[k] = J.partials[3 * (int64_t)t + 2];
             r[k] = tiles[t].a1;
+            rs_r[k] = tiles[t].rs_end;
             rprev[k] = t > 0 ? tiles[t - 1].a1 : -1;
             c[k] = cv[t];
             nx_a1[k] = t + 1 < nt ? tiles[t + 1].a1 : -1;
@@ -295,13 +301,7 @@ __global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
         }
 #pragma unroll
         for (int k = 0; k < kU; ++k) {
-            const int t = t0 + k * kEngBS;
-            if (t >= nt) break;
-            rs_r[k] = r[k] < J.n ? J.rs[r[k]] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < kU; ++k) {
-            const int t = t0 + k * kEngBS;
+            const int t = t0 + k * kFinBS;
             if (t >= nt) break;
             yy += p0[k];
             xy += p1[k];
@@ -325,9 +325,9 @@ __global__ __launch_bounds__(kEngBS) void k_eng_finish(EArgs a) {
             xx += xd * xd;
         }
     }
-    const double YY = block_sum<kEngBS>(yy, red);
-    const double XY = block_sum<kEngBS>(xy, red);
-    const double XX = block_sum<kEngBS>(xx, red);
+    const double YY = block_sum<kFinBS>(yy, red);
+    const double XY = block_sum<kFinBS>(xy, red);
+    const double XX = block_sum<kFinBS>(xx, red);
     if (threadIdx.x == 0) {
         EState& S = a.st[s];
         const double lam = XY / XX;
@@ -350,7 +350,7 @@ static hipError_t step_t(const EArgs& a, int gw, int gp, int gs, hipStream_t s) 
     else
         hipLaunchKernelGGL((k_eng_tiles<VT, ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a.work, a.jobs,
                            (const EState*)a.st, a.ctl);
-    hipLaunchKernelGGL((k_eng_finish<VT>), dim3(gs), dim3(kEngBS), 0, s, a);
+    hipLaunchKernelGGL((k_eng_finish<VT>), dim3(gs), dim3(kFinBS), 0, s, a);
     return hipGetLastError();
 }
 
@@ -388,7 +388,7 @@ template <int ITEMS>
 static hipError_t step_mixed(const EArgs& a, int gw, int gs, hipStream_t s) {
     hipLaunchKernelGGL((k_eng_tiles_mixed<ITEMS>), dim3(gw), dim3(kEngBS), 0, s, a.work, a.jobs, (const EState*)a.st,
                        a.ctl);
-    hipLaunchKernelGGL((k_eng_finish<float>), dim3(gs), dim3(kEngBS), 0, s, a);
+    hipLaunchKernelGGL((k_eng_finish<float>), dim3(gs), dim3(kFinBS), 0, s, a);
     return hipGetLastError();
 }
 

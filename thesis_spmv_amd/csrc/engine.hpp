@@ -39,7 +39,8 @@ namespace pwmk {
 struct ETile {  // rows [a0,a1), nonzeros [b0,b1), matrix-relative
     int32_t a0, a1, b0, b1;
     int32_t stream;  // 1: row-aligned, every row <= kStreamRow nonzeros
-    int32_t pad[3];
+    int32_t rs_end;  // row_start[a1]
+    int32_t pad[2];
 };
 
 // Tiles hold at most 256*ipt merge items.  The planner cuts the merge path

@@ -251,3 +251,44 @@ def test_borrowed_columns_checked_at_upload(values):
         held[len(held) // 2] = A.col_ind[len(held) // 2]
         res = _drain(srv, {j: A.nor})  # the queued job runs on the repaired arrays
         assert res[j][2] == 5
+
+
+def test_many_queued_jobs_match_alone():
+    """600 jobs submitted at once on a server with 64 slots: they queue, are
+    staged while earlier ones run, and every copy of a job gives the result
+    of that job run alone, bit for bit."""
+    rng = np.random.default_rng(21)
+    A = _powerlaw(rng, 2000)
+    combos = [("P", 7, -1.0, 1), ("A", 30, 1e-9, 2), ("Q", 3, -1.0, 3), ("P", 200, 1e-10, 4), ("A", 1, -1.0, 5),
+              ("Q", 64, -1.0, 6)]
+
+    def fresh():
+        srv = pw.Server("f32", budget_bytes=64 << 20, max_jobs=64, chunk=4)
+        ids = {"P": srv.add_poisson(40, 30), "A": srv.add_matrix(A.row_start, A.col_ind, A.data),
+               "Q": srv.add_poisson(20, 20)}
+        return srv, ids, {"P": 1200, "A": A.nor, "Q": 400}
+
+    alone = []
+    for name, L, tol, seed in combos:
+        srv, ids, n = fresh()
+        j = srv.submit(ids[name], L, tol, seed=seed)
+        alone.append(_drain(srv, {j: n[name]})[j])
+    srv, ids, n = fresh()
+    jobs, which = {}, {}
+    for copy in range(100):
+        for c, (name, L, tol, seed) in enumerate(combos):
+            j = srv.submit(ids[name], L, tol, seed=seed)
+            jobs[j] = n[name]
+            which[j] = c
+    assert srv.stats()["queued"] == 600
+    res = {}
+    for _ in range(100000):
+        srv.step()
+        for j in srv.finished():
+            res[j] = srv.result(j, jobs[j])
+        if len(res) == 600:
+            break
+    assert len(res) == 600 and srv.stats()["queued"] == 0
+    for j, (lam, yn, its, vec) in res.items():
+        ref = alone[which[j]]
+        assert its == ref[2] and lam == ref[0] and yn == ref[1] and np.array_equal(vec, ref[3])

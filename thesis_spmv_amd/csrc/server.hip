@@ -83,6 +83,7 @@ struct pwm_server {
     std::vector<SrvMatrix> mats;
     std::map<int64_t, SrvJob> jobs;
     std::deque<int64_t> queue;
+    size_t staged_front = 0;  // the first staged_front queued jobs are staged (staging goes in order)
     std::vector<int64_t> slot_job;
     std::vector<int64_t> last_finished;  // jobs retired by the last step
     int64_t next_job = 0, finished_total = 0;
@@ -429,10 +430,8 @@ extern "C" int pwm_server_submit(pwm_server* s, int32_t mat, int64_t max_it, dou
 static int stage(pwm_server* s, size_t max_jobs, size_t max_new, int* staged) {
     int rc = PWM_OK;
     *staged = 0;
-    size_t k = 0;
-    for (auto it = s->queue.begin(); it != s->queue.end() && k < max_jobs && (size_t)*staged < max_new; ++it, ++k) {
-        SrvJob& J = s->jobs[*it];
-        if (J.has_block) continue;
+    for (size_t k = s->staged_front; k < s->queue.size() && k < max_jobs && (size_t)*staged < max_new; ++k) {
+        SrvJob& J = s->jobs[s->queue[k]];
         SrvMatrix& M = s->mats[J.mat];
         if (!M.resident) {
             if (!alloc_evicting(s, M.bytes, &M.off, J.mat)) break;
@@ -452,6 +451,7 @@ static int stage(pwm_server* s, size_t max_jobs, size_t max_new, int* staged) {
         }
         J.bytes = jb;
         J.has_block = true;
+        ++s->staged_front;
         ++*staged;
         const hipError_t e = pwmk::engine_start_vector(s->arena.at(J.off), M.n, s->vtype, J.seed, 0, s->ustream);
         if (e != hipSuccess) {
@@ -503,6 +503,7 @@ static void add_staged(pwm_server* s) {
         J.state = 1;
         s->slot_job[slot] = id;
         s->queue.pop_front();
+        --s->staged_front;
     }
 }
 

@@ -158,6 +158,9 @@ int pwm_matrix_stats(const pwm_matrix* h, pwm_stats_t* out);
  * columns per window, 0 = 1 MB of x), "csb_sync_every" (swept slabs: windows
  * between workgroup barriers, 1, 2, 4 or 8, default 2), "csb_sweep_unroll"
  * (swept slabs: groups of 4 entries per lane per chunk, 1, 2 or 4, default 2),
+ * "csb_sweep_order" (swept slabs: 1, the default, stores each window's
+ * entries in column order so that the lanes of a gather instruction read
+ * consecutive columns and share x lines; 0 keeps CRS order),
  * "csb_sweep_waves" (swept slabs: slabs per CU, 1..8, default 1; above 1 the
  * workgroups run in that many waves, so matrices with more rows than one
  * LDS-sized slab per CU can be swept),

@@ -20,10 +20,10 @@ FORMATS = [pw.CSBHip, pw.BlockCOHHip]
 # automatic choice past 2^24 columns), 4 = swept slabs (persistent
 # nnz-balanced slabs walking x window by window)
 # layout options as stored: slab pieces built on the device and on the host,
-# packed and wide slab streams, swept slabs with 1 MB windows and with
-# 1024-column windows
+# packed and wide slab streams, swept slabs with 1 MB windows (runs in column
+# order, the default, and in row order) and with 1024-column windows
 CSB_VARIANTS = [{"csb_layout": 1}, {"csb_layout": 1, "csb_build": 1}, {"csb_layout": 2}, {"csb_layout": 3},
-                {"csb_layout": 4}, {"csb_layout": 4, "csb_window_bits": 10}]
+                {"csb_layout": 4}, {"csb_layout": 4, "csb_sweep_order": 0}, {"csb_layout": 4, "csb_window_bits": 10}]
 
 
 def _relerr(a, b):
@@ -375,7 +375,8 @@ def test_csb_wide_streams_past_2_24_columns():
 
 @pytest.mark.parametrize("values,tol", [("f64", 1e-13), ("f32", 2e-5), ("bf16", 2e-5)])
 @pytest.mark.parametrize("opt", [{}, {"csb_window_bits": 12}, {"csb_nt": 0}, {"csb_sweep_unroll": 1},
-                                 {"csb_sweep_unroll": 4, "csb_window_bits": 12}, {"csb_sweep_waves": 2}],
+                                 {"csb_sweep_unroll": 4, "csb_window_bits": 12}, {"csb_sweep_waves": 2},
+                                 {"csb_sweep_order": 0}],
                          ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()) or "default")
 def test_csb_swept_slabs(values, tol, opt):
     """Swept slabs on power-law (FP64/FP32) and Poisson (BF16, exact values)
@@ -416,9 +417,33 @@ def test_csb_swept_slabs_edge_shapes():
         cases.append(oracle.CRSMatrix(nor, noc, rs, ci, rng.standard_normal(int(rs[-1]))))
     for A in cases:
         for wbits in (0, 10, 11):
-            m = _load(pw.CSBHip, A, csb_layout=4, csb_window_bits=wbits)
-            x = rng.standard_normal(A.noc)
-            y = np.full(A.nor, np.nan)
-            m.mv(x, y)
-            assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13, (A.nor, A.noc, wbits)
+            for order in (1, 0):
+                m = _load(pw.CSBHip, A, csb_layout=4, csb_window_bits=wbits, csb_sweep_order=order)
+                x = rng.standard_normal(A.noc)
+                y = np.full(A.nor, np.nan)
+                m.mv(x, y)
+                assert _relerr(y, oracle.crs_mv(A, x)) <= 1e-13, (A.nor, A.noc, wbits, order)
+
+
+def test_csb_swept_padding_adds_nothing():
+    """Column-ordered runs are padded to whole blocks of 256 entries; a
+    padding entry gathers its window's first column and must add nothing.
+    No row of this matrix uses a window's first column and x is infinite
+    there, so a padding entry that added 0 * inf would turn a row into NaN."""
+    rng = np.random.default_rng(13)
+    n, wbits = 30000, 10
+    lens = rng.integers(0, 12, n)
+    cols = np.arange(n)[np.arange(n) % (1 << wbits) != 0]
+    rs = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(cols, int(L), replace=False)) for L in lens]).astype(np.int32)
+    A = oracle.CRSMatrix(n, n, rs, ci, rng.standard_normal(int(rs[-1])))
+    x = rng.standard_normal(n)
+    x[:: 1 << wbits] = np.inf
+    ref = oracle.crs_mv(A, x)
+    assert np.all(np.isfinite(ref))
+    for values, tol in (("f64", 1e-13), ("f32", 2e-5)):
+        m = _load(pw.CSBHip, A, values=values, csb_layout=4, csb_window_bits=wbits)
+        y = np.full(n, np.nan)
+        m.mv(x, y)
+        assert np.all(np.isfinite(y)) and _relerr(y, ref) <= tol, values
 

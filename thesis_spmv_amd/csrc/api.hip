@@ -743,6 +743,9 @@ extern "C" int pwm_set_option(pwm_matrix* h, const char* key, int64_t value) {
     } else if (k == "csb_sweep_unroll") {
         if (value != 1 && value != 2 && value != 4) return fail(h, PWM_ERR_INVALID, "csb_sweep_unroll must be 1, 2 or 4");
         h->csb_sweep_unroll = (int32_t)value;
+    } else if (k == "csb_sweep_order") {
+        if (value < 0 || value > 1) return fail(h, PWM_ERR_INVALID, "csb_sweep_order must be 0 (rows) or 1 (columns)");
+        h->csb_sweep_order = (int32_t)value;
     } else if (k == "csb_sweep_waves") {
         if (value < 1 || value > 8) return fail(h, PWM_ERR_INVALID, "csb_sweep_waves must be 1..8");
         h->csb_sweep_waves = (int32_t)value;

@@ -95,6 +95,12 @@ struct CsbDev {
     int64_t sweep_W = 0;
     int32_t sweep_rows = 0;      // LDS rows of a slab (the largest slab)
     DevBuf sweep_start;
+    // sweep_order 1: inside a run the entries are in column order, the run
+    // is padded to whole blocks of 256 (index 0xffffffff, value 0) and each
+    // block stores sorted entry j*64 + L at 4L + j; sweep_len holds the runs'
+    // entry counts (host)
+    int32_t sweep_order = 0;
+    std::vector<int32_t> sweep_len;
     void clear() {
         bits = 0;
         H = V = 0;
@@ -112,6 +118,8 @@ struct CsbDev {
         sweep_wb = 0;
         sweep_W = 0;
         sweep_rows = 0;
+        sweep_order = 0;
+        sweep_len.clear();
         wide = 0;
         stream = 0;
         col_bits = 0;
@@ -199,6 +207,7 @@ struct pwm_matrix {
     int32_t csb_sync_every = 2;   // swept slabs: windows between workgroup barriers
     int32_t csb_sweep_unroll = 2; // swept slabs: groups of 4 entries per lane per chunk (1, 2 or 4)
     int32_t csb_sweep_waves = 1;  // swept slabs: slabs per CU (workgroups run in that many waves)
+    int32_t csb_sweep_order = 1;  // swept slabs: 0 = rows inside a window's run, 1 = columns (lanes share x lines)
     int32_t coh_occ8 = -1;        // BlockCOH kernel at 8 waves per SIMD: -1 timed at load, 0 no, 1 yes
     int32_t csb_unroll = 0;  // stream kernel groups of 4 per thread per step (0 = default)
     int32_t csb_nt = -1;     // slab streams: non-temporal entry loads / partial stores (-1 = timed)

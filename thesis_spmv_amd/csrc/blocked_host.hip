@@ -68,13 +68,17 @@ hipError_t upload(DevBuf& d, const void* src, size_t bytes) {
 // A swept-slab layout set aside while the other layouts are built and timed.
 struct SweptSave {
     DevBuf ind, data, soff, start;
-    int32_t wb = 0, rows = 0, nt = 1;
+    int32_t wb = 0, rows = 0, nt = 1, order = 0;
     int64_t W = 0, T = 0;
+    std::vector<int32_t> len;
     void take(CsbDev& c) {
         ind.swap(c.ind);
         data.swap(c.data);
         soff.swap(c.soff);
         start.swap(c.sweep_start);
+        len.swap(c.sweep_len);
+        order = c.sweep_order;
+        c.sweep_order = 0;
         wb = c.sweep_wb;
         W = c.sweep_W;
         rows = c.sweep_rows;
@@ -90,6 +94,8 @@ struct SweptSave {
         c.data.swap(data);
         c.soff.swap(soff);
         c.sweep_start.swap(start);
+        c.sweep_len.swap(len);
+        c.sweep_order = order;
         c.sweep = 1;
         c.sweep_wb = wb;
         c.sweep_W = W;
@@ -983,32 +989,47 @@ static int csb_sweep_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t*
     const int64_t beta = (int64_t)1 << bits;
     const uint32_t wmask = (uint32_t)(((uint64_t)1 << wb) - 1);
     std::vector<int32_t> sstart(T + 1), woff(T * W + 1);
-    std::vector<uint32_t> sind(std::max<int64_t>(nnz, 1));
-    std::vector<double> sdat(std::max<int64_t>(nnz, 1));
     hipError_t e = hipMemcpy(sstart.data(), c.sweep_start.p, (T + 1) * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(woff.data(), c.soff.p, woff.size() * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && nnz) e = hipMemcpy(sind.data(), c.ind.p, nnz * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && nnz) {
+    if (e != hipSuccess) return hip_fail(h, e, "pwm_csb_layout");
+    const int64_t stored = woff[T * W];  // nnz, or with column order the padded runs
+    std::vector<uint32_t> sind(std::max<int64_t>(stored, 1));
+    std::vector<double> sdat(std::max<int64_t>(stored, 1));
+    if (stored) e = hipMemcpy(sind.data(), c.ind.p, stored * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && stored) {
         DevBuf tmp;
-        e = tmp.alloc(nnz * 8);
-        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(c.data.p, h->vtype, tmp.as<double>(), nnz, h->stream);
+        e = tmp.alloc(stored * 8);
+        if (e == hipSuccess) e = pwmk::launch_copy_values_f64(c.data.p, h->vtype, tmp.as<double>(), stored, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-        if (e == hipSuccess) e = hipMemcpy(sdat.data(), tmp.p, nnz * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(sdat.data(), tmp.p, stored * 8, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) return hip_fail(h, e, "pwm_csb_layout");
-    // decode in storage order, then count per block
-    std::vector<int64_t> blk(std::max<int64_t>(nnz, 1));
+    // entry n of the runs' sorted order (runs in slab and window order) and
+    // where it is stored: run r holds sorted entries [sorted[r], sorted[r+1])
+    const bool by_col = c.sweep_order != 0;
+    std::vector<int64_t> sorted(T * W + 1, 0);
+    for (int64_t r = 0; r < T * W; ++r)
+        sorted[r + 1] = sorted[r] + (by_col ? (int64_t)c.sweep_len[r] : (int64_t)woff[r + 1] - woff[r]);
+    if (sorted[T * W] != nnz) return fail(h, PWM_ERR_STATE, "pwm_csb_layout: swept runs do not hold nnz entries");
+    // decode in sorted order, then count per block
+    std::vector<int64_t> blk(std::max<int64_t>(nnz, 1)), at(std::max<int64_t>(nnz, 1));
     std::vector<uint32_t> lr(std::max<int64_t>(nnz, 1)), lc(std::max<int64_t>(nnz, 1));
     parallel_for(T, 1, [&](int64_t t0, int64_t t1, int) {
         for (int64_t t = t0; t < t1; ++t)
-            for (int64_t p = 0; p < W; ++p)
-                for (int64_t q = woff[t * W + p]; q < woff[t * W + p + 1]; ++q) {
+            for (int64_t p = 0; p < W; ++p) {
+                const int64_t r = t * W + p;
+                for (int64_t n = sorted[r]; n < sorted[r + 1]; ++n) {
+                    const int64_t i = n - sorted[r];
+                    const int64_t q =
+                        by_col ? woff[r] + (i & ~(int64_t)255) + 4 * (i & 63) + ((i >> 6) & 3) : woff[r] + i;
                     const int64_t row = sstart[t] + (int64_t)(sind[q] >> wb);
                     const int64_t col = (p << wb) + (int64_t)(sind[q] & wmask);
-                    blk[q] = (row / beta) * H + col / beta;
-                    lr[q] = (uint32_t)(row % beta);
-                    lc[q] = (uint32_t)(col % beta);
+                    blk[n] = (row / beta) * H + col / beta;
+                    lr[n] = (uint32_t)(row % beta);
+                    lc[n] = (uint32_t)(col % beta);
+                    at[n] = q;
                 }
+            }
     });
     std::vector<int64_t> cnt(H * V + 1, 0);
     for (int64_t q = 0; q < nnz; ++q) cnt[blk[q] + 1]++;
@@ -1027,7 +1048,7 @@ static int csb_sweep_layout(pwm_matrix* h, uint32_t* ind, double* data, int32_t*
             for (int64_t i = cnt[b]; i < cnt[b + 1]; ++i) {
                 const int64_t q = order[i];
                 ind[i] = (lr[q] << 16) | lc[q];
-                data[i] = sdat[q];
+                data[i] = sdat[at[q]];
             }
         }
     });

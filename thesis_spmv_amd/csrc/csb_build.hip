@@ -19,9 +19,15 @@
 //
 // The swept-slab layout (csb_sweep_device) cuts the rows into nnz-balanced
 // slabs on the host (one per CU), keys entry k of slab t, column c with
-// t*W + (c >> w) (W windows of 2^w columns), sorts stably (so a window's
-// entries stay in CRS order: rows ascending) and packs each index as
-// (row in slab << w) | (c mod 2^w); soff gives every (slab, window) run.
+// cell = t*W + (c >> w) (W windows of 2^w columns) and, in column order (the
+// default), cell << w | (c mod 2^w); sorts stably (equal keys keep CRS
+// order) and packs each index as (row in slab << w) | (c mod 2^w).  In
+// column order every (slab, window) run is padded to whole blocks of 256
+// entries (index 0xffffffff, value 0) and each block stores its sorted entry
+// j*64 + L at 4L + j, so that lane L, reading 4 consecutive stored entries,
+// holds sorted entries L, 64 + L, 128 + L and 192 + L: each gather
+// instruction of a wave reads 64 consecutive columns.  soff gives every
+// run's stored start.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "api_internal.hpp"
@@ -123,6 +129,9 @@ __global__ void k_csb_soff(const uint64_t* __restrict__ keys, int64_t nnz, int64
 struct SweepKeyArgs {
     int64_t nor, nnz, T, W;
     int wb;
+    int order;  // 1: the column in the window joins the key (column order inside a run)
+    const int32_t* roff = nullptr;  // order 1: sorted run starts
+    const int32_t* poff = nullptr;  // order 1: stored (padded) run starts
 };
 
 __device__ __forceinline__ int64_t last_at_or_below(const int32_t* __restrict__ a, int64_t n, int64_t v) {
@@ -143,7 +152,8 @@ __global__ void k_sweep_keys(SweepKeyArgs a, const int32_t* __restrict__ rs, con
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.nnz; k += (int64_t)gridDim.x * blockDim.x) {
         const int64_t row = last_at_or_below(rs, a.nor, k);
         const int64_t t = last_at_or_below(sstart, a.T, row);
-        keys[k] = (uint64_t)t * (uint64_t)a.W + (uint64_t)((uint32_t)ci[k] >> a.wb);
+        const uint64_t cell = (uint64_t)t * (uint64_t)a.W + (uint64_t)((uint32_t)ci[k] >> a.wb);
+        keys[k] = a.order ? (cell << a.wb) | ((uint32_t)ci[k] & (((uint32_t)1 << a.wb) - 1)) : cell;
         pos[k] = (uint32_t)k;
     }
 }
@@ -156,10 +166,20 @@ __global__ void k_sweep_gather(SweepKeyArgs a, const int32_t* __restrict__ rs, c
     const uint32_t wmask = (uint32_t)(((uint64_t)1 << a.wb) - 1);
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < a.nnz; p += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = pos[p];
-        const int64_t t = (int64_t)(keys[p] / (uint64_t)a.W);
+        const uint64_t cell = a.order ? keys[p] >> a.wb : keys[p];
+        const int64_t t = (int64_t)(cell / (uint64_t)a.W);
         const int64_t row = last_at_or_below(rs, a.nor, k);
-        sind[p] = ((uint32_t)(row - sstart[t]) << a.wb) | ((uint32_t)ci[k] & wmask);
-        sdat[p] = val[k];
+        // order 1: sorted entry i of a run goes to block i / 256 of the run's
+        // padded storage, at 4 * (i mod 64) + (i / 64 mod 4): lane L of a
+        // wave reading 4 consecutive stored entries per lane then holds
+        // sorted entries L, 64 + L, 128 + L, 192 + L of the block
+        int64_t q = p;
+        if (a.order) {
+            const int64_t i = p - a.roff[cell];
+            q = a.poff[cell] + (i & ~(int64_t)255) + 4 * (i & 63) + ((i >> 6) & 3);
+        }
+        sind[q] = ((uint32_t)(row - sstart[t]) << a.wb) | ((uint32_t)ci[k] & wmask);
+        sdat[q] = val[k];
     }
 }
 
@@ -383,25 +403,24 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
     for (int64_t t = 0; t < T; ++t) rows_max = std::max(rows_max, sstart[t + 1] - sstart[t]);
     const int64_t W = std::max<int64_t>(1, (noc + ((int64_t)1 << wb) - 1) >> wb);
     const int64_t cells = T * W;
-    const int64_t zpad = ((nnz + 3) & ~(int64_t)3) + 4;
+    const int order = h->csb_sweep_order;
     const size_t vs = value_size(h->vtype);
     hipError_t e;
-    if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess ||
-        (e = c.soff.alloc((cells + 1) * 4)) != hipSuccess ||
+    if ((e = c.soff.alloc((cells + 1) * 4)) != hipSuccess ||
         (e = c.sweep_start.alloc(sstart.size() * 4)) != hipSuccess)
         return hip_fail(h, e, "CSB sweep allocation");
-    if ((e = hipMemsetAsync(c.ind.p, 0, zpad * 4, h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(c.sweep_start.p, sstart.data(), sstart.size() * 4, hipMemcpyHostToDevice, h->stream)) !=
-            hipSuccess)
+    if ((e = hipMemcpyAsync(c.sweep_start.p, sstart.data(), sstart.size() * 4, hipMemcpyHostToDevice, h->stream)) !=
+        hipSuccess)
         return hip_fail(h, e, "CSB sweep allocation");
     c.srows.release();
     c.blk_ptr.release();
     c.work.release();
-    int32_t last = (int32_t)nnz;
+    c.sweep_order = order;
+    c.sweep_len.clear();
+    int64_t stored = nnz;  // entries stored (order 1: every run padded to whole blocks of 256)
     if (nnz > 0) {
-        const pwmk::SweepKeyArgs a{nor, nnz, T, W, wb};
-        DevBuf keys, keys_out, pos, pos_out, tmp;
+        pwmk::SweepKeyArgs a{nor, nnz, T, W, wb, order};
+        DevBuf keys, keys_out, pos, pos_out, tmp, roff;
         if ((e = keys.alloc(nnz * 8)) != hipSuccess || (e = keys_out.alloc(nnz * 8)) != hipSuccess ||
             (e = pos.alloc(nnz * 4)) != hipSuccess || (e = pos_out.alloc(nnz * 4)) != hipSuccess)
             return hip_fail(h, e, "CSB sweep build allocation");
@@ -411,6 +430,7 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep keys");
         unsigned end_bit = 1;
         while (end_bit < 64 && ((uint64_t)1 << end_bit) < (uint64_t)cells) ++end_bit;
+        if (order) end_bit = std::min(64u, end_bit + (unsigned)wb);
         size_t tb = 0;
         e = rocprim::radix_sort_pairs(nullptr, tb, keys.as<uint64_t>(), keys_out.as<uint64_t>(), pos.as<uint32_t>(),
                                       pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit, h->stream);
@@ -420,6 +440,44 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
                                           pos.as<uint32_t>(), pos_out.as<uint32_t>(), (size_t)nnz, 0u, end_bit,
                                           h->stream);
         if (e != hipSuccess) return hip_fail(h, e, "CSB sweep sort");
+        // run starts; order 1 pads every run to whole blocks of 256 entries
+        DevBuf& runs = order ? roff : c.soff;
+        if (order && (e = roff.alloc((cells + 1) * 4)) != hipSuccess) return hip_fail(h, e, "CSB sweep allocation");
+        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
+                           keys_out.as<uint64_t>(), nnz, cells, order ? wb : 0, runs.as<int32_t>());
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep runs");
+        if (order) {
+            std::vector<int32_t> r(cells + 1), pr(cells + 1);
+            if ((e = hipMemcpyAsync(r.data(), roff.p, cells * 4, hipMemcpyDeviceToHost, h->stream)) != hipSuccess ||
+                (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+                return hip_fail(h, e, "CSB sweep runs");
+            r[cells] = (int32_t)nnz;
+            c.sweep_len.resize(cells);
+            int64_t acc = 0;
+            for (int64_t i = 0; i < cells; ++i) {
+                pr[i] = (int32_t)acc;
+                c.sweep_len[i] = r[i + 1] - r[i];
+                acc += ((int64_t)c.sweep_len[i] + 255) & ~(int64_t)255;
+                if (acc > INT32_MAX - 8) return fail(h, PWM_ERR_UNSUPPORTED, "CSB swept slabs: padded runs exceed int32");
+            }
+            pr[cells] = (int32_t)acc;
+            stored = acc;
+            if ((e = hipMemcpyAsync(roff.p, r.data(), (cells + 1) * 4, hipMemcpyHostToDevice, h->stream)) !=
+                    hipSuccess ||
+                (e = hipMemcpyAsync(c.soff.p, pr.data(), (cells + 1) * 4, hipMemcpyHostToDevice, h->stream)) !=
+                    hipSuccess ||
+                (e = hipStreamSynchronize(h->stream)) != hipSuccess)
+                return hip_fail(h, e, "CSB sweep runs");
+            a.roff = roff.as<int32_t>();
+            a.poff = c.soff.as<int32_t>();
+        }
+        const int64_t zpad = ((stored + 3) & ~(int64_t)3) + 4;
+        if ((e = c.ind.alloc(zpad * 4)) != hipSuccess || (e = c.data.alloc(zpad * vs)) != hipSuccess)
+            return hip_fail(h, e, "CSB sweep allocation");
+        // padding entries: index 0xffffffff (skipped by the kernel), value 0
+        if ((e = hipMemsetAsync(c.ind.p, order ? 0xff : 0, zpad * 4, h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(c.data.p, 0, zpad * vs, h->stream)) != hipSuccess)
+            return hip_fail(h, e, "CSB sweep allocation");
         if (vs == 8)
             hipLaunchKernelGGL(pwmk::k_sweep_gather<uint64_t>, g, dim3(256), 0, h->stream, a, h->rs.as<int32_t>(),
                                h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys_out.as<uint64_t>(),
@@ -435,16 +493,19 @@ int csb_sweep_device(pwm_matrix* h, const int32_t* rs, int64_t target_slabs, int
                                h->ci.as<int32_t>(), c.sweep_start.as<int32_t>(), keys_out.as<uint64_t>(),
                                pos_out.as<uint32_t>(), h->val.as<uint16_t>(), c.ind.as<uint32_t>(),
                                c.data.as<uint16_t>());
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep gather");
-        hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream,
-                           keys_out.as<uint64_t>(), nnz, cells, 0, c.soff.as<int32_t>());
         if ((e = hipGetLastError()) == hipSuccess) e = hipStreamSynchronize(h->stream);  // before the temporaries go
         if (e != hipSuccess) return hip_fail(h, e, "CSB sweep build");
     } else {
+        if ((e = c.ind.alloc(16)) != hipSuccess || (e = c.data.alloc(4 * vs)) != hipSuccess ||
+            (e = hipMemsetAsync(c.ind.p, 0, 16, h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(c.data.p, 0, 4 * vs, h->stream)) != hipSuccess)
+            return hip_fail(h, e, "CSB sweep allocation");
         hipLaunchKernelGGL(pwmk::k_csb_soff, dim3(pwmk::grid_of(cells)), dim3(256), 0, h->stream, (const uint64_t*)nullptr,
                            (int64_t)0, cells, 0, c.soff.as<int32_t>());
         if ((e = hipGetLastError()) != hipSuccess) return hip_fail(h, e, "CSB sweep build");
+        c.sweep_len.assign(order ? cells : 0, 0);
     }
+    const int32_t last = (int32_t)stored;
     if ((e = hipMemcpy((char*)c.soff.p + cells * 4, &last, 4, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(h, e, "CSB sweep build");
     c.stream = 0;

@@ -19,7 +19,12 @@
 // (y-slice in all of the CU's LDS) and walks x in windows of 1 MB, the slab's
 // entries stored window by window.  The 32 slabs of an XCD start together and
 // sweep at about the same rate, so the XCD fetches each window about once per
-// SpMV and the gathers hit its L2; no partial sums are written.
+// SpMV and the gathers hit its L2; no partial sums are written.  Inside a
+// window the entries are stored in column order, four to a lane and
+// permuted so that the 64 lanes of one gather instruction read 64
+// consecutive entries: with about one entry per x line per slab and window,
+// a third of the gathers then share a line, and so an L2 request, with a
+// neighbouring lane.
 #include "blocked.hpp"
 #include "kernels.hpp"
 #include "merge_tile.hpp"
@@ -447,8 +452,9 @@ template <typename VT, int kSU, bool NT, typename X, typename Acc>
 __device__ __forceinline__ void sweep_add(const SweepChunk<VT, kSU, NT>& c, int32_t qb, int lane, int32_t e0, int32_t e1,
                                           const X* __restrict__ xw, uint32_t wmask, int wb, Acc* ylds) {
     // an entry outside [e0, e1) belongs to a neighbouring run (its column is
-    // in another window): it gathers the window's first column, which
-    // exists, and adds nothing
+    // in another window), and an index 0xffffffff pads a column-ordered run:
+    // such an entry gathers the window's first column, which exists, and
+    // adds nothing
     bool ok[kSU][4];
     X xv[kSU][4];
 #pragma unroll
@@ -456,7 +462,7 @@ __device__ __forceinline__ void sweep_add(const SweepChunk<VT, kSU, NT>& c, int3
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int64_t k = 4 * (int64_t)(qb + lane + u * 64) + j;
-            ok[u][j] = k >= e0 && k < e1;
+            ok[u][j] = k >= e0 && k < e1 && (uint32_t)c.w[u][j] != 0xffffffffu;
             xv[u][j] = xw[ok[u][j] ? ((uint32_t)c.w[u][j] & wmask) : 0u];
         }
 #pragma unroll

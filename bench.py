@@ -636,6 +636,7 @@ def bench_serve(args):
         resident_ids = {}  # pool index -> list of server matrix ids
         lat, jobs, pending, k = [], {}, list(reqs), 0
         served_bytes = 0  # algorithmic bytes of every step served (FP32 values and vectors)
+        reg_s = 0.0  # host time registering fresh matrix copies (validation included)
         start = time.perf_counter()
         while k < len(pending) or jobs:
             now = time.perf_counter() - start
@@ -646,7 +647,9 @@ def bench_serve(args):
                 if r["repeat"] and ids:
                     mid = ids[0]
                 else:
+                    t_reg = time.perf_counter()
                     mid = register(srv, p)
+                    reg_s += time.perf_counter() - t_reg
                     resident_ids.setdefault(r["mat"], []).append(mid)
                 jobs[srv.submit(mid, r["steps"], r["tol"], seed=r["seed"])] = r
                 k += 1
@@ -663,8 +666,9 @@ def bench_serve(args):
                 lat.append(done_at - r["t"])
         makespan = time.perf_counter() - start
         tm = srv.timing()
-        log(f"#   server phases: admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), run {tm['run_s']:.2f}s, "
-            f"retire {tm['retire_s']:.2f}s of {makespan:.2f}s; {served_bytes / 1e9:.1f} GB of steps served")
+        log(f"#   server phases: registration {reg_s:.2f}s, admit {tm['admit_s']:.2f}s ({tm['uploads']} uploads), "
+            f"run {tm['run_s']:.2f}s, retire {tm['retire_s']:.2f}s of {makespan:.2f}s; "
+            f"{served_bytes / 1e9:.1f} GB of steps served")
         del srv
         return lat, makespan, served_bytes
 

@@ -196,7 +196,8 @@ static bool stage_ready() {
 // a chunk); host threads fill one buffer while the other is in flight.  The
 // caller holds g_stage_mu and stage_ready() is true.
 template <typename Fill>
-static hipError_t staged_upload(void* dst, size_t bytes, hipStream_t s, Fill fill) {
+static hipError_t staged_upload(void* dst, size_t bytes, hipStream_t s, Fill fill,
+                                const std::function<void()>* between) {
     hipError_t e = hipSuccess;
     bool busy[2] = {false, false};
     int b = 0;
@@ -210,13 +211,15 @@ static hipError_t staged_upload(void* dst, size_t bytes, hipStream_t s, Fill fil
         e = hipMemcpyAsync((char*)dst + off, stage, n, hipMemcpyHostToDevice, s);
         if (e == hipSuccess) e = hipEventRecord(g_stage_ev[b], s);
         busy[b] = true;
+        if (between && *between) (*between)();
     }
     // the buffers are the next upload's: wait until the copies have read them
     const hipError_t e2 = hipStreamSynchronize(s);
     return e != hipSuccess ? e : e2;
 }
 
-hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s,
+                         const std::function<void()>* between) {
     if (bytes == 0) return hipSuccess;
     std::lock_guard<std::mutex> lk(g_stage_mu);
     if (!stage_ready()) {  // no pinned memory: a plain copy from pageable memory
@@ -224,12 +227,12 @@ hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s
         return e == hipSuccess ? hipStreamSynchronize(s) : e;
     }
     const char* from = (const char*)src;
-    return staged_upload(dst, bytes, s,
-                         [from](char* to, size_t a0, size_t a1) { memcpy(to, from + a0, a1 - a0); });
+    return staged_upload(
+        dst, bytes, s, [from](char* to, size_t a0, size_t a1) { memcpy(to, from + a0, a1 - a0); }, between);
 }
 
 hipError_t upload_indices_staged(void* dst, const int32_t* src, int64_t n, int64_t bound, bool* out_of_range,
-                                 hipStream_t s) {
+                                 hipStream_t s, const std::function<void()>* between) {
     std::atomic<bool> bad{false};
     *out_of_range = false;
     if (n <= 0) return hipSuccess;
@@ -253,10 +256,13 @@ hipError_t upload_indices_staged(void* dst, const int32_t* src, int64_t n, int64
         }
         // pieces start and end on whole indices (multiples of 64 KB)
         const char* from = (const char*)src;
-        e = staged_upload(dst, (size_t)n * 4, s, [from, &scan](char* to, size_t a0, size_t a1) {
-            memcpy(to, from + a0, a1 - a0);
-            scan((const int32_t*)to, (int64_t)((a1 - a0) / 4));
-        });
+        e = staged_upload(
+            dst, (size_t)n * 4, s,
+            [from, &scan](char* to, size_t a0, size_t a1) {
+                memcpy(to, from + a0, a1 - a0);
+                scan((const int32_t*)to, (int64_t)((a1 - a0) / 4));
+            },
+            between);
     }
     *out_of_range = bad;
     return e;
@@ -276,8 +282,9 @@ static inline void narrow_value(const double* v, int vt, int64_t k, char* out) {
     }
 }
 
-hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s) {
-    if (vt == PWM_VALUES_F64) return upload_staged(dst, src, (size_t)n * 8, s);
+hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s,
+                                const std::function<void()>* between) {
+    if (vt == PWM_VALUES_F64) return upload_staged(dst, src, (size_t)n * 8, s, between);
     if (n <= 0) return hipSuccess;
     const size_t es = vt == PWM_VALUES_F32 ? 4 : 2;
     std::lock_guard<std::mutex> lk(g_stage_mu);
@@ -292,9 +299,12 @@ hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt,
     }
     // pieces of a chunk start and end on whole values (chunks and pieces are
     // multiples of 256 KB, and every range below is split at multiples of es)
-    return staged_upload(dst, (size_t)n * es, s, [src, vt, es](char* to, size_t a0, size_t a1) {
-        for (size_t k = a0 / es; k < a1 / es; ++k) narrow_value(src, vt, (int64_t)k, to + (k - a0 / es) * es);
-    });
+    return staged_upload(
+        dst, (size_t)n * es, s,
+        [src, vt, es](char* to, size_t a0, size_t a1) {
+            for (size_t k = a0 / es; k < a1 / es; ++k) narrow_value(src, vt, (int64_t)k, to + (k - a0 / es) * es);
+        },
+        between);
 }
 
 hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s) {

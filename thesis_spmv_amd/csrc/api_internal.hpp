@@ -232,16 +232,20 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // Host -> device copy through pinned staging buffers (double-buffered,
 // threads fill one while the other is in flight); returns once the copy has
 // completed.  A plain copy when pinned memory cannot be allocated.
-hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
+// `between` (may be NULL) is called after each chunk's copy is queued, so
+// that a caller can do other work between the chunks of a long upload.
+hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s,
+                         const std::function<void()>* between = nullptr);
 // n FP64 host values uploaded as stored in value type vt (FP64, FP32, or
 // BF16 for values exact in BF16), narrowed by host threads into the pinned
 // buffers on the way
-hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s);
+hipError_t upload_values_staged(void* dst, const double* src, int64_t n, int vt, hipStream_t s,
+                                const std::function<void()>* between = nullptr);
 // n int32 indices uploaded as they are, each checked on the way against
 // [0, bound): *out_of_range tells whether one was not (the device copy is
 // then complete but must not be used)
 hipError_t upload_indices_staged(void* dst, const int32_t* src, int64_t n, int64_t bound, bool* out_of_range,
-                                 hipStream_t s);
+                                 hipStream_t s, const std::function<void()>* between = nullptr);
 // Device -> host the same way (after the work queued on s).
 hipError_t download_staged(void* dst, const void* src, size_t bytes, hipStream_t s);
 // CSB slab-stream layout built on the device from the device CRS arrays

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <vector>
@@ -75,6 +76,10 @@ struct pwm_server {
     hipStream_t stream = nullptr;   // the engine's steps, retirement, result downloads
     hipStream_t ustream = nullptr;  // staging: uploads, Poisson generation, tile plans, start vectors
     int overlap = 1;                // stage queued jobs while a chunk runs
+    // called between the chunks of a long upload while chunks run: starts the
+    // next chunk as soon as the one in flight has finished
+    std::function<void()> pump;
+    int pump_rc = PWM_OK;
     std::string err;
     pwm_impl::Arena arena;
     std::unique_ptr<pwm_impl::Engine> eng;
@@ -387,11 +392,13 @@ static int make_resident(pwm_server* s, int mi) {
             if (rc != PWM_OK) return rc;
         }
         bool bad_col = false;
-        e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->ustream);
-        if (e == hipSuccess) e = pwm_impl::upload_indices_staged(base + M.o_ci, M.ci_p, M.nnz, M.n, &bad_col, s->ustream);
+        const std::function<void()>* pump = s->pump ? &s->pump : nullptr;
+        e = pwm_impl::upload_staged(rs, M.rs_p, (M.n + 1) * 4, s->ustream, pump);
+        if (e == hipSuccess)
+            e = pwm_impl::upload_indices_staged(base + M.o_ci, M.ci_p, M.nnz, M.n, &bad_col, s->ustream, pump);
         if (e == hipSuccess && bad_col)
             return sfail(s, PWM_ERR_INVALID, "server: borrowed matrix: column index out of range");
-        if (e == hipSuccess) e = pwm_impl::upload_values_staged(base + M.o_val, M.val_p, M.nnz, vt, s->ustream);
+        if (e == hipSuccess) e = pwm_impl::upload_values_staged(base + M.o_val, M.val_p, M.nnz, vt, s->ustream, pump);
     }
     if (e == hipSuccess)
         e = pwmk::engine_plan_tiles(rs, M.n, M.nnz, s->ipt, (pwmk::ETile*)(base + M.o_tiles), s->ustream);
@@ -609,23 +616,38 @@ extern "C" int pwm_server_step(pwm_server* s, int32_t* active, int32_t* finished
             return sfail(s, rc, err);
         }
         // the other queued jobs that free slots can take are staged one at a
-        // time while chunks run; whenever the chunk in flight has finished,
-        // the next one starts with the jobs staged so far
+        // time while chunks run; whenever the chunk in flight has finished
+        // (checked after each staged job and between the chunks of a long
+        // upload), the next one starts with the jobs staged so far
+        auto chunk_done = [s] {
+            if (!s->eng->pending()) return true;
+            const bool done = hipStreamQuery(s->stream) == hipSuccess;
+            (void)hipGetLastError();  // a "not ready" answer is no error for later checks
+            return done;
+        };
+        s->pump_rc = PWM_OK;
+        double pumped = 0.0;  // seconds in cycle() from inside uploads (counted as run and retire time)
+        s->pump = [s, chunk_done, &pumped] {
+            if (s->pump_rc == PWM_OK && s->eng->pending() && chunk_done()) {
+                const double tp = now_s();
+                s->pump_rc = cycle(s, true);
+                pumped += now_s() - tp;
+            }
+        };
         int stage_rc = PWM_OK;
         for (;;) {
             int n = 0;
-            const double ts = now_s();
+            const double ts = now_s(), p0 = pumped;
             stage_rc = stage(s, free_slots(s), 1, &n);
-            s->t_admit += now_s() - ts;
-            if (stage_rc != PWM_OK || n == 0) break;
-            bool done = !s->eng->pending();
-            if (!done) {
-                done = hipStreamQuery(s->stream) == hipSuccess;
-                (void)hipGetLastError();  // a "not ready" answer is no error for later checks
-            }
-            if (done && (rc = cycle(s, true)) != PWM_OK) return rc;
+            s->t_admit += now_s() - ts - (pumped - p0);
+            if (stage_rc != PWM_OK || n == 0 || s->pump_rc != PWM_OK) break;
+            if (chunk_done() && (rc = cycle(s, true)) != PWM_OK) break;
         }
-        if ((rc = cycle(s, false)) != PWM_OK) return rc;  // the chunk in flight
+        s->pump = nullptr;
+        if (rc == PWM_OK) rc = s->pump_rc;
+        const int crc = cycle(s, false);  // the chunk in flight
+        if (rc == PWM_OK) rc = crc;
+        if (rc != PWM_OK) return rc;
         if (stage_rc != PWM_OK) return stage_rc;
     }
     if (active) *active = s->eng->active();

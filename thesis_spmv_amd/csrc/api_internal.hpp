@@ -233,7 +233,9 @@ int fail(pwm_matrix* h, int code, const std::string& msg);
 // threads fill one while the other is in flight); returns once the copy has
 // completed.  A plain copy when pinned memory cannot be allocated.
 // `between` (may be NULL) is called after each chunk's copy is queued, so
-// that a caller can do other work between the chunks of a long upload.
+// that a caller can do other work between the chunks of a long upload; it
+// runs with the staging buffers locked and must not start a staged
+// transfer itself.
 hipError_t upload_staged(void* dst, const void* src, size_t bytes, hipStream_t s,
                          const std::function<void()>* between = nullptr);
 // n FP64 host values uploaded as stored in value type vt (FP64, FP32, or

@@ -292,3 +292,32 @@ def test_many_queued_jobs_match_alone():
     for j, (lam, yn, its, vec) in res.items():
         ref = alone[which[j]]
         assert its == ref[2] and lam == ref[0] and yn == ref[1] and np.array_equal(vec, ref[3])
+
+
+def test_refused_upload_while_others_run():
+    """A borrowed matrix whose columns went out of range is refused while
+    another job runs (its upload is staged during that job's chunks): the
+    step reports the error, the running job goes on unharmed, and once the
+    arrays are repaired the refused job is served."""
+    A = oracle.poisson_crs(40, 30)
+    srv = pw.Server("f32", budget_bytes=64 << 20, max_jobs=4, chunk=2)
+    p = srv.add_poisson(50, 50)
+    first = srv.submit(p, 200, -1.0, seed=3)
+    srv.step()  # the first job runs
+    rs, ci, dv = A.row_start.astype(np.int32), A.col_ind.astype(np.int32), A.data.copy()
+    mat = srv.add_matrix(rs, ci, dv, borrow=True)
+    held = srv._held[mat][1]
+    good = int(held[17])
+    held[17] = A.nor + 5
+    second = srv.submit(mat, 6, -1.0, seed=4)
+    with pytest.raises(pw.PwmError, match="column index out of range") as e:
+        srv.step()
+    assert e.value.code == -1
+    held[17] = good
+    res = _drain(srv, {first: 2500, second: A.nor})
+    assert res[first][2] == 200 and res[second][2] == 6
+    ref = pw.Server("f32", budget_bytes=64 << 20, max_jobs=4, chunk=2)
+    p2 = ref.add_poisson(50, 50)
+    alone = _drain(ref, {ref.submit(p2, 200, -1.0, seed=3): 2500})
+    (lam0, yn0, it0, v0), = alone.values()
+    assert res[first][0] == lam0 and np.array_equal(res[first][3], v0)
